@@ -1,0 +1,265 @@
+// C entry points of the CPU oracle, for ctypes.  TEST INFRASTRUCTURE ONLY:
+// loaded by tests/, the smoke check and bench.py's cpu_baseline leg.
+#include <chrono>
+#include <cstdio>
+
+#include "oracle.hpp"
+
+using namespace orc;
+
+namespace {
+
+template <class S, int D>
+std::vector<Sphere<S, D>> pack(uint32_t n, const S* xyr) {
+    std::vector<Sphere<S, D>> v(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int a = 0; a < D; ++a) v[i].c[a] = xyr[(D + 1) * i + a];
+        v[i].r = xyr[(D + 1) * i + D];
+    }
+    return v;
+}
+
+// LBVH over spheres; leaf_mode 0 = centre box (alpha-shape functor),
+// 1 = sphere box (ComputeBB_Functor).
+template <class S, int D>
+int lbvh_build(uint32_t n, const S* xyr, int bits, int leaf_mode, uint64_t* codes, uint64_t* codes_sorted,
+               uint32_t* perm, int32_t* sep, uint8_t* cl, uint32_t* leaf_parent, uint32_t* internal_parent,
+               S* internal_box, S* global_box) {
+    if (n < 2) return -1;
+    auto obj = pack<S, D>(n, xyr);
+    Lbvh<S, D> t;
+    if (leaf_mode == 0)
+        build_lbvh(t, obj, bits, [](const Sphere<S, D>& s) { return centre_box(s); });
+    else
+        build_lbvh(t, obj, bits, [](const Sphere<S, D>& s) { return sphere_box(s); });
+    for (uint32_t i = 0; i < n; ++i) {
+        codes[i] = t.codes[i];
+        codes_sorted[i] = t.codes_sorted[i];
+        perm[i] = t.perm[i];
+        leaf_parent[i] = t.leaf_parent[i];
+    }
+    for (uint32_t i = 0; i + 1 < n; ++i) {
+        sep[i] = t.sep[i];
+        cl[i] = t.child_is_leaf[i];
+        internal_parent[i] = t.internal_parent[i];
+        for (int a = 0; a < D; ++a) {
+            internal_box[2 * D * i + a] = t.internal_box[i].lo[a];
+            internal_box[2 * D * i + D + a] = t.internal_box[i].hi[a];
+        }
+    }
+    for (int a = 0; a < D; ++a) {
+        global_box[a] = t.global.lo[a];
+        global_box[D + a] = t.global.hi[a];
+    }
+    return 0;
+}
+
+// Sphere-sphere collision count over a sphere-box LBVH, the query of
+// tests/geometry/lbvh_simple_collision.cpp:100-140 (per query, excluding self).
+template <class S, int D>
+uint64_t lbvh_collisions(uint32_t n, const S* xyr, int bits) {
+    auto obj = pack<S, D>(n, xyr);
+    Lbvh<S, D> t;
+    build_lbvh(t, obj, bits, [](const Sphere<S, D>& s) { return sphere_box(s); });
+    uint64_t total = 0;
+#pragma omp parallel for reduction(+ : total) schedule(dynamic, 256)
+    for (uint32_t tid = 0; tid < n; ++tid) {
+        int stack[64];
+        int ss = 0;
+        stack[ss++] = 0;
+        const Sphere<S, D> q = t.obj_m[tid];
+        uint64_t c = 0;
+        while (ss) {
+            const int cur = stack[--ss];
+            const int sepv = t.sep[cur];
+            const uint8_t cl = t.child_is_leaf[cur];
+            for (int ch = 0; ch < 2; ++ch) {
+                if (!(cl & (ch + 1))) {
+                    const Box<S, D>& b = t.internal_box[sepv + ch];
+                    bool in = true;
+                    for (int a = 0; a < D; ++a) {
+                        in &= q.c[a] - q.r < b.hi[a];
+                        in &= q.c[a] + q.r > b.lo[a];
+                    }
+                    if (in) stack[ss++] = sepv + ch;
+                } else {
+                    const Sphere<S, D>& o = t.obj_m[sepv + ch];
+                    const S rs = q.r + o.r;
+                    S d2 = 0;
+                    for (int a = 0; a < D; ++a) {
+                        const S dd = q.c[a] - o.c[a];
+                        d2 += dd * dd;
+                    }
+                    if (d2 <= rs * rs && (uint32_t)(sepv + ch) != tid) ++c;
+                }
+            }
+        }
+        total += c;
+    }
+    return total;
+}
+
+template <class S>
+void* ashape2d_run(uint32_t n, const S* xyr, int bits) {
+    auto* a = new AlphaShape2D<S>();
+    a->morton_bits = bits;
+    a->init(pack<S, 2>(n, xyr));
+    a->compute();
+    return a;
+}
+
+template <class S>
+void get2d(AlphaShape2D<S>* a, uint32_t* elem, uint32_t* row, uint32_t* neig, uint8_t* edge_bnd,
+                  uint8_t* node_bnd, uint32_t* perm, S* xy_m, uint32_t* nineig) {
+    const uint32_t n = a->n;
+    if (elem) std::memcpy(elem, a->elem.data(), sizeof(uint32_t) * 3ull * a->n_elems);
+    if (row) std::memcpy(row, a->row.data(), sizeof(uint32_t) * (n + 1));
+    if (neig) std::memcpy(neig, a->neig.data(), sizeof(uint32_t) * a->n_edges);
+    if (edge_bnd) std::memcpy(edge_bnd, a->edge_is_bnd.data(), a->n_edges);
+    if (node_bnd) std::memcpy(node_bnd, a->node_is_bnd.data(), n);
+    if (perm) std::memcpy(perm, a->lbvh.perm.data(), sizeof(uint32_t) * n);
+    if (xy_m)
+        for (uint32_t i = 0; i < n; ++i) {
+            xy_m[2 * i] = a->coords_m[i].c[0];
+            xy_m[2 * i + 1] = a->coords_m[i].c[1];
+        }
+    if (nineig)
+        for (uint32_t i = 0; i < n; ++i) nineig[i] = a->nineig[i];
+}
+}  // namespace
+
+extern "C" {
+
+int orc_lbvh2d_build_f32(uint32_t n, const float* xyr, int bits, int leaf_mode, uint64_t* codes,
+                         uint64_t* codes_sorted, uint32_t* perm, int32_t* sep, uint8_t* cl, uint32_t* lp,
+                         uint32_t* ip, float* ibox, float* gbox) {
+    return lbvh_build<float, 2>(n, xyr, bits, leaf_mode, codes, codes_sorted, perm, sep, cl, lp, ip, ibox, gbox);
+}
+int orc_lbvh2d_build_f64(uint32_t n, const double* xyr, int bits, int leaf_mode, uint64_t* codes,
+                         uint64_t* codes_sorted, uint32_t* perm, int32_t* sep, uint8_t* cl, uint32_t* lp,
+                         uint32_t* ip, double* ibox, double* gbox) {
+    return lbvh_build<double, 2>(n, xyr, bits, leaf_mode, codes, codes_sorted, perm, sep, cl, lp, ip, ibox, gbox);
+}
+int orc_lbvh3d_build_f64(uint32_t n, const double* xyr, int bits, int leaf_mode, uint64_t* codes,
+                         uint64_t* codes_sorted, uint32_t* perm, int32_t* sep, uint8_t* cl, uint32_t* lp,
+                         uint32_t* ip, double* ibox, double* gbox) {
+    return lbvh_build<double, 3>(n, xyr, bits, leaf_mode, codes, codes_sorted, perm, sep, cl, lp, ip, ibox, gbox);
+}
+uint64_t orc_lbvh2d_collisions_f32(uint32_t n, const float* xyr, int bits) {
+    return lbvh_collisions<float, 2>(n, xyr, bits);
+}
+uint64_t orc_lbvh2d_collisions_f64(uint32_t n, const double* xyr, int bits) {
+    return lbvh_collisions<double, 2>(n, xyr, bits);
+}
+
+uint64_t orc_interleave(const uint32_t* q, int dim, int bits) { return interleave(q, dim, bits); }
+
+float orc_orient2d_f32(const float* a, const float* b, const float* c) { return orient2d(a, b, c); }
+double orc_orient2d_f64(const double* a, const double* b, const double* c) { return orient2d(a, b, c); }
+int orc_orient2d_exact_f32(const float* a, const float* b, const float* c) { return exact::orient2d_sign(a, b, c); }
+int orc_orient2d_exact_f64(const double* a, const double* b, const double* c) {
+    return exact::orient2d_sign(a, b, c);
+}
+float orc_incircle_filter_f32(const float* a, const float* b, const float* c, const float* d, float* err) {
+    return incircle_filter(a, b, c, d, err);
+}
+double orc_incircle_filter_f64(const double* a, const double* b, const double* c, const double* d, double* err) {
+    return incircle_filter(a, b, c, d, err);
+}
+int orc_incircle_exact_f32(const float* a, const float* b, const float* c, const float* d) {
+    return exact::incircle_sign(a, b, c, d);
+}
+int orc_incircle_exact_f64(const double* a, const double* b, const double* c, const double* d) {
+    return exact::incircle_sign(a, b, c, d);
+}
+
+// ---- alpha-shape 2D: run, query sizes, copy outputs, free ----
+void* orc_ashape2d_run_f32(uint32_t n, const float* xyr, int bits) { return ashape2d_run<float>(n, xyr, bits); }
+void* orc_ashape2d_run_f64(uint32_t n, const double* xyr, int bits) { return ashape2d_run<double>(n, xyr, bits); }
+
+// sizes: [n, n_elems, n_edges, exact_calls]
+void orc_ashape2d_sizes(void* h, int is_f64, uint64_t* out) {
+    if (is_f64) {
+        auto* a = (AlphaShape2D<double>*)h;
+        out[0] = a->n; out[1] = a->n_elems; out[2] = a->n_edges; out[3] = a->exact_calls;
+    } else {
+        auto* a = (AlphaShape2D<float>*)h;
+        out[0] = a->n; out[1] = a->n_elems; out[2] = a->n_edges; out[3] = a->exact_calls;
+    }
+}
+
+void orc_ashape2d_get_f32(void* h, uint32_t* elem, uint32_t* row, uint32_t* neig, uint8_t* eb, uint8_t* nb,
+                          uint32_t* perm, float* xy_m, uint32_t* nineig) {
+    get2d((AlphaShape2D<float>*)h, elem, row, neig, eb, nb, perm, xy_m, nineig);
+}
+void orc_ashape2d_get_f64(void* h, uint32_t* elem, uint32_t* row, uint32_t* neig, uint8_t* eb, uint8_t* nb,
+                          uint32_t* perm, double* xy_m, uint32_t* nineig) {
+    get2d((AlphaShape2D<double>*)h, elem, row, neig, eb, nb, perm, xy_m, nineig);
+}
+void orc_ashape2d_free(void* h, int is_f64) {
+    if (is_f64)
+        delete (AlphaShape2D<double>*)h;
+    else
+        delete (AlphaShape2D<float>*)h;
+}
+
+// ---- P1 assembly: returns nnz; row/col/val/b must hold n+1 / cap / cap / n ----
+int64_t orc_assemble_p1(uint32_t n, const double* xy, uint32_t ne, const uint32_t* elem, const uint8_t* bnd,
+                        const double* f, uint64_t cap, uint32_t* row, uint32_t* col, double* val, double* b) {
+    CsrD A;
+    std::vector<double> bb;
+    assemble_p1(n, xy, ne, elem, bnd, f, A, bb);
+    if (A.col.size() > cap) return -(int64_t)A.col.size();
+    std::memcpy(row, A.row.data(), sizeof(uint32_t) * (n + 1));
+    std::memcpy(col, A.col.data(), sizeof(uint32_t) * A.col.size());
+    std::memcpy(val, A.val.data(), sizeof(double) * A.val.size());
+    std::memcpy(b, bb.data(), sizeof(double) * n);
+    return (int64_t)A.col.size();
+}
+
+// ---- FP64 Jacobi-PCG with relative stopping ----
+uint32_t orc_jacobi_pcg(uint32_t n, const uint32_t* row, const uint32_t* col, const double* val, const double* b,
+                        double* x, double rtol, uint32_t maxit, double* true_rel) {
+    CsrD A;
+    A.n = n;
+    A.row.assign(row, row + n + 1);
+    A.col.assign(col, col + row[n]);
+    A.val.assign(val, val + row[n]);
+    return jacobi_pcg(A, b, x, rtol, maxit, true_rel);
+}
+
+// ---- timed CPU step (baseline): LBVH + alpha-shape, P1 assembly, and a
+// bounded number of Jacobi-PCG iterations.  times_ms: [mesh, assembly,
+// cg_total]; returns the CG iterations run.
+uint32_t orc_step_f64(uint32_t n, const double* xyr, const double* f, uint32_t cg_iters, double* times_ms,
+                      uint64_t* sizes) {
+    using clk = std::chrono::steady_clock;
+    auto t0 = clk::now();
+    AlphaShape2D<double> a;
+    a.morton_bits = 32;
+    a.init(pack<double, 2>(n, xyr));
+    a.compute();
+    auto t1 = clk::now();
+    std::vector<double> xy(2ull * n), fm(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        xy[2 * i] = a.coords_m[i].c[0];
+        xy[2 * i + 1] = a.coords_m[i].c[1];
+        fm[i] = f[a.lbvh.perm[i]];
+    }
+    CsrD A;
+    std::vector<double> bb;
+    assemble_p1(n, xy.data(), a.n_elems, a.elem.data(), a.node_is_bnd.data(), fm.data(), A, bb);
+    auto t2 = clk::now();
+    std::vector<double> x(n, 0.0);
+    double rel = 0;
+    const uint32_t it = jacobi_pcg(A, bb.data(), x.data(), 1e-30, cg_iters, &rel);
+    auto t3 = clk::now();
+    times_ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    times_ms[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    times_ms[2] = std::chrono::duration<double, std::milli>(t3 - t2).count();
+    sizes[0] = a.n_elems;
+    sizes[1] = A.col.size();
+    return it;
+}
+
+}  // extern "C"
