@@ -1,0 +1,58 @@
+"""MI355X-native implementation of Stream's PFEM mesh-and-solve path.
+
+The compute lives in one HIP library, ``lib/libstream_amd.so`` (built by
+``python -m stream_amd.build``).  The reference's three library names
+(``libgeometry``, ``libmesh``, ``libnumerics``) are aliases of it, loaded
+here the way the reference's ``python/__init__.py:26-42`` loads its own,
+with ``numpy.ctypeslib.load_library``.
+
+Importing this package without the built library raises: there is no CPU
+fallback for any operation.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+LIBDIR = Path(__file__).resolve().parent / "lib"
+
+if not (LIBDIR / "libstream_amd.so").exists():
+    raise ImportError(
+        f"stream_amd native library missing under {LIBDIR}; build it with 'python -m stream_amd.build'"
+    )
+
+libgeo = np.ctypeslib.load_library("libgeometry", str(LIBDIR))
+libnum = np.ctypeslib.load_library("libnumerics", str(LIBDIR))
+libmesh = np.ctypeslib.load_library("libmesh", str(LIBDIR))
+lib = np.ctypeslib.load_library("libstream_amd", str(LIBDIR))
+
+lib.Stream_last_error.restype = ctypes.c_char_p
+lib.Stream_last_error.argtypes = []
+lib.Stream_device_count.restype = ctypes.c_int32
+lib.Stream_device_count.argtypes = []
+lib.Stream_synchronize.restype = ctypes.c_int32
+lib.Stream_synchronize.argtypes = []
+
+
+class StreamError(RuntimeError):
+    """A native entry point returned a non-zero status."""
+
+
+def check(status: int) -> None:
+    """Raise StreamError carrying the library's last message on failure."""
+    if status != 0:
+        msg = lib.Stream_last_error()
+        raise StreamError(f"status {status}: {msg.decode() if msg else ''}")
+
+
+def device_count() -> int:
+    return int(lib.Stream_device_count())
+
+
+def synchronize() -> None:
+    check(lib.Stream_synchronize())
+
+
+from . import geometry, mesh, numerics  # noqa: E402,F401

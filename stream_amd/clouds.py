@@ -1,0 +1,106 @@
+"""Seeded synthetic point clouds and source terms for the PFEM workloads.
+
+One portable generator, SplitMix64 mapped to doubles in [0, 1) as
+(z >> 11) * 2^-53, feeds every workload so that the GPU path, the CPU
+oracle and any other consumer see the same points.  Clouds are returned in
+input order as (coords [N, dim] float64, alpha [N] float64).
+
+Workloads (all FP64):
+  * ``jittered_square``   : 128 x 128 cell-centred grid on the unit square,
+                            h = 1/128, jitter U(-0.3h, 0.3h) per coordinate,
+                            alpha = 1.3h  -> N = 16,384
+  * ``dam_break``         : fluid column [0, 0.5] x [0, 1] as a 704 x 1407
+                            cell-centred grid of spacing h = 1/1407 with
+                            jitter +-0.3h, plus 9,472 points uniform in
+                            [0.5, 1] x [0, 0.3] (free-surface fragments),
+                            alpha = 1.2h  -> N = 1,000,000
+  * ``uniform_glibc``     : lives in the tests (needs the C library's rand)
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """n successive SplitMix64 outputs from state = seed."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed: int, n: int) -> np.ndarray:
+    """n doubles in [0, 1)."""
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def jittered_grid(nx: int, ny: int, h: float, jitter: float, seed: int, x0=0.0, y0=0.0) -> np.ndarray:
+    """Cell-centred nx x ny grid, index k = j*nx + i, each coordinate moved by
+    U(-jitter*h, jitter*h)."""
+    i = np.tile(np.arange(nx, dtype=np.float64), ny)
+    j = np.repeat(np.arange(ny, dtype=np.float64), nx)
+    u = uniform(seed, 2 * nx * ny).reshape(-1, 2)
+    x = x0 + (i + 0.5) * h + (2.0 * u[:, 0] - 1.0) * jitter * h
+    y = y0 + (j + 0.5) * h + (2.0 * u[:, 1] - 1.0) * jitter * h
+    return np.stack([x, y], axis=1)
+
+
+def jittered_square(n_side: int = 128, seed: int = 0):
+    """Unit-square jittered grid, alpha = 1.3h."""
+    h = 1.0 / n_side
+    pts = jittered_grid(n_side, n_side, h, 0.3, seed)
+    return pts, np.full(pts.shape[0], 1.3 * h)
+
+
+DAM_NX, DAM_NY, DAM_SCATTER = 704, 1407, 9472
+
+
+def dam_break(nx: int = DAM_NX, ny: int = DAM_NY, n_scatter: int = DAM_SCATTER, seed: int = 1):
+    """Dam-break-like cloud: jittered fluid column plus scattered fragments,
+    alpha = 1.2h.  Defaults give exactly 1,000,000 points."""
+    h = 1.0 / ny
+    col = jittered_grid(nx, ny, h, 0.3, seed)
+    u = uniform(seed + 0x5EED, 2 * n_scatter).reshape(-1, 2)
+    frag = np.stack([0.5 + 0.5 * u[:, 0], 0.3 * u[:, 1]], axis=1)
+    pts = np.concatenate([col, frag], axis=0)
+    return pts, np.full(pts.shape[0], 1.2 * h)
+
+
+def dam_break_scaled(n_target: int, seed: int = 1):
+    """Smaller dam-break cloud with the same aspect and scatter fraction."""
+    s = np.sqrt(n_target / 1_000_000.0)
+    nx = max(4, int(round(DAM_NX * s)))
+    ny = max(8, int(round(DAM_NY * s)))
+    ns = max(1, int(round(DAM_SCATTER * s * s)))
+    h = 1.0 / ny
+    col = jittered_grid(nx, ny, h, 0.3, seed)
+    u = uniform(seed + 0x5EED, 2 * ns).reshape(-1, 2)
+    frag = np.stack([0.5 + 0.5 * u[:, 0], 0.3 * u[:, 1]], axis=1)
+    pts = np.concatenate([col, frag], axis=0)
+    return pts, np.full(pts.shape[0], 1.2 * h)
+
+
+def fourier_modes(seed: int = 11, n_modes: int = 8):
+    """Random modes u = sum_m a_m sin(k_m . x + phi_m), |a_m| = 1, integer
+    wave numbers in [1, 4] (times pi) per axis."""
+    u = uniform(seed, 4 * n_modes).reshape(n_modes, 4)
+    kx = np.pi * (1.0 + np.floor(4.0 * u[:, 0]))
+    ky = np.pi * (1.0 + np.floor(4.0 * u[:, 1]))
+    phi = 2.0 * np.pi * u[:, 2]
+    th = 2.0 * np.pi * u[:, 3]
+    return np.stack([kx, ky, phi, np.cos(th), np.sin(th)], axis=1)
+
+
+def divergence_source(xy: np.ndarray, modes: np.ndarray) -> np.ndarray:
+    """f = div u = sum_m (a_m . k_m) cos(k_m . x + phi_m) at the points."""
+    f = np.zeros(xy.shape[0])
+    for kx, ky, phi, ax, ay in modes:
+        f += (ax * kx + ay * ky) * np.cos(kx * xy[:, 0] + ky * xy[:, 1] + phi)
+    return f

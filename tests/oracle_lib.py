@@ -1,0 +1,173 @@
+"""ctypes loader of the CPU oracle (oracle/lib/liboracle.so) and of the
+reference-header harness (oracle/_ref/libref_geom.so).  Test
+infrastructure only."""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "lib", "liboracle.so")
+REF_SO = os.path.join(ORACLE_DIR, "_ref", "libref_geom.so")
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-C", ORACLE_DIR], check=True, capture_output=True)
+        _lib = ctypes.CDLL(ORACLE_SO)
+        _declare(_lib)
+    return _lib
+
+
+def ref():
+    """The reference's own predicates/Morton headers compiled (None if absent)."""
+    global _ref
+    if _ref is None and os.path.exists(REF_SO):
+        _ref = ctypes.CDLL(REF_SO)
+        P = ctypes.POINTER
+        f = ctypes.c_float
+        for name in ("ref_orient2d", "ref_orient2dexact"):
+            getattr(_ref, name).restype = f
+            getattr(_ref, name).argtypes = [P(f)] * 3
+        _ref.ref_incircle.restype = f
+        _ref.ref_incircle.argtypes = [P(f)] * 5
+        _ref.ref_incircleexact.restype = f
+        _ref.ref_incircleexact.argtypes = [P(f)] * 4
+        _ref.ref_encode2d.restype = ctypes.c_uint64
+        _ref.ref_encode2d.argtypes = [ctypes.c_uint32] * 2
+        _ref.ref_encode3d.restype = ctypes.c_uint64
+        _ref.ref_encode3d.argtypes = [ctypes.c_uint32] * 3
+    return _ref
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _declare(L):
+    vp, u32, u64, i32, f64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_double
+    for tag in ("f32", "f64"):
+        getattr(L, f"orc_lbvh2d_build_{tag}").restype = i32
+        getattr(L, f"orc_ashape2d_run_{tag}").restype = vp
+        getattr(L, f"orc_ashape2d_run_{tag}").argtypes = [u32, vp, i32]
+        getattr(L, f"orc_lbvh2d_collisions_{tag}").restype = u64
+        getattr(L, f"orc_lbvh2d_collisions_{tag}").argtypes = [u32, vp, i32]
+    L.orc_lbvh3d_build_f64.restype = i32
+    L.orc_interleave.restype = u64
+    L.orc_interleave.argtypes = [vp, i32, i32]
+    L.orc_orient2d_f32.restype = ctypes.c_float
+    L.orc_orient2d_f64.restype = f64
+    L.orc_orient2d_exact_f32.restype = i32
+    L.orc_orient2d_exact_f64.restype = i32
+    L.orc_incircle_filter_f32.restype = ctypes.c_float
+    L.orc_incircle_filter_f64.restype = f64
+    L.orc_incircle_exact_f32.restype = i32
+    L.orc_incircle_exact_f64.restype = i32
+    L.orc_ashape2d_sizes.argtypes = [vp, i32, vp]
+    L.orc_ashape2d_free.argtypes = [vp, i32]
+    L.orc_assemble_p1.restype = ctypes.c_int64
+    L.orc_assemble_p1.argtypes = [u32, vp, u32, vp, vp, vp, u64, vp, vp, vp, vp]
+    L.orc_jacobi_pcg.restype = u32
+    L.orc_jacobi_pcg.argtypes = [u32, vp, vp, vp, vp, vp, f64, u32, vp]
+    L.orc_step_f64.restype = u32
+    L.orc_step_f64.argtypes = [u32, vp, vp, u32, vp, vp]
+
+
+def spheres(coords, alpha, dtype):
+    c = np.asarray(coords, dtype)
+    out = np.empty((c.shape[0], c.shape[1] + 1), dtype)
+    out[:, :-1] = c
+    out[:, -1] = alpha
+    return np.ascontiguousarray(out)
+
+
+def lbvh2d(coords, radii, bits=32, leaf_mode=0, dtype=np.float64):
+    L = lib()
+    xyr = spheres(coords, radii, dtype)
+    n = xyr.shape[0]
+    r = dict(codes=np.empty(n, np.uint64), codes_sorted=np.empty(n, np.uint64), perm=np.empty(n, np.uint32),
+             sep=np.empty(n - 1, np.int32), child_is_leaf=np.empty(n - 1, np.uint8),
+             leaf_parent=np.empty(n, np.uint32), internal_parent=np.empty(n - 1, np.uint32),
+             internal_box=np.empty((n - 1, 4), dtype), global_box=np.empty(4, dtype))
+    fn = L.orc_lbvh2d_build_f64 if dtype == np.float64 else L.orc_lbvh2d_build_f32
+    rc = fn(ctypes.c_uint32(n), _p(xyr), ctypes.c_int32(bits), ctypes.c_int32(leaf_mode), _p(r["codes"]),
+            _p(r["codes_sorted"]), _p(r["perm"]), _p(r["sep"]), _p(r["child_is_leaf"]), _p(r["leaf_parent"]),
+            _p(r["internal_parent"]), _p(r["internal_box"]), _p(r["global_box"]))
+    assert rc == 0
+    return r
+
+
+def lbvh2d_collisions(coords, radii, bits=32, dtype=np.float64):
+    L = lib()
+    xyr = spheres(coords, radii, dtype)
+    fn = L.orc_lbvh2d_collisions_f64 if dtype == np.float64 else L.orc_lbvh2d_collisions_f32
+    return int(fn(xyr.shape[0], _p(xyr), bits))
+
+
+def ashape2d(coords, alpha, bits=32, dtype=np.float64):
+    """Run the oracle alpha-shape; returns a dict of outputs."""
+    L = lib()
+    xyr = spheres(coords, alpha, dtype)
+    n = xyr.shape[0]
+    is64 = dtype == np.float64
+    h = (L.orc_ashape2d_run_f64 if is64 else L.orc_ashape2d_run_f32)(n, _p(xyr), bits)
+    sz = np.zeros(4, np.uint64)
+    L.orc_ashape2d_sizes(h, int(is64), _p(sz))
+    ne, nedge = int(sz[1]), int(sz[2])
+    r = dict(elem=np.empty((ne, 3), np.uint32), row=np.empty(n + 1, np.uint32), neig=np.empty(nedge, np.uint32),
+             edge_is_bnd=np.empty(nedge, np.uint8), node_is_bnd=np.empty(n, np.uint8), perm=np.empty(n, np.uint32),
+             ordered=np.empty((n, 2), dtype), nineig=np.empty(n, np.uint32), n_exact_calls=int(sz[3]))
+    (L.orc_ashape2d_get_f64 if is64 else L.orc_ashape2d_get_f32)(
+        ctypes.c_void_p(h), _p(r["elem"]), _p(r["row"]), _p(r["neig"]), _p(r["edge_is_bnd"]), _p(r["node_is_bnd"]),
+        _p(r["perm"]), _p(r["ordered"]), _p(r["nineig"]))
+    L.orc_ashape2d_free(h, int(is64))
+    return r
+
+
+def assemble_p1(xy, elem, bnd, f):
+    L = lib()
+    n = xy.shape[0]
+    xy = np.ascontiguousarray(xy, np.float64)
+    elem = np.ascontiguousarray(elem, np.uint32)
+    bnd = np.ascontiguousarray(bnd, np.uint8)
+    f = np.ascontiguousarray(f, np.float64)
+    cap = n + 6 * elem.shape[0] + 16
+    row = np.empty(n + 1, np.uint32)
+    col = np.empty(cap, np.uint32)
+    val = np.empty(cap, np.float64)
+    b = np.empty(n, np.float64)
+    nnz = L.orc_assemble_p1(n, _p(xy), elem.shape[0], _p(elem), _p(bnd), _p(f), cap, _p(row), _p(col), _p(val), _p(b))
+    assert nnz >= 0
+    return row, col[:nnz].copy(), val[:nnz].copy(), b
+
+
+def jacobi_pcg(row, col, val, b, x0=None, rtol=1e-8, maxit=100000):
+    L = lib()
+    n = b.shape[0]
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    rel = np.zeros(1)
+    row = np.ascontiguousarray(row, np.uint32)
+    col = np.ascontiguousarray(col, np.uint32)
+    val = np.ascontiguousarray(val, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    it = L.orc_jacobi_pcg(n, _p(row), _p(col), _p(val), _p(b), _p(x), rtol, maxit, _p(rel))
+    return x, int(it), float(rel[0])
+
+
+def glibc_uniform(n, dim, seed=42):
+    """Points as testcases/IMR26/uniform2D.cpp:130-141 draws them:
+    srand(seed); (float)rand()/RAND_MAX per coordinate, x then y."""
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(ctypes.c_uint(seed))
+    rnd = libc.rand
+    r = np.fromiter((rnd() for _ in range(n * dim)), dtype=np.int64, count=n * dim)
+    return (r.astype(np.float32) / np.float32(2147483647)).reshape(n, dim)

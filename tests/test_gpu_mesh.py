@@ -1,0 +1,199 @@
+"""GPU parity: scan, radix sort, LBVH and the 2D alpha-shape against the
+CPU oracle, bit for bit (integer/index outputs and Morton-ordered
+coordinates)."""
+from __future__ import annotations
+
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from stream_amd import clouds
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _lib():
+    import stream_amd
+
+    return stream_amd
+
+
+# ---------------------------------------------------------------------------
+# primitives
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [0, 1, 2, 2047, 2048, 2049, 100_003, 1_000_000])
+def test_exclusive_scan(n):
+    S = _lib()
+    _say(f"scan n={n}")
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 300, n).astype(np.uint32)
+    out = np.empty(n + 1, np.uint32)
+    fn = S.lib.Stream_exclusive_scan_u32
+    fn.restype = ctypes.c_int32
+    fn.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]
+    S.check(fn(n, a.ctypes.data, out.ctypes.data))
+    exp = np.zeros(n + 1, np.uint64)
+    exp[1:] = np.cumsum(a, dtype=np.uint64)
+    assert np.array_equal(out, exp.astype(np.uint32))
+    b = rng.integers(0, 256, n).astype(np.uint8)
+    fn8 = S.lib.Stream_exclusive_scan_u8
+    fn8.restype = ctypes.c_int32
+    fn8.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]
+    S.check(fn8(n, b.ctypes.data, out.ctypes.data))
+    exp[1:] = np.cumsum(b, dtype=np.uint64)
+    assert np.array_equal(out, exp.astype(np.uint32))
+
+
+@pytest.mark.parametrize("n,bits,dups", [(1, 64, False), (2, 64, False), (2047, 64, False), (2048, 32, True),
+                                         (2049, 64, True), (300_000, 64, True), (1_000_000, 64, False),
+                                         (65_537, 16, True)])
+def test_radix_sort_pairs_stable(n, bits, dups):
+    S = _lib()
+    _say(f"sort n={n} bits={bits} dups={dups}")
+    rng = np.random.default_rng(n + bits)
+    if dups:
+        keys = rng.integers(0, 1000, n).astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    else:
+        keys = rng.integers(0, 2 ** 63, n, dtype=np.int64).astype(np.uint64) * np.uint64(2) + np.uint64(
+            rng.integers(0, 2))
+    if bits < 64:
+        keys &= np.uint64((1 << bits) - 1)
+    vals = np.arange(n, dtype=np.uint32)
+    ko = np.empty(n, np.uint64)
+    vo = np.empty(n, np.uint32)
+    fn = S.lib.Stream_sort_pairs_u64
+    fn.restype = ctypes.c_int32
+    fn.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]
+    S.check(fn(n, keys.ctypes.data, vals.ctypes.data, ko.ctypes.data, vo.ctypes.data, bits))
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(ko, keys[order])
+    assert np.array_equal(vo, order.astype(np.uint32))
+
+
+# ---------------------------------------------------------------------------
+# LBVH
+# ---------------------------------------------------------------------------
+def _lbvh_cases():
+    yield "square128_b15", clouds.jittered_square(128, seed=0), 15, np.float64
+    yield "square128_b32_f32", clouds.jittered_square(128, seed=0), 32, np.float32
+    rng = np.random.default_rng(3)
+    p = rng.uniform(-2, 3, (50_001, 2))
+    yield "uniform50k", (p, np.full(p.shape[0], 0.01)), 32, np.float64
+    yield "dam100k", clouds.dam_break_scaled(100_000), 32, np.float64
+
+
+@pytest.mark.parametrize("case", list(_lbvh_cases()), ids=lambda c: c[0])
+def test_lbvh_matches_oracle(case):
+    from stream_amd import geometry
+
+    name, (pts, al), bits, dt = case
+    _say(f"lbvh {name}")
+    g = geometry.build(pts, al, bits=bits, leaf_mode=0, dtype=dt)
+    o = O.lbvh2d(pts, al, bits=bits, leaf_mode=0, dtype=dt)
+    codes = geometry.morton_codes(pts, al, bits=bits, dtype=dt)
+    assert np.array_equal(codes, o["codes"])
+    for k in ("codes_sorted", "perm", "sep", "child_is_leaf", "leaf_parent", "internal_parent"):
+        assert np.array_equal(g[k], o[k]), k
+    assert np.array_equal(g["internal_box"], o["internal_box"])
+    assert np.array_equal(g["global_box"], o["global_box"])
+
+
+def test_lbvh_line_of_spheres_kat():
+    from stream_amd import geometry
+
+    _say("lbvh line KAT")
+    for dt in (np.float32, np.float64):
+        n = 100
+        pts = np.stack([np.arange(n), np.zeros(n)], axis=1).astype(dt)
+        rad = np.full(n, 0.55, dt)
+        g = geometry.build(pts, rad, leaf_mode=geometry.LEAF_SPHERE, dtype=dt)
+        root = g["internal_box"][0]
+        assert root[0] == dt(-0.55) and root[2] == dt(n - 1) + dt(0.55)
+        assert root[1] == dt(-0.55) and root[3] == dt(0.55)
+        assert geometry.collisions(pts, rad, dtype=dt) == 2 * (n - 2) + 2
+
+
+# ---------------------------------------------------------------------------
+# 2D alpha-shape
+# ---------------------------------------------------------------------------
+def _run_gpu_ashape(pts, al, bits, dt):
+    from stream_amd.mesh import AlphaShape2D
+
+    a = AlphaShape2D(dtype=dt, morton_bits=bits)
+    a.set_nodes(pts, al)
+    a.compute()
+    row, neig, eb = a.get_edges()
+    return dict(elem=a.get_elem(), row=row, neig=neig, edge_is_bnd=eb, node_is_bnd=a.get_boundary_nodes(),
+                perm=a.get_permutation(), ordered=a.get_ordered_nodes(), nexact=a.get_nexact())
+
+
+def _compare(g, o):
+    assert g["elem"].shape == o["elem"].shape
+    for k in ("perm", "ordered", "node_is_bnd", "row", "neig", "edge_is_bnd", "elem"):
+        assert np.array_equal(g[k], o[k]), k
+
+
+def _ashape_cases():
+    yield "square128_b15_f64", clouds.jittered_square(128, seed=0), 15, np.float64
+    yield "dam100k_f64", clouds.dam_break_scaled(100_000), 32, np.float64
+    rng = np.random.default_rng(9)
+    p = rng.uniform(0, 1, (20_000, 2))
+    yield "uniform20k_varalpha_f64", (p, rng.uniform(0.005, 0.02, p.shape[0])), 32, np.float64
+    h = 1.0 / 40
+    g = (np.stack(np.meshgrid(np.arange(40), np.arange(40), indexing="ij"), -1).reshape(-1, 2) + 0.5) * h
+    yield "grid40_cocircular_f64", (g, np.full(g.shape[0], 1.3 * h)), 32, np.float64
+    yield "grid40_cocircular_f32", (g.astype(np.float32), np.full(g.shape[0], 1.3 * h, np.float32)), 32, np.float32
+
+
+@pytest.mark.parametrize("case", list(_ashape_cases()), ids=lambda c: c[0])
+def test_ashape2d_matches_oracle(case):
+    name, (pts, al), bits, dt = case
+    _say(f"ashape2d {name}")
+    g = _run_gpu_ashape(pts, al, bits, dt)
+    o = O.ashape2d(pts, al, bits=bits, dtype=dt)
+    _compare(g, o)
+    if name.startswith("grid40"):
+        assert g["nexact"] > 0
+
+
+def test_ashape2d_readme_kat_fp32_1m():
+    # testcases/IMR26/README.md:49-54: 1,877,465 triangles; full outputs
+    # compared with the oracle
+    _say("ashape2d README KAT 1M fp32")
+    pts = O.glibc_uniform(1_000_000, 2, 42)
+    al = np.full(pts.shape[0], 0.0012, np.float32)
+    g = _run_gpu_ashape(pts, al, 32, np.float32)
+    assert g["elem"].shape[0] == 1_877_465
+    o = O.ashape2d(pts, al, bits=32, dtype=np.float32)
+    _compare(g, o)
+
+
+def test_ashape2d_dam_break_1m_fp64():
+    _say("ashape2d dam-break 1M fp64")
+    pts, al = clouds.dam_break()
+    g = _run_gpu_ashape(pts, al, 32, np.float64)
+    o = O.ashape2d(pts, al, bits=32, dtype=np.float64)
+    _compare(g, o)
+
+
+def test_ashape2d_repeat_is_deterministic():
+    from stream_amd.mesh import AlphaShape2D
+
+    _say("ashape2d repeat")
+    pts, al = clouds.dam_break_scaled(200_000)
+    a = AlphaShape2D(dtype=np.float64)
+    a.set_nodes(pts, al)
+    a.compute()
+    e1 = a.get_elem()
+    a.init()
+    a.compute()
+    e2 = a.get_elem()
+    assert np.array_equal(e1, e2)
