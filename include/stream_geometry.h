@@ -1,0 +1,56 @@
+/*
+ * Geometry and runtime C-ABI of the MI355X build (libgeometry.so ->
+ * libstream_amd.so).
+ *
+ * The reference's libgeometry exports no C symbol (src/geometry/lbvh.cpp:
+ * 22-27 only instantiates templates); these entry points expose the GPU
+ * LBVH of include/geometry/lbvh.hpp:61-362 (Morton codes, stable sort,
+ * Karras topology, refit) and the device primitives that replace AVA's
+ * sort/scan (include/geometry/lbvh.hpp:217-235,
+ * src/mesh/DirectAlphaShape2D.cpp:228-281) for tests and tools.  Host
+ * arrays in and out; 0 = success.
+ */
+#ifndef STREAM_AMD_GEOMETRY_H
+#define STREAM_AMD_GEOMETRY_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* Stream_last_error(void);
+int32_t Stream_device_count(void);
+int32_t Stream_synchronize(void);
+
+/* xyr: rows (x, y, r).  Outputs may be NULL.  ibox rows (xmin, ymin, xmax,
+ * ymax) per internal node; gbox the global sphere box.  leaf_mode 0 boxes
+ * centres (alpha-shape functor, DirectAlphaShape2D.hpp:87-99), 1 boxes
+ * spheres (ComputeBB_Functor, primitives.hpp:307-322). */
+int32_t LBVH2D_build_f32(uint32_t n, const float* xyr, int32_t bits, int32_t leaf_mode, uint64_t* codes_sorted,
+                         uint32_t* perm, int32_t* sep, uint8_t* child_is_leaf, uint32_t* leaf_parent,
+                         uint32_t* internal_parent, float* ibox, float* gbox);
+int32_t LBVH2D_build_f64(uint32_t n, const double* xyr, int32_t bits, int32_t leaf_mode, uint64_t* codes_sorted,
+                         uint32_t* perm, int32_t* sep, uint8_t* child_is_leaf, uint32_t* leaf_parent,
+                         uint32_t* internal_parent, double* ibox, double* gbox);
+int32_t LBVH3D_build_f64(uint32_t n, const double* xyzr, int32_t bits, int32_t leaf_mode, uint64_t* codes_sorted,
+                         uint32_t* perm, int32_t* sep, uint8_t* child_is_leaf, uint32_t* leaf_parent,
+                         uint32_t* internal_parent, double* ibox, double* gbox);
+/* unsorted Morton codes, lbvh.hpp:178-213 */
+int32_t LBVH2D_morton_f32(uint32_t n, const float* xyr, int32_t bits, uint64_t* codes);
+int32_t LBVH2D_morton_f64(uint32_t n, const double* xyr, int32_t bits, uint64_t* codes);
+int32_t LBVH3D_morton_f64(uint32_t n, const double* xyzr, int32_t bits, uint64_t* codes);
+/* sphere-sphere overlaps (self excluded), tests/geometry/lbvh_simple_collision.cpp:100-140 */
+int32_t LBVH2D_collisions_f32(uint32_t n, const float* xyr, int32_t bits, uint64_t* total);
+int32_t LBVH2D_collisions_f64(uint32_t n, const double* xyr, int32_t bits, uint64_t* total);
+
+/* stable onesweep radix sort of (u64, u32) pairs over the low `bits` bits */
+int32_t Stream_sort_pairs_u64(uint32_t n, const uint64_t* keys, const uint32_t* vals, uint64_t* keys_out,
+                              uint32_t* vals_out, int32_t bits);
+/* exclusive scans, out [n+1] with out[n] = total */
+int32_t Stream_exclusive_scan_u32(uint32_t n, const uint32_t* in, uint32_t* out);
+int32_t Stream_exclusive_scan_u8(uint32_t n, const uint8_t* in, uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

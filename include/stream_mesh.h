@@ -1,0 +1,63 @@
+/*
+ * Mesh C-ABI of the MI355X build (libmesh.so -> libstream_amd.so).
+ *
+ * The first block is the reference's alpha-shape interface, same names,
+ * argument meaning and fp_tt = float signatures; each line cites the
+ * reference declaration it replaces.  The second block adds FP64 nodes and
+ * the reference's C++-only queries.  Status-returning extensions return 0
+ * on success; the message of a failure is in Stream_last_error().  The
+ * reference's void entry points record their status on the handle
+ * (AlphaShape2D_last_status).
+ */
+#ifndef STREAM_AMD_MESH_H
+#define STREAM_AMD_MESH_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct AlphaShape2D AlphaShape2D;
+
+/* include/mesh/DirectAlphaShape2D.hpp:48 */
+AlphaShape2D* AlphaShape2D_create(void);
+/* include/mesh/DirectAlphaShape2D.hpp:49 */
+void AlphaShape2D_destroy(AlphaShape2D* ashape);
+/* include/mesh/DirectAlphaShape2D.hpp:53  coords row-major [2N], alpha [N] */
+void AlphaShape2D_set_nodes(AlphaShape2D* ashape, uint32_t nnodes, const float* coords, const float* alpha);
+/* include/mesh/DirectAlphaShape2D.hpp:56  LBVH + infinity nodes + allocation */
+void AlphaShape2D_init(AlphaShape2D* ashape);
+/* include/mesh/DirectAlphaShape2D.hpp:59  compute + compress */
+void AlphaShape2D_compute(AlphaShape2D* ashape);
+/* include/mesh/DirectAlphaShape2D.hpp:62 */
+uint32_t AlphaShape2D_get_nelem(const AlphaShape2D* ashape);
+/* include/mesh/DirectAlphaShape2D.hpp:65  elems [3*nelem], Morton ids */
+void AlphaShape2D_get_elem(const AlphaShape2D* ashape, uint32_t* elems);
+/* include/mesh/DirectAlphaShape2D.hpp:67  nodes [2N], Morton order */
+void AlphaShape2D_get_ordered_nodes(AlphaShape2D* ashape, float* nodes);
+
+/* ---- extensions ---- */
+/* Morton bits per axis (1..32, default 32 as lbvh.hpp:186) for the next init. */
+int32_t AlphaShape2D_set_morton_bits(AlphaShape2D* ashape, int32_t bits);
+/* FP64 nodes and exact FP64 predicates: coords [2N], alpha [N]. */
+int32_t AlphaShape2D_set_nodes_f64(AlphaShape2D* ashape, uint32_t nnodes, const double* coords, const double* alpha);
+/* FP64 nodes already in device memory: rows (x, y, unused, alpha). */
+int32_t AlphaShape2D_set_nodes_device_f64(AlphaShape2D* ashape, uint32_t nnodes, const double* d_xypr);
+int32_t AlphaShape2D_get_ordered_nodes_f64(AlphaShape2D* ashape, double* nodes);
+/* number of directed edges; AlphaShape2D::getEdge, src/mesh/DirectAlphaShape2D.cpp:90-98 */
+uint32_t AlphaShape2D_get_nedge(const AlphaShape2D* ashape);
+/* edge CSR row [N+1], neig [nedge], edge_is_bnd [nedge], node_is_bnd [N];
+ * getEdge / getBoundaryNodes, DirectAlphaShape2D.cpp:90-105.  Any may be NULL. */
+int32_t AlphaShape2D_get_edges(const AlphaShape2D* ashape, uint32_t* row, uint32_t* neig, uint8_t* edge_is_bnd,
+                               uint8_t* node_is_bnd);
+/* perm[i] = input index of Morton slot i; getPermutation, DirectAlphaShape2D.cpp:75-80 */
+int32_t AlphaShape2D_get_permutation(const AlphaShape2D* ashape, uint32_t* perm);
+/* nodes whose local triangulation needed the exact predicate stage */
+uint32_t AlphaShape2D_get_nexact(const AlphaShape2D* ashape);
+/* status of the last call on the handle (0 = ok) */
+int32_t AlphaShape2D_last_status(const AlphaShape2D* ashape);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,85 @@
+/*
+ * Numerics C-ABI of the MI355X build (libnumerics.so -> libstream_amd.so).
+ *
+ * First block: the reference's CSR / linear-system / Jacobi / CG interface
+ * with its fp_tt = float signatures and semantics (SolverCG_* stop on an
+ * absolute 1e-6 on sqrt(r.z) or sqrt(r.r), at most 10000 iterations, and
+ * return the reference's post-incremented count).  Second block: FP64
+ * values, the FP64 relative-residual and mixed-precision solvers, and the
+ * mesh-and-solve step, which has no reference counterpart.
+ */
+#ifndef STREAM_AMD_NUMERICS_H
+#define STREAM_AMD_NUMERICS_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct d_CSR d_CSR;
+typedef struct h_CSR h_CSR;
+typedef struct LinSys LinSys;
+typedef struct PrecJacobi_st PrecJacobi_st;
+typedef struct SolverCG SolverCG;
+typedef struct PfemStep2D PfemStep2D;
+
+/* include/numerics/csr.hpp:51-68 */
+d_CSR* d_csr_create(void);
+void d_csr_destroy(d_CSR* A);
+h_CSR* h_csr_create(uint32_t n, const uint32_t* row, const uint32_t* col, const float* val);
+void h_csr_destroy(h_CSR* A);
+void d_csr_h2d(d_CSR* dcsr, const h_CSR* hcsr);
+/* include/numerics/linear_system.hpp:48-54 */
+LinSys* LinSys_create(void);
+void LinSys_set(LinSys* sys, const d_CSR* A, const float* b);
+void LinSys_destroy(LinSys* sys);
+/* include/numerics/preconditioner.hpp:39-40 */
+PrecJacobi_st* PrecJacobi_create(const d_CSR* A);
+void PrecJacobi_destroy(PrecJacobi_st* prec);
+/* include/numerics/solver.hpp:54-61 */
+SolverCG* SolverCG_create(void);
+void SolverCG_destroy(SolverCG* solver);
+uint32_t SolverCG_jacobi_solve(SolverCG* solver, const LinSys* sys, const PrecJacobi_st* prec, float* x);
+uint32_t SolverCG_solve(SolverCG* solver, const LinSys* sys, float* x);
+
+/* ---- extensions ---- */
+h_CSR* h_csr_create_f64(uint32_t n, const uint32_t* row, const uint32_t* col, const double* val);
+int32_t LinSys_set_f64(LinSys* sys, const d_CSR* A, const double* b);
+/* rows without a usable diagonal (applied as identity) */
+uint32_t PrecJacobi_missing_diagonals(const PrecJacobi_st* prec);
+/* FP64 Jacobi-PCG to ||b-Ax||/||b|| <= rtol; stats [4]: iterations, true
+ * relative residual, ms, residual replacements.  Non-zero status when the
+ * residual is not reached. */
+int32_t SolverCG_solve_rel_f64(SolverCG* solver, const LinSys* sys, const PrecJacobi_st* prec, double* x, double rtol,
+                               uint32_t maxit, double* stats);
+/* FP32 inner iterations, FP64 dots and FP64 residual replacement; same stats. */
+int32_t SolverCG_solve_mixed(SolverCG* solver, const LinSys* sys, const PrecJacobi_st* prec, double* x, double rtol,
+                             uint32_t maxit, double* stats);
+
+/* Process-wide solver knobs: "replace_factor" (mixed precision residual
+ * replacement threshold, default 0.1) and "batch" (iterations between host
+ * checks, default 32). */
+int32_t Stream_tune(const char* key, double value);
+
+/* Mesh-and-solve step on device-resident points: LBVH + 2D alpha-shape +
+ * P1 pressure assembly (Dirichlet p = 0 on free-surface nodes) + CG. */
+PfemStep2D* PfemStep2D_create(void);
+void PfemStep2D_destroy(PfemStep2D* step);
+/* xy [2N], alpha [N], nodal source f [N], input order */
+int32_t PfemStep2D_set_points(PfemStep2D* step, uint32_t n, const double* xy, const double* alpha, const double* f);
+int32_t PfemStep2D_set_morton_bits(PfemStep2D* step, int32_t bits);
+/* solver 0 = FP64 Jacobi-PCG, 1 = mixed FP32/FP64 Jacobi-PCG.  stats [8]:
+ * mesh ms, assembly ms, solve ms, iterations, true relative residual,
+ * replacements, nnz, triangles. */
+int32_t PfemStep2D_run(PfemStep2D* step, int32_t solver, double rtol, uint32_t maxit, double* stats);
+int32_t PfemStep2D_get_solution(PfemStep2D* step, double* p);
+int32_t PfemStep2D_get_solution_morton(PfemStep2D* step, double* p);
+uint32_t PfemStep2D_nnz(PfemStep2D* step);
+int32_t PfemStep2D_get_system(PfemStep2D* step, uint32_t* row, uint32_t* col, double* val, double* b);
+int32_t PfemStep2D_get_mesh(PfemStep2D* step, uint32_t* elem, uint8_t* node_is_bnd, double* xy, uint32_t* perm);
+int32_t PfemStep2D_time_dir_kernel(PfemStep2D* step, int32_t fp32, uint32_t reps, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,7 +1,7 @@
 """MI355X-native implementation of Stream's PFEM mesh-and-solve path.
 
 The compute lives in one HIP library, ``lib/libstream_amd.so`` (built by
-``python -m stream_amd.build``).  The reference's three library names
+``python stream_amd/build.py``).  The reference's three library names
 (``libgeometry``, ``libmesh``, ``libnumerics``) are aliases of it, loaded
 here the way the reference's ``python/__init__.py:26-42`` loads its own,
 with ``numpy.ctypeslib.load_library``.
@@ -20,7 +20,7 @@ LIBDIR = Path(__file__).resolve().parent / "lib"
 
 if not (LIBDIR / "libstream_amd.so").exists():
     raise ImportError(
-        f"stream_amd native library missing under {LIBDIR}; build it with 'python -m stream_amd.build'"
+        f"stream_amd native library missing under {LIBDIR}; build it with 'python stream_amd/build.py'"
     )
 
 libgeo = np.ctypeslib.load_library("libgeometry", str(LIBDIR))
@@ -34,6 +34,8 @@ lib.Stream_device_count.restype = ctypes.c_int32
 lib.Stream_device_count.argtypes = []
 lib.Stream_synchronize.restype = ctypes.c_int32
 lib.Stream_synchronize.argtypes = []
+lib.Stream_tune.restype = ctypes.c_int32
+lib.Stream_tune.argtypes = [ctypes.c_char_p, ctypes.c_double]
 
 
 class StreamError(RuntimeError):
@@ -55,4 +57,9 @@ def synchronize() -> None:
     check(lib.Stream_synchronize())
 
 
-from . import geometry, mesh, numerics  # noqa: E402,F401
+def tune(key: str, value: float) -> None:
+    """Set a solver knob: 'replace_factor' or 'batch'."""
+    check(lib.Stream_tune(key.encode(), float(value)))
+
+
+from . import geometry, mesh, numerics, pfem  # noqa: E402,F401

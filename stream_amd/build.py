@@ -7,7 +7,8 @@ CMakeLists.txt:93-95, python/__init__.py:40-42) are created as symlinks to
 it, so a loader written for the reference finds every symbol under the name
 it expects.
 
-Usage: ``python -m stream_amd.build`` (or ``build()`` from Python).
+Usage: ``python stream_amd/build.py`` (or ``build()`` from Python; the
+package itself needs the built library, so load this file by path).
 """
 from __future__ import annotations
 

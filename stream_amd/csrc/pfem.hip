@@ -1,0 +1,735 @@
+// P1 pressure-Laplacian assembly on the alpha-shape, the mesh-and-solve
+// step, and the reference numerics C-ABI (d_CSR / h_CSR / LinSys /
+// PrecJacobi / SolverCG).
+//
+// P1 assembly has no counterpart in the reference (its README lists an
+// fem module that does not exist).  Row i of the matrix is assembled by
+// the thread of node i from its own kept local triangles, with no atomics:
+//   K_ij = sum over the (at most two) triangles sharing edge ij of
+//          -cot(theta_k) / 2, the cotangent evaluated from the triangle's
+//          vertices in ascending id order so that rows i and j compute the
+//          same bits (the matrix is bitwise symmetric);
+//   K_ii = -(sum of the row's off-diagonals in ascending column order);
+//   Dirichlet p = 0 on free-surface nodes: identity rows, zeroed columns;
+//   b_i  = f_i * sum_T |T| / 3 (lumped load) on free rows.
+// Columns are sorted; the row of node i starts at edge_row[i] + i.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ashape2d.hpp"
+#include "numerics.hpp"
+
+namespace stream_amd {
+
+namespace {
+
+__device__ __forceinline__ double cot_at(const double* pk, const double* pi, const double* pj) {
+    const double ex = pi[0] - pk[0], ey = pi[1] - pk[1];
+    const double fx = pj[0] - pk[0], fy = pj[1] - pk[1];
+    const double dot = ex * fx + ey * fy;
+    const double crs = fabs(ex * fy - ey * fx);
+    return dot / crs;
+}
+
+struct P1Args {
+    uint32_t n;
+    const Sph4<double>* P;
+    const uint32_t* cand;
+    const uint16_t* tri;
+    const uint8_t* nelem;
+    const uint8_t* bnd;
+    const uint32_t* erow;
+    const uint32_t* eneig;
+    const double* f;  // Morton-ordered source
+    uint32_t* arow;
+    uint32_t* acol;
+    double* aval;
+    float* aval32;
+    double* idiag;
+    float* idiag32;
+    double* b;
+    uint32_t cap;  // capacity of acol / aval
+    uint32_t* flags;
+};
+
+__global__ __launch_bounds__(256) void p1_assemble_kernel(P1Args a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t n = a.n;
+    const uint32_t e0 = a.erow[i];
+    const uint32_t nf = a.erow[i + 1] - e0;
+    const uint32_t base = e0 + i;
+    a.arow[i] = base;
+    if (i == n - 1) a.arow[n] = a.erow[n] + n;
+    if (base + nf + 1 > a.cap) {
+        // only reachable after an edge-capacity overflow of the mesh
+        raise_flag(a.flags, kErrCapacity);
+        return;
+    }
+    uint32_t* col = a.acol + base;
+    double* val = a.aval + base;
+    // sorted column list (insertion sort; rows hold ~7 entries)
+    col[0] = i;
+    uint32_t cnt = 1;
+    for (uint32_t k = 0; k < nf; ++k) {
+        const uint32_t v = a.eneig[e0 + k];
+        uint32_t p = cnt;
+        while (p > 0 && col[p - 1] > v) {
+            col[p] = col[p - 1];
+            --p;
+        }
+        col[p] = v;
+        ++cnt;
+    }
+    for (uint32_t k = 0; k < cnt; ++k) val[k] = 0.0;
+    auto find = [&](uint32_t v) -> uint32_t {
+        uint32_t lo = 0, hi = cnt;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (col[mid] <= v)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        return lo;
+    };
+    auto gneig = [&](uint32_t j) -> uint32_t { return j < kNInf2D ? n + j : a.cand[slab_index(i, j - kNInf2D)]; };
+    double mass = 0.0;
+    const uint32_t nt = a.nelem[i];
+    for (uint32_t t = 0; t < nt; ++t) {
+        const uint16_t e = a.tri[slab_index(i, t)];
+        const uint32_t ga = gneig(e & 255u), gb = gneig(e >> 8);
+        uint32_t v[3] = {i, ga, gb};
+        if (v[1] < v[0]) { const uint32_t x = v[0]; v[0] = v[1]; v[1] = x; }
+        if (v[2] < v[1]) { const uint32_t x = v[1]; v[1] = v[2]; v[2] = x; }
+        if (v[1] < v[0]) { const uint32_t x = v[0]; v[0] = v[1]; v[1] = x; }
+        double p[3][2];
+        for (int k = 0; k < 3; ++k) {
+            const Sph4<double> s = a.P[v[k]];
+            p[k][0] = s.c[0];
+            p[k][1] = s.c[1];
+        }
+        const double c0 = cot_at(p[0], p[1], p[2]);
+        const double c1 = cot_at(p[1], p[0], p[2]);
+        const double c2 = cot_at(p[2], p[0], p[1]);
+        const double ux = p[1][0] - p[0][0], uy = p[1][1] - p[0][1];
+        const double wx = p[2][0] - p[0][0], wy = p[2][1] - p[0][1];
+        const double area = 0.5 * fabs(ux * wy - uy * wx);
+        // k(edge) = -cot(opposite)/2: (v1,v2) <- c0, (v0,v2) <- c1, (v0,v1) <- c2
+        const double kk[3] = {-0.5 * c0, -0.5 * c1, -0.5 * c2};
+        // the two neighbours of i in this triangle and the vertex opposite
+        // each edge (i, nb)
+        for (int s2 = 0; s2 < 2; ++s2) {
+            const uint32_t nb = s2 == 0 ? ga : gb;
+            const uint32_t opp = s2 == 0 ? gb : ga;
+            const int ko = (opp == v[0]) ? 0 : ((opp == v[1]) ? 1 : 2);
+            val[find(nb)] += kk[ko];
+        }
+        mass += area / 3.0;
+    }
+    const bool bi = a.bnd[i] != 0;
+    double diag = 0.0;
+    uint32_t dpos = 0;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        if (col[k] == i) {
+            dpos = k;
+            continue;
+        }
+        diag -= val[k];
+    }
+    double rowsum = 0.0;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        double v;
+        if (k == dpos)
+            v = bi ? 1.0 : diag;
+        else
+            v = (bi || a.bnd[col[k]]) ? 0.0 : val[k];
+        val[k] = v;
+        rowsum += v;
+        a.aval32[base + k] = (float)v;
+    }
+    // FP32 copy in row-sum form: the diagonal slot holds the row sum of the
+    // final row (zero on interior rows away from the Dirichlet set)
+    a.aval32[base + dpos] = (float)rowsum;
+    const double dg = bi ? 1.0 : diag;
+    const double id = 1.0 / dg;
+    a.idiag[i] = id;
+    a.idiag32[i] = (float)id;
+    a.b[i] = bi ? 0.0 : a.f[i] * mass;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void permute_fwd_kernel(const T* __restrict__ in, const uint32_t* __restrict__ perm,
+                                                          uint32_t n, T* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[perm[i]];
+}
+template <class T>
+__global__ __launch_bounds__(256) void permute_bwd_kernel(const T* __restrict__ in, const uint32_t* __restrict__ perm,
+                                                          uint32_t n, T* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[perm[i]] = in[i];
+}
+
+__global__ void pack_spheres_kernel(const double* __restrict__ xy, const double* __restrict__ alpha, uint32_t n,
+                                    Sph4<double>* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Sph4<double> s;
+    s.c[0] = xy[2ull * i];
+    s.c[1] = xy[2ull * i + 1];
+    s.c[2] = 0.0;
+    s.r = alpha[i];
+    out[i] = s;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// The PFEM mesh-and-solve step on device-resident points.
+// ---------------------------------------------------------------------------
+struct Step2D {
+    Ash2D<double> mesh;
+    DBuf<Sph4<double>> pts;  // input order
+    DBuf<double> f_in, f_m, b, x, p_out;
+    DevCsr<double> A;
+    DBuf<float> val32, idiag32;
+    DBuf<double> idiag;
+    CgWork work;
+    SolveStats last;
+    double t_mesh = 0, t_asm = 0, t_solve = 0;
+    uint32_t n = 0;
+
+    void set_points(uint32_t n_, const double* h_xy, const double* h_alpha, const double* h_f) {
+        n = n_;
+        hipStream_t s = ctx().stream;
+        DBuf<double> dxy, da;
+        dxy.resize(2ull * n);
+        da.resize(n);
+        h2d(dxy.p, h_xy, 2ull * n);
+        h2d(da.p, h_alpha, n);
+        pts.resize(n);
+        hipLaunchKernelGGL(pack_spheres_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, dxy.p, da.p, n, pts.p);
+        f_in.resize(n);
+        h2d(f_in.p, h_f, n);
+        SA_HIP(hipStreamSynchronize(s));
+    }
+
+    // mesh (LBVH + alpha-shape), permute the source, assemble
+    void mesh_and_assemble(int bits) {
+        hipStream_t s = ctx().stream;
+        hipEvent_t ev[3];
+        for (auto& e : ev) SA_HIP(hipEventCreate(&e));
+        SA_HIP(hipEventRecord(ev[0], s));
+        mesh.bits = bits;
+        mesh.set_nodes_device(n, pts.p);
+        mesh.init(s);
+        mesh.compute(s);
+        SA_HIP(hipEventRecord(ev[1], s));
+        assemble(s);
+        SA_HIP(hipEventRecord(ev[2], s));
+        SA_HIP(hipEventSynchronize(ev[2]));
+        float a = 0, c = 0;
+        SA_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+        SA_HIP(hipEventElapsedTime(&c, ev[1], ev[2]));
+        t_mesh = a;
+        t_asm = c;
+        for (auto& e : ev) SA_HIP(hipEventDestroy(e));
+    }
+
+    void assemble(hipStream_t s) {
+        const uint32_t cap = mesh.cap_edges + n;  // nnz <= edges + n
+        A.n = n;
+        A.row.resize(n + 1);
+        A.col.resize(cap);
+        A.val.resize(cap);
+        val32.resize(cap);
+        idiag.resize(n);
+        idiag32.resize(n);
+        b.resize(n);
+        f_m.resize(n);
+        x.resize(n);
+        hipLaunchKernelGGL((permute_fwd_kernel<double>), dim3(grid_for(n, 256)), dim3(256), 0, s, f_in.p,
+                           mesh.lbvh.perm, n, f_m.p);
+        P1Args a;
+        a.n = n;
+        a.P = mesh.lbvh.obj_m.p;
+        a.cand = mesh.cand.p;
+        a.tri = mesh.tri.p;
+        a.nelem = mesh.nelem.p;
+        a.bnd = mesh.node_is_bnd.p;
+        a.erow = mesh.row.p;
+        a.eneig = mesh.neig.p;
+        a.f = f_m.p;
+        a.arow = A.row.p;
+        a.acol = A.col.p;
+        a.aval = A.val.p;
+        a.aval32 = val32.p;
+        a.idiag = idiag.p;
+        a.idiag32 = idiag32.p;
+        a.b = b.p;
+        a.cap = cap;
+        a.flags = ctx().d_flags;
+        hipLaunchKernelGGL(p1_assemble_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, a);
+    }
+
+    // solver: 0 = FP64 Jacobi-PCG, 1 = mixed FP32/FP64 Jacobi-PCG
+    void solve(int solver, double rtol, uint32_t maxit) {
+        hipStream_t s = ctx().stream;
+        fill<double>(x.p, n, 0.0, s);
+        A.nnz = n + mesh.cap_edges;
+        if (solver == 0) {
+            cg_fp64(A, idiag.p, b.p, x.p, rtol, maxit, work, last, s);
+        } else if (solver == 1) {
+            CgOptions o;
+            o.rtol = rtol;
+            o.maxit = maxit;
+            o.replace_factor = tuning().replace_factor;
+            o.batch = tuning().batch;
+            cg_fp32_opts(A, val32.p, idiag32.p, kPrecJacobi, b.p, x.p, o, work, last, s, true);
+        } else {
+            throw std::invalid_argument("unknown solver kind");
+        }
+        t_solve = last.ms;
+    }
+
+    void solution_input_order(double* h_p) {
+        hipStream_t s = ctx().stream;
+        p_out.resize(n);
+        hipLaunchKernelGGL((permute_bwd_kernel<double>), dim3(grid_for(n, 256)), dim3(256), 0, s, x.p,
+                           mesh.lbvh.perm, n, p_out.p);
+        d2h(h_p, p_out.p, n);
+    }
+};
+
+}  // namespace stream_amd
+
+using namespace stream_amd;
+
+// ===========================================================================
+// Reference numerics C-ABI (csr.hpp:51-68, linear_system.hpp:48-54,
+// preconditioner.hpp:39-40, solver.hpp:54-61) plus FP64 extensions.
+// ===========================================================================
+struct h_CSR {
+    uint32_t n = 0;
+    std::vector<uint32_t> row, col;
+    std::vector<float> val32;
+    std::vector<double> val64;
+    bool f64 = false;
+};
+struct d_CSR {
+    DevCsr<double> A;  // row/col always; val when f64
+    DBuf<float> val32;
+    bool f64 = false;
+    void ensure_f64() {
+        if (f64) return;
+        // widen the FP32 values exactly
+        std::vector<float> h(A.nnz);
+        d2h(h.data(), val32.p, A.nnz);
+        std::vector<double> d(h.begin(), h.end());
+        A.val.resize(A.nnz ? A.nnz : 1);
+        h2d(A.val.p, d.data(), A.nnz);
+        f64 = true;
+    }
+};
+struct LinSys {
+    uint32_t n = 0;
+    const d_CSR* A = nullptr;  // shares the CSR, as LinSys_set copies the DeviceCSR handle
+    DBuf<double> b;
+};
+struct PrecJacobi_st {
+    DBuf<double> idiag;
+    DBuf<float> idiag32;
+    uint32_t missing = 0;
+};
+struct SolverCG {
+    CgWork work;
+    SolveStats last;
+    DBuf<double> x;
+};
+
+namespace {
+std::vector<float> h_vec_f32(const h_CSR* h) {
+    if (!h->f64) return h->val32;
+    return std::vector<float>(h->val64.begin(), h->val64.end());
+}
+
+__global__ void jacobi_kernel(const uint32_t* row, const uint32_t* col, const float* v32, const double* v64,
+                              uint32_t n, double* idiag, float* idiag32, uint32_t* missing) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = 0.0;
+    bool found = false;
+    for (uint32_t k = row[i]; k < row[i + 1]; ++k)
+        if (col[k] == i) {
+            d = v64 ? v64[k] : (double)v32[k];
+            found = true;
+            break;
+        }
+    if (!found || d == 0.0) {
+        // the reference leaves such entries uninitialised
+        // (preconditioner_jacobi.cpp:39-49); here they act as identity
+        atomicAdd(missing, 1u);
+        idiag[i] = 1.0;
+        idiag32[i] = 1.0f;
+        return;
+    }
+    idiag[i] = 1.0 / d;
+    idiag32[i] = v64 ? (float)(1.0 / d) : 1.0f / (float)d;
+}
+
+template <class F> int32_t g(F&& f) { return guarded(std::forward<F>(f)); }
+
+// Run a solve; a solve that ends above rtol returns kErrNotConverged with
+// its outputs and stats written.
+template <class F> int32_t not_converged_status(SolverCG* sol, double rtol, F&& f) {
+    const int32_t st = guarded(std::forward<F>(f));
+    if (st) return st;
+    if (!(sol->last.true_rel <= rtol)) {
+        char buf[160];
+        std::snprintf(buf, sizeof(buf), "CG stopped at relative residual %.3e above %.3e after %u iterations",
+                      sol->last.true_rel, rtol, sol->last.iters);
+        set_last_error(buf);
+        return kErrNotConverged;
+    }
+    return kOk;
+}
+}  // namespace
+
+extern "C" {
+
+d_CSR* d_csr_create(void) {
+    try {
+        (void)ctx();
+        return new d_CSR();
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return nullptr;
+    }
+}
+void d_csr_destroy(d_CSR* A) { delete A; }
+
+h_CSR* h_csr_create(uint32_t n, const uint32_t* row, const uint32_t* col, const float* val) {
+    auto* h = new h_CSR();
+    h->n = n;
+    const uint32_t nnz = row[n];
+    h->row.assign(row, row + n + 1);
+    h->col.assign(col, col + nnz);
+    h->val32.assign(val, val + nnz);
+    return h;
+}
+// FP64 values (extension).
+h_CSR* h_csr_create_f64(uint32_t n, const uint32_t* row, const uint32_t* col, const double* val) {
+    auto* h = new h_CSR();
+    h->n = n;
+    const uint32_t nnz = row[n];
+    h->row.assign(row, row + n + 1);
+    h->col.assign(col, col + nnz);
+    h->val64.assign(val, val + nnz);
+    h->f64 = true;
+    return h;
+}
+void h_csr_destroy(h_CSR* A) { delete A; }
+
+void d_csr_h2d(d_CSR* d, const h_CSR* h) {
+    int32_t st = g([&] {
+        if (!d || !h) throw std::invalid_argument("null CSR handle");
+        const uint32_t n = h->n, nnz = h->row[n];
+        d->A.n = n;
+        d->A.nnz = nnz;
+        d->A.row.resize(n + 1);
+        d->A.col.resize(nnz ? nnz : 1);
+        d->val32.resize(nnz ? nnz : 1);
+        h2d(d->A.row.p, h->row.data(), n + 1);
+        h2d(d->A.col.p, h->col.data(), nnz);
+        const auto v32 = h_vec_f32(h);
+        h2d(d->val32.p, v32.data(), nnz);
+        d->f64 = h->f64;
+        if (h->f64) {
+            d->A.val.resize(nnz ? nnz : 1);
+            h2d(d->A.val.p, h->val64.data(), nnz);
+        }
+    });
+    (void)st;
+}
+
+LinSys* LinSys_create(void) { return new LinSys(); }
+void LinSys_destroy(LinSys* s) { delete s; }
+
+void LinSys_set(LinSys* sys, const d_CSR* A, const float* b) {
+    (void)g([&] {
+        sys->n = A->A.n;
+        sys->A = A;
+        std::vector<double> bd(b, b + A->A.n);
+        sys->b.resize(sys->n ? sys->n : 1);
+        h2d(sys->b.p, bd.data(), sys->n);
+    });
+}
+// FP64 right-hand side (extension).
+int32_t LinSys_set_f64(LinSys* sys, const d_CSR* A, const double* b) {
+    return g([&] {
+        sys->n = A->A.n;
+        sys->A = A;
+        sys->b.resize(sys->n ? sys->n : 1);
+        h2d(sys->b.p, b, sys->n);
+    });
+}
+
+PrecJacobi_st* PrecJacobi_create(const d_CSR* A) {
+    auto* p = new PrecJacobi_st();
+    int32_t st = g([&] {
+        const uint32_t n = A->A.n;
+        p->idiag.resize(n ? n : 1);
+        p->idiag32.resize(n ? n : 1);
+        DBuf<uint32_t> miss;
+        miss.resize(1);
+        fill<uint32_t>(miss.p, 1, 0u, ctx().stream);
+        hipLaunchKernelGGL(jacobi_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx().stream, A->A.row.p, A->A.col.p,
+                           A->val32.p, A->f64 ? A->A.val.p : nullptr, n, p->idiag.p, p->idiag32.p, miss.p);
+        p->missing = read_scalar(miss.p);
+    });
+    if (st) {
+        delete p;
+        return nullptr;
+    }
+    return p;
+}
+void PrecJacobi_destroy(PrecJacobi_st* p) { delete p; }
+// Rows without a usable diagonal (treated as identity).
+uint32_t PrecJacobi_missing_diagonals(const PrecJacobi_st* p) { return p ? p->missing : 0u; }
+
+SolverCG* SolverCG_create(void) {
+    try {
+        (void)ctx();
+        return new SolverCG();
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return nullptr;
+    }
+}
+void SolverCG_destroy(SolverCG* s) { delete s; }
+
+// Reference FP32 semantics (conjugate_gradient.cpp:61-136,138-208): stop
+// when sqrt(r.z) (Jacobi) or sqrt(r.r) (plain) <= 1e-6, at most 10001
+// loop bodies, and return the reference's post-incremented count.
+static uint32_t ref_solve(SolverCG* sol, const LinSys* sys, const PrecJacobi_st* prec, float* x) {
+    uint32_t ret = 0;
+    (void)g([&] {
+        const d_CSR* A = sys->A;
+        const uint32_t n = sys->n;
+        sol->x.resize(n ? n : 1);
+        std::vector<double> xd(x, x + n);
+        h2d(sol->x.p, xd.data(), n);
+        const_cast<d_CSR*>(A)->ensure_f64();
+        CgOptions o;
+        o.abs_tol2 = 1e-12;
+        o.maxit = 10001;
+        o.crit = prec ? 1u : 0u;
+        o.verify = false;
+        o.min_one = true;
+        cg_fp32_opts(A->A, A->val32.p, prec ? prec->idiag32.p : nullptr, prec ? kPrecJacobi : kPrecNone, sys->b.p,
+                     sol->x.p, o, sol->work, sol->last, ctx().stream);
+        d2h(xd.data(), sol->x.p, n);
+        for (uint32_t i = 0; i < n; ++i) x[i] = (float)xd[i];
+        ret = sol->last.iters >= 10001 ? 10001u : sol->last.iters - 1;
+    });
+    return ret;
+}
+
+uint32_t SolverCG_jacobi_solve(SolverCG* solver, const LinSys* sys, const PrecJacobi_st* prec, float* x) {
+    if (!prec) {
+        set_last_error("SolverCG_jacobi_solve: null preconditioner");
+        return 0;
+    }
+    return ref_solve(solver, sys, prec, x);
+}
+
+uint32_t SolverCG_solve(SolverCG* solver, const LinSys* sys, float* x) { return ref_solve(solver, sys, nullptr, x); }
+
+// FP64 Jacobi-PCG to a relative true residual ||b-Ax||/||b|| <= rtol
+// (extension).  x: initial guess in, solution out.  stats (may be null):
+// [iterations, true relative residual, milliseconds, replacements].
+int32_t SolverCG_solve_rel_f64(SolverCG* sol, const LinSys* sys, const PrecJacobi_st* prec, double* x, double rtol,
+                               uint32_t maxit, double* stats) {
+    return not_converged_status(sol, rtol, [&] {
+        const d_CSR* A = sys->A;
+        const uint32_t n = sys->n;
+        const_cast<d_CSR*>(A)->ensure_f64();
+        sol->x.resize(n ? n : 1);
+        h2d(sol->x.p, x, n);
+        cg_fp64(A->A, prec->idiag.p, sys->b.p, sol->x.p, rtol, maxit, sol->work, sol->last, ctx().stream);
+        d2h(x, sol->x.p, n);
+        if (stats) {
+            stats[0] = sol->last.iters;
+            stats[1] = sol->last.true_rel;
+            stats[2] = sol->last.ms;
+            stats[3] = sol->last.replacements;
+        }
+    });
+}
+
+// Mixed precision (extension): FP32 matrix/vectors/preconditioner, FP64
+// dots, FP64 solution and residual replacement.  Same stats layout.
+int32_t SolverCG_solve_mixed(SolverCG* sol, const LinSys* sys, const PrecJacobi_st* prec, double* x, double rtol,
+                             uint32_t maxit, double* stats) {
+    return not_converged_status(sol, rtol, [&] {
+        const d_CSR* A = sys->A;
+        const uint32_t n = sys->n;
+        const_cast<d_CSR*>(A)->ensure_f64();
+        sol->x.resize(n ? n : 1);
+        h2d(sol->x.p, x, n);
+        CgOptions o;
+        o.rtol = rtol;
+        o.maxit = maxit;
+        o.replace_factor = tuning().replace_factor;
+        o.batch = tuning().batch;
+        cg_fp32_opts(A->A, A->val32.p, prec->idiag32.p, kPrecJacobi, sys->b.p, sol->x.p, o, sol->work, sol->last,
+                     ctx().stream);
+        d2h(x, sol->x.p, n);
+        if (stats) {
+            stats[0] = sol->last.iters;
+            stats[1] = sol->last.true_rel;
+            stats[2] = sol->last.ms;
+            stats[3] = sol->last.replacements;
+        }
+    });
+}
+
+// Process-wide solver knobs: "replace_factor" (mixed precision: replace
+// the recursive residual by the FP64 true residual whenever its norm fell
+// by this factor, default 0.1) and "batch" (CG iterations between host
+// checks, default 32).
+int32_t Stream_tune(const char* key, double value) {
+    return g([&] {
+        const std::string k(key ? key : "");
+        if (k == "replace_factor") {
+            if (!(value > 0.0 && value < 1.0)) throw std::invalid_argument("replace_factor must be in (0, 1)");
+            tuning().replace_factor = value;
+        } else if (k == "batch") {
+            if (!(value >= 1 && value <= 4096)) throw std::invalid_argument("batch must be in [1, 4096]");
+            tuning().batch = (uint32_t)value;
+        } else {
+            throw std::invalid_argument("unknown tuning key: " + k);
+        }
+    });
+}
+
+// ===========================================================================
+// PfemStep2D: LBVH + alpha-shape + P1 assembly + CG on device-resident
+// points (no reference counterpart: the reference never connects mesh and
+// solve).
+// ===========================================================================
+struct PfemStep2D {
+    Step2D s;
+    int bits = 32;
+};
+
+PfemStep2D* PfemStep2D_create(void) {
+    try {
+        (void)ctx();
+        return new PfemStep2D();
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return nullptr;
+    }
+}
+void PfemStep2D_destroy(PfemStep2D* h) { delete h; }
+
+// Points (row-major xy [2N]), alpha [N] and nodal source f [N], input
+// order; copied once into device memory.
+int32_t PfemStep2D_set_points(PfemStep2D* h, uint32_t n, const double* xy, const double* alpha, const double* f) {
+    return g([&] { h->s.set_points(n, xy, alpha, f); });
+}
+int32_t PfemStep2D_set_morton_bits(PfemStep2D* h, int32_t bits) {
+    return g([&] {
+        if (bits < 1 || bits > 32) throw std::invalid_argument("Morton bits per axis must be in [1, 32]");
+        h->bits = bits;
+    });
+}
+
+// One step: mesh + assemble + solve.  solver: 0 = FP64 Jacobi-PCG,
+// 1 = mixed FP32/FP64 Jacobi-PCG.  stats [8]: mesh ms, assembly ms, solve
+// ms, iterations, true relative residual, replacements, nnz, n_elems.
+int32_t PfemStep2D_run(PfemStep2D* h, int32_t solver, double rtol, uint32_t maxit, double* stats) {
+    return g([&] {
+        h->s.mesh_and_assemble(h->bits);
+        h->s.solve(solver, rtol, maxit);
+        if (stats) {
+            stats[0] = h->s.t_mesh;
+            stats[1] = h->s.t_asm;
+            stats[2] = h->s.t_solve;
+            stats[3] = h->s.last.iters;
+            stats[4] = h->s.last.true_rel;
+            stats[5] = h->s.last.replacements;
+            stats[6] = read_scalar(h->s.A.row.p + h->s.n);
+            stats[7] = h->s.mesh.n_elems();
+        }
+        const uint32_t fl = read_scalar(ctx().d_flags + kFlagError);
+        if (fl) {
+            fill<uint32_t>(ctx().d_flags, 1, 0u, ctx().stream);
+            throw std::runtime_error("device error flag raised during the step");
+        }
+        if (h->s.last.true_rel > rtol) throw std::runtime_error("CG did not reach the requested residual");
+    });
+}
+
+// Average duration in ms of the CG direction kernel (d update + SpMV +
+// d.q), FP64 (fp32 = 0) or FP32 (fp32 = 1) values, on the system of the
+// last step; timed with HIP events on the library stream over `reps`
+// launches.
+int32_t PfemStep2D_time_dir_kernel(PfemStep2D* h, int32_t fp32, uint32_t reps, double* ms) {
+    return g([&] {
+        CgWork w;
+        *ms = time_dir_kernel(h->s.A, h->s.val32.p, fp32 != 0, reps, w, ctx().stream);
+    });
+}
+
+// Solution in input order [N].
+int32_t PfemStep2D_get_solution(PfemStep2D* h, double* p) {
+    return g([&] { h->s.solution_input_order(p); });
+}
+
+// Solution in Morton order [N].
+int32_t PfemStep2D_get_solution_morton(PfemStep2D* h, double* p) { return g([&] { d2h(p, h->s.x.p, h->s.n); }); }
+
+uint32_t PfemStep2D_nnz(PfemStep2D* h) {
+    uint32_t r = 0;
+    (void)g([&] { r = read_scalar(h->s.A.row.p + h->s.n); });
+    return r;
+}
+
+// Assembled system in Morton order: row [N+1], col/val [nnz], b [N].
+int32_t PfemStep2D_get_system(PfemStep2D* h, uint32_t* row, uint32_t* col, double* val, double* b) {
+    return g([&] {
+        const uint32_t n = h->s.n;
+        const uint32_t nnz = read_scalar(h->s.A.row.p + n);
+        if (row) d2h(row, h->s.A.row.p, n + 1);
+        if (col) d2h(col, h->s.A.col.p, nnz);
+        if (val) d2h(val, h->s.A.val.p, nnz);
+        if (b) d2h(b, h->s.b.p, n);
+    });
+}
+
+// Mesh outputs of the last step (Morton ids): elements [3*nelem],
+// node_is_bnd [N], Morton-ordered xy [2N], permutation [N].  Any may be null.
+int32_t PfemStep2D_get_mesh(PfemStep2D* h, uint32_t* elem, uint8_t* node_is_bnd, double* xy, uint32_t* perm) {
+    return g([&] {
+        auto& m = h->s.mesh;
+        const uint32_t n = m.n;
+        if (elem) d2h(elem, m.elemglob.p, 3ull * m.n_elems());
+        if (node_is_bnd) d2h(node_is_bnd, m.node_is_bnd.p, n);
+        if (perm) d2h(perm, m.lbvh.perm, n);
+        if (xy) {
+            std::vector<Sph4<double>> v(n);
+            d2h(v.data(), m.lbvh.obj_m.p, n);
+            for (uint32_t i = 0; i < n; ++i) {
+                xy[2 * i] = v[i].c[0];
+                xy[2 * i + 1] = v[i].c[1];
+            }
+        }
+    });
+}
+
+}  // extern "C"

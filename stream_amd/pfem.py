@@ -1,0 +1,87 @@
+"""The PFEM mesh-and-solve step on device-resident points: LBVH, 2D
+alpha-shape, P1 pressure assembly and CG to a true FP64 relative residual.
+No reference counterpart (the reference never connects mesh and solve)."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import StreamError, check, lib
+from ._ct import arr, bind, f64, i32, opt, u32
+
+_p = ctypes.c_void_p
+PfemStep2D_create = bind(lib, "PfemStep2D_create", _p, [])
+PfemStep2D_destroy = bind(lib, "PfemStep2D_destroy", None, [_p])
+PfemStep2D_set_points = bind(lib, "PfemStep2D_set_points", i32,
+                             [_p, u32, arr(np.float64, 2), arr(np.float64, 1), arr(np.float64, 1)])
+PfemStep2D_set_morton_bits = bind(lib, "PfemStep2D_set_morton_bits", i32, [_p, i32])
+PfemStep2D_run = bind(lib, "PfemStep2D_run", i32, [_p, i32, f64, u32, arr(np.float64, 1, writeable=True)])
+PfemStep2D_get_solution = bind(lib, "PfemStep2D_get_solution", i32, [_p, arr(np.float64, 1, writeable=True)])
+PfemStep2D_get_solution_morton = bind(lib, "PfemStep2D_get_solution_morton", i32,
+                                      [_p, arr(np.float64, 1, writeable=True)])
+PfemStep2D_nnz = bind(lib, "PfemStep2D_nnz", u32, [_p])
+PfemStep2D_get_system = bind(lib, "PfemStep2D_get_system", i32,
+                             [_p, opt(np.uint32), opt(np.uint32), opt(np.float64), opt(np.float64)])
+PfemStep2D_get_mesh = bind(lib, "PfemStep2D_get_mesh", i32,
+                           [_p, opt(np.uint32), opt(np.uint8), opt(np.float64), opt(np.uint32)])
+
+PfemStep2D_time_dir_kernel = bind(lib, "PfemStep2D_time_dir_kernel", i32,
+                                  [_p, i32, u32, ctypes.POINTER(ctypes.c_double)])
+
+SOLVER_FP64_JACOBI = 0
+SOLVER_MIXED_JACOBI = 1
+SOLVER_MIXED_LEARNED = 2
+SOLVER_NAMES = {"fp64": SOLVER_FP64_JACOBI, "mixed": SOLVER_MIXED_JACOBI, "learned": SOLVER_MIXED_LEARNED}
+
+
+class PfemStep2D:
+    def __init__(self, xy, alpha, f, morton_bits: int = 32):
+        self.h = PfemStep2D_create()
+        if not self.h:
+            raise StreamError(lib.Stream_last_error().decode())
+        xy = np.ascontiguousarray(xy, np.float64)
+        self.n = xy.shape[0]
+        check(PfemStep2D_set_morton_bits(self.h, morton_bits))
+        check(PfemStep2D_set_points(self.h, self.n, xy, np.ascontiguousarray(alpha, np.float64),
+                                    np.ascontiguousarray(f, np.float64)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            PfemStep2D_destroy(self.h)
+            self.h = None
+
+    def run(self, solver="fp64", rtol=1e-8, maxit=200000):
+        st = np.zeros(8)
+        kind = SOLVER_NAMES[solver] if isinstance(solver, str) else int(solver)
+        check(PfemStep2D_run(self.h, kind, rtol, maxit, st))
+        return dict(mesh_ms=st[0], assemble_ms=st[1], solve_ms=st[2], iters=int(st[3]), true_rel=st[4],
+                    replacements=int(st[5]), nnz=int(st[6]), n_elems=int(st[7]))
+
+    def time_dir_kernel(self, fp32: bool, reps: int = 200) -> float:
+        """Average ms of the CG direction kernel on the current system."""
+        ms = ctypes.c_double(0.0)
+        check(PfemStep2D_time_dir_kernel(self.h, int(fp32), reps, ctypes.byref(ms)))
+        return ms.value
+
+    def solution(self, morton=False):
+        p = np.empty(self.n)
+        check((PfemStep2D_get_solution_morton if morton else PfemStep2D_get_solution)(self.h, p))
+        return p
+
+    def system(self):
+        nnz = PfemStep2D_nnz(self.h)
+        row = np.empty(self.n + 1, np.uint32)
+        col = np.empty(nnz, np.uint32)
+        val = np.empty(nnz)
+        b = np.empty(self.n)
+        check(PfemStep2D_get_system(self.h, row, col, val, b))
+        return row, col, val, b
+
+    def mesh(self, n_elems):
+        elem = np.empty((n_elems, 3), np.uint32)
+        bnd = np.empty(self.n, np.uint8)
+        xy = np.empty((self.n, 2))
+        perm = np.empty(self.n, np.uint32)
+        check(PfemStep2D_get_mesh(self.h, elem, bnd, xy, perm))
+        return elem, bnd, xy, perm
