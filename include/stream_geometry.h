@@ -46,6 +46,9 @@ int32_t LBVH2D_collisions_f64(uint32_t n, const double* xyr, int32_t bits, uint6
 /* stable onesweep radix sort of (u64, u32) pairs over the low `bits` bits */
 int32_t Stream_sort_pairs_u64(uint32_t n, const uint64_t* keys, const uint32_t* vals, uint64_t* keys_out,
                               uint32_t* vals_out, int32_t bits);
+/* plain copy of nbytes (multiple of 16) repeated reps times; average ms.
+ * Calibration kernel for hardware byte counters. */
+int32_t Stream_copy_calibration(uint64_t nbytes, uint32_t reps, double* ms);
 /* exclusive scans, out [n+1] with out[n] = total */
 int32_t Stream_exclusive_scan_u32(uint32_t n, const uint32_t* in, uint32_t* out);
 int32_t Stream_exclusive_scan_u8(uint32_t n, const uint8_t* in, uint32_t* out);

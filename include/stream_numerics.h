@@ -68,7 +68,8 @@ void PfemStep2D_destroy(PfemStep2D* step);
 /* xy [2N], alpha [N], nodal source f [N], input order */
 int32_t PfemStep2D_set_points(PfemStep2D* step, uint32_t n, const double* xy, const double* alpha, const double* f);
 int32_t PfemStep2D_set_morton_bits(PfemStep2D* step, int32_t bits);
-/* solver 0 = FP64 Jacobi-PCG, 1 = mixed FP32/FP64 Jacobi-PCG.  stats [8]:
+/* solver 0 = FP64 Jacobi-PCG, 1 = mixed FP32/FP64 Jacobi-PCG, 2 = mixed PCG
+ * with the FSAI L L^T factor.  stats [8]:
  * mesh ms, assembly ms, solve ms, iterations, true relative residual,
  * replacements, nnz, triangles. */
 int32_t PfemStep2D_run(PfemStep2D* step, int32_t solver, double rtol, uint32_t maxit, double* stats);

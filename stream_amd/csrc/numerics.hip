@@ -23,7 +23,7 @@ void CgWork::resize(uint32_t n) {
     if (blocks == 0) blocks = 1;
     sc.resize(1);
     out2.resize(2);
-    partials.resize(4ull * blocks);
+    partials.resize(4ull * kMaxCgBlocks);
     if (!counter.p) {
         counter.resize(1);
         fill<uint32_t>(counter.p, 1, 0u, ctx().stream);
@@ -101,36 +101,165 @@ __device__ __forceinline__ double row_dot(const uint32_t* __restrict__ row, cons
     return s;
 }
 
+
 #define CG_ROWS(i) for (uint32_t i = blockIdx.x * kCgThreads + threadIdx.x; i < n; i += gridDim.x * kCgThreads)
 
+template <class V> struct V2T;
+template <> struct V2T<float> {
+    using T = float2;
+};
+template <> struct V2T<double> {
+    using T = double2;
+};
+template <class V> using V2 = typename V2T<V>::T;
+
 // ---------------------------------------------------------------------------
-// init / replacement: r = b - A64 x64 (FP64), stored in the vector
-// precision; Jacobi z; partials r.r and r.z (and b.b at init).
-// mode 0: init (d_old = 0, beta = 0, iter = 0)
+// SELL-64 construction
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool in_part(uint32_t c, uint32_t i, int part) {
+    return part == kSellOffDiag ? c != i : (part == kSellLower ? c < i : c > i);
+}
+
+// One wave per slice: row lengths and the slice width (max length).
+__global__ __launch_bounds__(256) void sell_len_kernel(uint32_t n, uint32_t nslices, const uint32_t* __restrict__ row,
+                                                       const uint32_t* __restrict__ col, int part,
+                                                       uint16_t* __restrict__ len, uint16_t* __restrict__ width,
+                                                       uint32_t* __restrict__ w64) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t s = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (s >= nslices) return;
+    const uint32_t i = s * kSlice + lane;
+    uint32_t l = 0;
+    if (i < n)
+        for (uint32_t e = row[i]; e < row[i + 1]; ++e) l += in_part(col[e], i, part) ? 1u : 0u;
+    if (i < n) len[i] = (uint16_t)l;
+    uint32_t m = l;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+    if (lane == 0) {
+        width[s] = (uint16_t)m;
+        w64[s] = m * kSlice;
+    }
+}
+
+template <class V>
+__global__ __launch_bounds__(256) void sell_fill_kernel(uint32_t n, const uint32_t* __restrict__ row,
+                                                        const uint32_t* __restrict__ col,
+                                                        const double* __restrict__ val64,
+                                                        const float* __restrict__ val32, int form, int part,
+                                                        const uint32_t* __restrict__ off,
+                                                        const uint16_t* __restrict__ width,
+                                                        uint32_t* __restrict__ scol, V* __restrict__ sval,
+                                                        V* __restrict__ diag) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = i / kSlice, lane = i % kSlice;
+    const uint32_t base = off[s] + lane;
+    uint32_t k = 0;
+    double rowsum = 0.0, dv = 0.0;
+    for (uint32_t e = row[i]; e < row[i + 1]; ++e) {
+        const uint32_t c = col[e];
+        const double v = val64 ? val64[e] : (double)val32[e];
+        rowsum += v;
+        if (c == i) dv = v;
+        if (in_part(c, i, part)) {
+            scol[base + k * kSlice] = c;
+            sval[base + k * kSlice] = (V)v;
+            ++k;
+        }
+    }
+    for (; k < width[s]; ++k) {
+        scol[base + k * kSlice] = i;
+        sval[base + k * kSlice] = (V)0;
+    }
+    diag[i] = (V)(form == kSellRowSum ? rowsum : dv);
+}
+
+}  // namespace
+
+template <class V>
+void build_sell(uint32_t n, const uint32_t* row, const uint32_t* col, const double* val64, const float* val32,
+                SellForm form, SellPart part, Sell<V>& S, hipStream_t s) {
+    S.n = n;
+    S.nslices = (n + kSlice - 1) / kSlice;
+    const uint32_t ns = S.nslices ? S.nslices : 1;
+    S.off.resize(ns + 1);
+    S.width.resize(ns);
+    S.len.resize(n ? n : 1);
+    S.diag.resize(n ? n : 1);
+    S.tmp.resize(ns);
+    S.scan_ws.resize(scan_workspace_words(ns));
+    if (!n) {
+        S.entries = 0;
+        return;
+    }
+    hipLaunchKernelGGL(sell_len_kernel, dim3((S.nslices * 64 + 255) / 256), dim3(256), 0, s, n, S.nslices, row, col,
+                       (int)part, S.len.p, S.width.p, S.tmp.p);
+    exclusive_scan_u32(S.tmp.p, S.off.p, S.nslices, S.scan_ws.p, s);
+    S.entries = read_scalar(S.off.p + S.nslices);
+    S.col.resize(S.entries ? S.entries : 1);
+    S.val.resize(S.entries ? S.entries : 1);
+    hipLaunchKernelGGL((sell_fill_kernel<V>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, row, col, val64, val32,
+                       (int)form, (int)part, S.off.p, S.width.p, S.col.p, S.val.p, S.diag.p);
+}
+template void build_sell<float>(uint32_t, const uint32_t*, const uint32_t*, const double*, const float*, SellForm,
+                                SellPart, Sell<float>&, hipStream_t);
+template void build_sell<double>(uint32_t, const uint32_t*, const uint32_t*, const double*, const float*, SellForm,
+                                 SellPart, Sell<double>&, hipStream_t);
+
+namespace {
+
+// Matrix arguments of the SELL kernels.
+template <class V> struct SellArgs {
+    uint32_t n, nslices;
+    const uint32_t* off;
+    const uint16_t* width;
+    const uint16_t* len;
+    const uint32_t* col;
+    const V* val;
+    const V* diag;
+};
+template <class V> SellArgs<V> sell_args(const Sell<V>& S) {
+    return {S.n, S.nslices, S.off.p, S.width.p, S.len.p, S.col.p, S.val.p, S.diag.p};
+}
+
+// y_i = diag_i * x_i + sum_k A_ik x_k over one SELL slice, the vector given
+// through a loader; four entries per step, loads issued before use.
+#define SELL_FOR_SLICES(s)                                                             \
+    for (uint32_t s = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; s < M.nslices; \
+         s += (gridDim.x * blockDim.x) >> 6)
+
+// ---------------------------------------------------------------------------
+// init / replacement: r = b - A64 x64 (FP64), stored in precision V;
+// Jacobi z into the current {z, d} buffer; partial r.r, r.z, b.b.
+// mode 0: init (d = 0, beta = 0, iter = 0)
 // mode 1: replacement of the recursive residual by the true one.  The
 //         direction d and beta (computed by the recursion) are kept and only
 //         r, z and r.z change: when the recursive residual had drifted far
 //         below the true one, this degrades to a restart instead of scaling
 //         the old direction by r_true.z / r_rec.z.
 // ---------------------------------------------------------------------------
-template <class VV, int PREC>
+template <class V, int PREC>
 __global__ __launch_bounds__(kCgThreads) void cg_residual_kernel(
     uint32_t n, const uint32_t* __restrict__ row, const uint32_t* __restrict__ col, const double* __restrict__ val,
-    const double* __restrict__ b, const double* __restrict__ x, VV* __restrict__ r, VV* __restrict__ z,
-    VV* __restrict__ dold, const VV* __restrict__ idiag, CgScalars* sc, double* partials, uint32_t* counter,
-    int mode, double rtol2) {
+    const double* __restrict__ b, const double* __restrict__ x, V* __restrict__ r, V2<V>* __restrict__ P,
+    const V* __restrict__ idiag, CgScalars* sc, double* partials, uint32_t* counter, int mode, double rtol2) {
     double v[3] = {0.0, 0.0, 0.0};
     CG_ROWS(i) {
         const double ri = b[i] - row_dot(row, col, val, x, i);
-        const VV rv = (VV)ri;
+        const V rv = (V)ri;
         r[i] = rv;
-        if (mode == 0) dold[i] = (VV)0;
-        v[0] += (double)rv * (double)rv;
+        V2<V> p = P[i];
+        if (mode == 0) p.y = (V)0;
         if constexpr (PREC != kPrecLLt) {
-            const VV zv = (PREC == kPrecJacobi) ? (VV)(idiag[i] * rv) : rv;
-            z[i] = zv;
+            const V zv = (PREC == kPrecJacobi) ? (V)(idiag[i] * rv) : rv;
+            p.x = zv;
             v[1] += (double)rv * (double)zv;
+        } else {
+            if (mode == 0) p.x = (V)0;
         }
+        P[i] = p;
+        v[0] += (double)rv * (double)rv;
         v[2] += b[i] * b[i];
     }
     double tot[3];
@@ -154,41 +283,57 @@ __global__ __launch_bounds__(kCgThreads) void cg_residual_kernel(
     }
 }
 
-// d_new = z + beta d_old; q = A d_new; partial d_new.q.
+// d = z + beta d_old; q = A d; partial d.q.
 // DIFF: the values are in row-sum form (off-diagonals K_ij, diagonal slot
-// holding the row sum s_i) and q_i = sum_j K_ij (d_j - d_i) + s_i d_i.  For
+// holding the row sum s_i) and q_i = s_i d_i + sum_j K_ij (d_j - d_i).  For
 // a graph Laplacian the differences of neighbouring entries of a smooth
 // vector are exact in floating point (Sterbenz), so the FP32 product keeps
 // its relative accuracy on the smooth modes that standard-form summation
 // loses to cancellation.
-template <class MV, class VV, bool DIFF>
-__global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(uint32_t n, const uint32_t* __restrict__ row,
-                                                            const uint32_t* __restrict__ col,
-                                                            const MV* __restrict__ val, const VV* __restrict__ z,
-                                                            const VV* __restrict__ dold, VV* __restrict__ dnew,
-                                                            VV* __restrict__ q, CgScalars* sc, double* partials,
-                                                            uint32_t* counter) {
+template <class V, bool DIFF>
+__global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(SellArgs<V> M, const V2<V>* __restrict__ cur,
+                                                            V2<V>* __restrict__ nxt, V* __restrict__ q,
+                                                            CgScalars* sc, double* partials, uint32_t* counter) {
     if (sc->done) return;
-    const VV beta = (VV)sc->beta;
+    const V beta = (V)sc->beta;
+    const uint32_t lane = threadIdx.x & 63u;
     double v[1] = {0.0};
-    CG_ROWS(i) {
-        const VV dn = z[i] + beta * dold[i];
-        dnew[i] = dn;
-        VV s = (VV)0;
-        const uint32_t e = row[i + 1];
-        for (uint32_t k = row[i]; k < e; ++k) {
-            const uint32_t j = col[k];
-            if constexpr (DIFF) {
-                if (j == i)
-                    s += (VV)val[k] * dn;
+    SELL_FOR_SLICES(s) {
+        const uint32_t i = s * kSlice + lane;
+        const bool valid = i < M.n;
+        const uint32_t ii = valid ? i : 0u;
+        const V2<V> p = cur[ii];
+        const V dn = p.x + beta * p.y;
+        const uint32_t L = valid ? (uint32_t)M.len[i] : 0u;
+        const uint32_t W = M.width[s];
+        const uint32_t base = M.off[s] + lane;
+        V acc = M.diag[ii] * dn;
+        for (uint32_t k = 0; k < W; k += 4) {
+            uint32_t c[4];
+            V a[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool ok = k + u < L;
+                c[u] = ok ? M.col[base + (k + u) * kSlice] : ii;
+                a[u] = ok ? M.val[base + (k + u) * kSlice] : (V)0;
+            }
+            V2<V> g[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) g[u] = cur[c[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const V gd = g[u].x + beta * g[u].y;
+                if constexpr (DIFF)
+                    acc += a[u] * (gd - dn);
                 else
-                    s += (VV)val[k] * ((z[j] + beta * dold[j]) - dn);
-            } else {
-                s += (VV)val[k] * (z[j] + beta * dold[j]);
+                    acc += a[u] * gd;
             }
         }
-        q[i] = s;
-        v[0] += (double)dn * (double)s;
+        if (valid) {
+            nxt[i].y = dn;
+            q[i] = acc;
+            v[0] += (double)dn * (double)acc;
+        }
     }
     double tot[1];
     if (!grid_fold<1>(v, partials, counter, tot)) return;
@@ -198,24 +343,27 @@ __global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(uint32_t n, const ui
     }
 }
 
-// x += alpha d; r -= alpha q; z = M r (Jacobi / none); partial r.z, r.r
-template <class VV, class XV, int PREC>
-__global__ __launch_bounds__(kCgThreads) void cg_update_kernel(uint32_t n, XV* __restrict__ x, VV* __restrict__ r,
-                                                               VV* __restrict__ z, const VV* __restrict__ d,
-                                                               const VV* __restrict__ q, const VV* __restrict__ idiag,
-                                                               CgScalars* sc, double* partials, uint32_t* counter) {
+// x += alpha d; r -= alpha q; z = M r (Jacobi / none) into nxt.x;
+// partial r.z, r.r
+template <class V, class XV, int PREC>
+__global__ __launch_bounds__(kCgThreads) void cg_update_kernel(uint32_t n, XV* __restrict__ x, V* __restrict__ r,
+                                                               V2<V>* __restrict__ nxt, const V* __restrict__ q,
+                                                               const V* __restrict__ idiag, CgScalars* sc,
+                                                               double* partials, uint32_t* counter) {
     if (sc->done) return;
     const double alpha = sc->alpha;
-    const VV av = (VV)alpha;
+    const V av = (V)alpha;
     double v[2] = {0.0, 0.0};
     CG_ROWS(i) {
-        x[i] += (XV)(alpha * (double)d[i]);
-        const VV rv = r[i] - av * q[i];
+        V2<V> p = nxt[i];
+        x[i] += (XV)(alpha * (double)p.y);
+        const V rv = r[i] - av * q[i];
         r[i] = rv;
         v[1] += (double)rv * (double)rv;
         if constexpr (PREC != kPrecLLt) {
-            const VV zv = (PREC == kPrecJacobi) ? idiag[i] * rv : rv;
-            z[i] = zv;
+            const V zv = (PREC == kPrecJacobi) ? idiag[i] * rv : rv;
+            p.x = zv;
+            nxt[i] = p;
             v[0] += (double)rv * (double)zv;
         }
     }
@@ -234,40 +382,71 @@ __global__ __launch_bounds__(kCgThreads) void cg_update_kernel(uint32_t n, XV* _
     }
 }
 
-// w = L^T r : w_i = L_ii r_i + sum_{j>i} L_ji r_j
-template <class VV>
-__global__ __launch_bounds__(kCgThreads) void llt_up_kernel(uint32_t n, const uint32_t* __restrict__ up_row,
-                                                            const uint32_t* __restrict__ up_col,
-                                                            const VV* __restrict__ up_val, const VV* __restrict__ diag,
-                                                            const VV* __restrict__ r, VV* __restrict__ w,
-                                                            const CgScalars* sc, int force) {
+// w = L^T r : w_i = L_ii r_i + sum_{j>i} L_ji r_j (U = the upper SELL).
+template <class V>
+__global__ __launch_bounds__(kCgThreads) void llt_up_kernel(SellArgs<V> M, const V* __restrict__ r,
+                                                            V* __restrict__ w, const CgScalars* sc, int force) {
     if (!force && sc->done) return;
-    CG_ROWS(i) {
-        VV s = diag[i] * r[i];
-        const uint32_t e = up_row[i + 1];
-        for (uint32_t k = up_row[i]; k < e; ++k) s += up_val[k] * r[up_col[k]];
-        w[i] = s;
+    const uint32_t lane = threadIdx.x & 63u;
+    SELL_FOR_SLICES(s) {
+        const uint32_t i = s * kSlice + lane;
+        const bool valid = i < M.n;
+        const uint32_t ii = valid ? i : 0u;
+        const uint32_t L = valid ? (uint32_t)M.len[i] : 0u;
+        const uint32_t W = M.width[s];
+        const uint32_t base = M.off[s] + lane;
+        V acc = M.diag[ii] * r[ii];
+        for (uint32_t k = 0; k < W; k += 4) {
+            uint32_t c[4];
+            V a[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool ok = k + u < L;
+                c[u] = ok ? M.col[base + (k + u) * kSlice] : ii;
+                a[u] = ok ? M.val[base + (k + u) * kSlice] : (V)0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += a[u] * r[c[u]];
+        }
+        if (valid) w[i] = acc;
     }
 }
 
-// z = L w : z_i = L_ii w_i + sum_{j<i} L_ij w_j; partial r.z.
+// z = L w : z_i = L_ii w_i + sum_{j<i} L_ij w_j into P[i].x; partial r.z.
 // mode 0: iteration bookkeeping (beta, iter, done); mode 1: init;
 // mode 2: after a residual replacement (beta kept, as in cg_residual_kernel).
-template <class VV>
-__global__ __launch_bounds__(kCgThreads) void llt_lo_kernel(uint32_t n, const uint32_t* __restrict__ lo_row,
-                                                            const uint32_t* __restrict__ lo_col,
-                                                            const VV* __restrict__ lo_val, const VV* __restrict__ diag,
-                                                            const VV* __restrict__ w, const VV* __restrict__ r,
-                                                            VV* __restrict__ z, CgScalars* sc, double* partials,
-                                                            uint32_t* counter, int mode) {
+template <class V>
+__global__ __launch_bounds__(kCgThreads) void llt_lo_kernel(SellArgs<V> M, const V* __restrict__ w,
+                                                            const V* __restrict__ r, V2<V>* __restrict__ P,
+                                                            CgScalars* sc, double* partials, uint32_t* counter,
+                                                            int mode) {
     if (mode == 0 && sc->done) return;
+    const uint32_t lane = threadIdx.x & 63u;
     double v[1] = {0.0};
-    CG_ROWS(i) {
-        VV s = diag[i] * w[i];
-        const uint32_t e = lo_row[i + 1];
-        for (uint32_t k = lo_row[i]; k < e; ++k) s += lo_val[k] * w[lo_col[k]];
-        z[i] = s;
-        v[0] += (double)r[i] * (double)s;
+    SELL_FOR_SLICES(s) {
+        const uint32_t i = s * kSlice + lane;
+        const bool valid = i < M.n;
+        const uint32_t ii = valid ? i : 0u;
+        const uint32_t L = valid ? (uint32_t)M.len[i] : 0u;
+        const uint32_t W = M.width[s];
+        const uint32_t base = M.off[s] + lane;
+        V acc = M.diag[ii] * w[ii];
+        for (uint32_t k = 0; k < W; k += 4) {
+            uint32_t c[4];
+            V a[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool ok = k + u < L;
+                c[u] = ok ? M.col[base + (k + u) * kSlice] : ii;
+                a[u] = ok ? M.val[base + (k + u) * kSlice] : (V)0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += a[u] * w[c[u]];
+        }
+        if (valid) {
+            P[i].x = acc;
+            v[0] += (double)r[i] * (double)acc;
+        }
     }
     double tot[1];
     if (!grid_fold<1>(v, partials, counter, tot)) return;
@@ -282,6 +461,8 @@ __global__ __launch_bounds__(kCgThreads) void llt_lo_kernel(uint32_t n, const ui
         } else if (mode == 1) {
             sc->rz = tot[0];
             sc->rz_old = tot[0];
+            const double cv = sc->crit ? tot[0] : sc->rr;
+            sc->done = (cv <= sc->tol2 || sc->iter >= sc->maxit) ? 1u : 0u;
         } else {
             sc->rz = tot[0];
         }
@@ -340,45 +521,60 @@ void clear_done(CgWork& w, hipStream_t s) {
     SA_HIP(hipStreamSynchronize(s));
 }
 
-}  // namespace
-
-void true_residual(const DevCsr<double>& A, const double* b, const double* x, double* out2, CgWork& w,
-                   hipStream_t s) {
-    w.resize(A.n);
-    hipLaunchKernelGGL(true_residual_kernel, dim3(w.blocks), dim3(kCgThreads), 0, s, A.n, A.row.p, A.col.p, A.val.p,
-                       b, x, w.partials.p, w.counter.p, w.out2.p);
-    d2h(out2, w.out2.p, 2);
+// Vectors of one solve, carved from the grow-only workspace.
+template <class V> struct CgVecs {
+    V2<V>* P[2];
+    V* r;
+    V* q;
+    V* w;
+    float* delta;
+};
+template <class V> CgVecs<V> carve(CgWork& w, uint32_t n) {
+    const size_t nv = ((size_t)n + 63) & ~(size_t)63;
+    w.vbuf.resize(nv * (2 * sizeof(V2<V>) + 3 * sizeof(V) + sizeof(float)) + 256);
+    uint8_t* p = w.vbuf.p;
+    CgVecs<V> v;
+    v.P[0] = reinterpret_cast<V2<V>*>(p);
+    p += nv * sizeof(V2<V>);
+    v.P[1] = reinterpret_cast<V2<V>*>(p);
+    p += nv * sizeof(V2<V>);
+    v.r = reinterpret_cast<V*>(p);
+    p += nv * sizeof(V);
+    v.q = reinterpret_cast<V*>(p);
+    p += nv * sizeof(V);
+    v.w = reinterpret_cast<V*>(p);
+    p += nv * sizeof(V);
+    v.delta = reinterpret_cast<float*>(p);
+    return v;
 }
 
 // ---------------------------------------------------------------------------
-// Generic driver.  VV: vector precision, MV: matrix precision of the
-// iteration, XV: accumulator of the solution updates (double: x itself;
-// float: delta merged into x64 at replacements).
+// Driver.  V: iteration precision (matrix values, vectors); the solution
+// updates go to x itself (V = double) or to an FP32 correction merged into
+// x at each residual replacement (V = float).
 // ---------------------------------------------------------------------------
-template <class VV, class MV, class XV, int PREC, bool DIFF = false>
-static void cg_drive(const DevCsr<double>& A64, const uint32_t* row, const uint32_t* col, const MV* val,
-                     const VV* idiag, const LowerFactor<VV>* L, const double* b, double* x, const CgOptions& o,
-                     CgWork& w, SolveStats& st, hipStream_t s) {
+template <class V, bool DIFF, int PREC>
+void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const LowerFactor<V>* L, const double* b,
+              double* x, const CgOptions& o, CgWork& w, SolveStats& st, hipStream_t s) {
+    using XV = V;
     const uint32_t n = A64.n;
     w.resize(n);
     const dim3 grid(w.blocks), blk(kCgThreads);
-    // vectors carved from the grow-only workspace: r z d0 d1 q w delta
-    const size_t nv = ((size_t)n + 63) & ~(size_t)63;
-    w.vbuf.resize(7 * nv * sizeof(double));
-    VV* base = reinterpret_cast<VV*>(w.vbuf.p);
-    VV* r = base;
-    VV* z = base + nv;
-    VV* d0 = base + 2 * nv;
-    VV* d1 = base + 3 * nv;
-    VV* q = base + 4 * nv;
-    VV* wv = base + 5 * nv;
-    float* delta = reinterpret_cast<float*>(base + 6 * nv);
+    CgVecs<V> vv = carve<V>(w, n);
     XV* xacc;
-    if constexpr (sizeof(XV) == 4) {
-        fill<float>(delta, n, 0.0f, s);
-        xacc = (XV*)delta;
+    if constexpr (sizeof(V) == 4) {
+        fill<float>(vv.delta, n, 0.0f, s);
+        xacc = (XV*)vv.delta;
     } else {
         xacc = (XV*)x;
+    }
+    const SellArgs<V> MA = sell_args(S);
+    SellArgs<V> ML{}, MU{};
+    if constexpr (PREC == kPrecLLt) {
+        ML = sell_args(L->lo);
+        MU = sell_args(L->up);
+        ML.diag = L->diag.p;
+        MU.diag = L->diag.p;
     }
     hipEvent_t e0, e1;
     SA_HIP(hipEventCreate(&e0));
@@ -387,20 +583,20 @@ static void cg_drive(const DevCsr<double>& A64, const uint32_t* row, const uint3
 
     publish_options(w, o, s);
     const double rtol2 = o.abs_tol2 >= 0.0 ? -o.abs_tol2 : o.rtol * o.rtol;
-    auto residual = [&](int mode) {
-        hipLaunchKernelGGL((cg_residual_kernel<VV, PREC>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p, b,
-                           x, r, z, d0, idiag, w.sc.p, w.partials.p, w.counter.p, mode, rtol2);
+    auto residual = [&](int mode, uint32_t it) {
+        V2<V>* cur = vv.P[it & 1];
+        hipLaunchKernelGGL((cg_residual_kernel<V, PREC>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p, b, x,
+                           vv.r, cur, idiag, w.sc.p, w.partials.p, w.counter.p, mode, rtol2);
         if constexpr (PREC == kPrecLLt) {
-            hipLaunchKernelGGL((llt_up_kernel<VV>), grid, blk, 0, s, n, L->up_row.p, L->up_col.p, L->up_val.p,
-                               L->diag.p, r, wv, w.sc.p, 1);
-            hipLaunchKernelGGL((llt_lo_kernel<VV>), grid, blk, 0, s, n, L->lo_row.p, L->lo_col.p, L->lo_val.p,
-                               L->diag.p, wv, r, z, w.sc.p, w.partials.p, w.counter.p, mode == 0 ? 1 : 2);
+            hipLaunchKernelGGL((llt_up_kernel<V>), grid, blk, 0, s, MU, vv.r, vv.w, w.sc.p, 1);
+            hipLaunchKernelGGL((llt_lo_kernel<V>), grid, blk, 0, s, ML, vv.w, vv.r, cur, w.sc.p, w.partials.p,
+                               w.counter.p, mode == 0 ? 1 : 2);
         }
     };
     auto merge = [&] {
-        if constexpr (sizeof(XV) == 4) hipLaunchKernelGGL(merge_delta_kernel, grid, blk, 0, s, n, x, delta);
+        if constexpr (sizeof(V) == 4) hipLaunchKernelGGL(merge_delta_kernel, grid, blk, 0, s, n, x, vv.delta);
     };
-    residual(0);
+    residual(0, 0);
     CgScalars h = read_sc(w, s);
     if (o.min_one && h.done) {
         clear_done(w, s);
@@ -414,7 +610,7 @@ static void cg_drive(const DevCsr<double>& A64, const uint32_t* row, const uint3
             // the recursive residual says converged: verify with the FP64
             // true residual, which also replaces the recursive one
             merge();
-            residual(1);
+            residual(1, h.iter);
             ++reps;
             h = read_sc(w, s);
             rr_last_rep = h.aux;
@@ -424,23 +620,22 @@ static void cg_drive(const DevCsr<double>& A64, const uint32_t* row, const uint3
         const uint32_t it0 = h.iter;
         for (uint32_t k = 0; k < o.batch; ++k) {
             const uint32_t it = it0 + k;
-            VV* dold = (it & 1) ? d1 : d0;
-            VV* dnew = (it & 1) ? d0 : d1;
-            hipLaunchKernelGGL((cg_dir_kernel<MV, VV, DIFF>), grid, blk, 0, s, n, row, col, val, z, dold, dnew, q, w.sc.p,
-                               w.partials.p, w.counter.p);
-            hipLaunchKernelGGL((cg_update_kernel<VV, XV, PREC>), grid, blk, 0, s, n, xacc, r, z, dnew, q, idiag,
+            V2<V>* cur = vv.P[it & 1];
+            V2<V>* nxt = vv.P[(it + 1) & 1];
+            hipLaunchKernelGGL((cg_dir_kernel<V, DIFF>), grid, blk, 0, s, MA, cur, nxt, vv.q, w.sc.p, w.partials.p,
+                               w.counter.p);
+            hipLaunchKernelGGL((cg_update_kernel<V, XV, PREC>), grid, blk, 0, s, n, xacc, vv.r, nxt, vv.q, idiag,
                                w.sc.p, w.partials.p, w.counter.p);
             if constexpr (PREC == kPrecLLt) {
-                hipLaunchKernelGGL((llt_up_kernel<VV>), grid, blk, 0, s, n, L->up_row.p, L->up_col.p, L->up_val.p,
-                                   L->diag.p, r, wv, w.sc.p, 0);
-                hipLaunchKernelGGL((llt_lo_kernel<VV>), grid, blk, 0, s, n, L->lo_row.p, L->lo_col.p, L->lo_val.p,
-                                   L->diag.p, wv, r, z, w.sc.p, w.partials.p, w.counter.p, 0);
+                hipLaunchKernelGGL((llt_up_kernel<V>), grid, blk, 0, s, MU, vv.r, vv.w, w.sc.p, 0);
+                hipLaunchKernelGGL((llt_lo_kernel<V>), grid, blk, 0, s, ML, vv.w, vv.r, nxt, w.sc.p, w.partials.p,
+                                   w.counter.p, 0);
             }
         }
         h = read_sc(w, s);
         if (!h.done && o.replace_factor > 0 && h.rr < o.replace_factor * o.replace_factor * rr_last_rep) {
             merge();
-            residual(1);
+            residual(1, h.iter);
             ++reps;
             h = read_sc(w, s);
             rr_last_rep = h.aux;
@@ -464,84 +659,74 @@ static void cg_drive(const DevCsr<double>& A64, const uint32_t* row, const uint3
     st.ms = ms;
 }
 
-void cg_fp64(const DevCsr<double>& A, const double* idiag, const double* b, double* x, double rtol, uint32_t maxit,
-             CgWork& w, SolveStats& st, hipStream_t s) {
-    CgOptions o;
-    o.rtol = rtol;
-    o.maxit = maxit;
-    cg_drive<double, double, double, kPrecJacobi>(A, A.row.p, A.col.p, A.val.p, idiag, nullptr, b, x, o, w, st, s);
+}  // namespace
+
+void true_residual(const DevCsr<double>& A, const double* b, const double* x, double* out2, CgWork& w,
+                   hipStream_t s) {
+    w.resize(A.n);
+    hipLaunchKernelGGL(true_residual_kernel, dim3(w.blocks), dim3(kCgThreads), 0, s, A.n, A.row.p, A.col.p, A.val.p,
+                       b, x, w.partials.p, w.counter.p, w.out2.p);
+    d2h(out2, w.out2.p, 2);
 }
 
-void cg_mixed(const DevCsr<double>& A64, const DevCsr<float>& A32, const float* idiag32, const LowerFactor<float>* L,
-              int32_t prec, const double* b, double* x, double rtol, uint32_t maxit, CgWork& w, SolveStats& st,
+template <class V>
+void cg_solve(const DevCsr<double>& A64, const Sell<V>& S, bool diff, int32_t prec, const V* idiag,
+              const LowerFactor<V>* L, const double* b, double* x, const CgOptions& o, CgWork& w, SolveStats& st,
               hipStream_t s) {
-    CgOptions o;
-    o.rtol = rtol;
-    o.maxit = maxit;
-    o.replace_factor = tuning().replace_factor;
-    o.batch = tuning().batch;
-    if (prec == kPrecLLt) {
-        if (!L) throw std::invalid_argument("L L^T preconditioner requested without a factor");
-        cg_drive<float, float, float, kPrecLLt, true>(A64, A64.row.p, A64.col.p, A32.val.p, nullptr, L, b, x, o, w, st,
-                                                      s);
+    if (prec == kPrecLLt && !L) throw std::invalid_argument("L L^T preconditioner requested without a factor");
+    if (prec == kPrecJacobi && !idiag) throw std::invalid_argument("Jacobi preconditioner requested without a diagonal");
+    if (diff) {
+        if (prec == kPrecLLt)
+            cg_drive<V, true, kPrecLLt>(A64, S, idiag, L, b, x, o, w, st, s);
+        else if (prec == kPrecJacobi)
+            cg_drive<V, true, kPrecJacobi>(A64, S, idiag, L, b, x, o, w, st, s);
+        else
+            cg_drive<V, true, kPrecNone>(A64, S, idiag, L, b, x, o, w, st, s);
     } else {
-        cg_drive<float, float, float, kPrecJacobi, true>(A64, A64.row.p, A64.col.p, A32.val.p, idiag32, nullptr, b, x,
-                                                         o, w, st, s);
+        if (prec == kPrecLLt)
+            cg_drive<V, false, kPrecLLt>(A64, S, idiag, L, b, x, o, w, st, s);
+        else if (prec == kPrecJacobi)
+            cg_drive<V, false, kPrecJacobi>(A64, S, idiag, L, b, x, o, w, st, s);
+        else
+            cg_drive<V, false, kPrecNone>(A64, S, idiag, L, b, x, o, w, st, s);
     }
 }
-
-void cg_fp32_opts(const DevCsr<double>& A64, const float* A32val, const float* idiag32, int32_t prec,
-                  const double* b, double* x, const CgOptions& o, CgWork& w, SolveStats& st, hipStream_t s,
-                  bool rowsum_form) {
-    if (rowsum_form) {
-        if (prec != kPrecJacobi) throw std::invalid_argument("row-sum form is used with the Jacobi preconditioner");
-        cg_drive<float, float, float, kPrecJacobi, true>(A64, A64.row.p, A64.col.p, A32val, idiag32, nullptr, b, x, o,
-                                                         w, st, s);
-    } else if (prec == kPrecJacobi) {
-        cg_drive<float, float, float, kPrecJacobi>(A64, A64.row.p, A64.col.p, A32val, idiag32, nullptr, b, x, o, w,
-                                                   st, s);
-    } else {
-        cg_drive<float, float, float, kPrecNone>(A64, A64.row.p, A64.col.p, A32val, nullptr, nullptr, b, x, o, w, st,
-                                                 s);
-    }
-}
+template void cg_solve<float>(const DevCsr<double>&, const Sell<float>&, bool, int32_t, const float*,
+                              const LowerFactor<float>*, const double*, double*, const CgOptions&, CgWork&,
+                              SolveStats&, hipStream_t);
+template void cg_solve<double>(const DevCsr<double>&, const Sell<double>&, bool, int32_t, const double*,
+                               const LowerFactor<double>*, const double*, double*, const CgOptions&, CgWork&,
+                               SolveStats&, hipStream_t);
 
 // Average duration (ms, HIP events on the context stream) of the CG
-// direction kernel (d update + SpMV + d.q) over `reps` back-to-back
-// launches on the given system, FP64 or FP32 values.  The solver state is
-// scratch: the vectors are filled with ones first.
-double time_dir_kernel(const DevCsr<double>& A, const float* val32, bool fp32, uint32_t reps, CgWork& w,
-                       hipStream_t s) {
-    const uint32_t n = A.n;
+// direction kernel over `reps` back-to-back launches on S.  The vectors are
+// scratch (filled with ones).
+template <class V> double time_dir_kernel(const Sell<V>& S, bool diff, uint32_t reps, CgWork& w, hipStream_t s) {
+    const uint32_t n = S.n;
     w.resize(n);
-    const size_t nv = ((size_t)n + 63) & ~(size_t)63;
-    w.vbuf.resize(7 * nv * sizeof(double));
+    CgVecs<V> vv = carve<V>(w, n);
     CgScalars h{};
     h.maxit = 0xffffffffu;
     h.rz = 1.0;
     SA_HIP(hipMemcpyAsync(w.sc.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    const size_t nv = ((size_t)n + 63) & ~(size_t)63;
+    fill<V>(reinterpret_cast<V*>(vv.P[0]), 4 * nv, (V)1, s);
     const dim3 grid(w.blocks), blk(kCgThreads);
+    const SellArgs<V> M = sell_args(S);
+    auto launch = [&](uint32_t k) {
+        V2<V>* cur = vv.P[k & 1];
+        V2<V>* nxt = vv.P[(k + 1) & 1];
+        if (diff)
+            hipLaunchKernelGGL((cg_dir_kernel<V, true>), grid, blk, 0, s, M, cur, nxt, vv.q, w.sc.p, w.partials.p,
+                               w.counter.p);
+        else
+            hipLaunchKernelGGL((cg_dir_kernel<V, false>), grid, blk, 0, s, M, cur, nxt, vv.q, w.sc.p, w.partials.p,
+                               w.counter.p);
+    };
+    for (uint32_t k = 0; k < 3; ++k) launch(k);
     hipEvent_t e0, e1;
     SA_HIP(hipEventCreate(&e0));
     SA_HIP(hipEventCreate(&e1));
-    auto launch = [&](uint32_t k) {
-        if (fp32) {
-            float* base = reinterpret_cast<float*>(w.vbuf.p);
-            float *z = base, *d0 = base + nv, *d1 = base + 2 * nv, *q = base + 3 * nv;
-            hipLaunchKernelGGL((cg_dir_kernel<float, float, true>), grid, blk, 0, s, n, A.row.p, A.col.p, val32, z,
-                               (k & 1) ? d1 : d0, (k & 1) ? d0 : d1, q, w.sc.p, w.partials.p, w.counter.p);
-        } else {
-            double* base = reinterpret_cast<double*>(w.vbuf.p);
-            double *z = base, *d0 = base + nv, *d1 = base + 2 * nv, *q = base + 3 * nv;
-            hipLaunchKernelGGL((cg_dir_kernel<double, double, false>), grid, blk, 0, s, n, A.row.p, A.col.p, A.val.p, z,
-                               (k & 1) ? d1 : d0, (k & 1) ? d0 : d1, q, w.sc.p, w.partials.p, w.counter.p);
-        }
-    };
-    if (fp32)
-        fill<float>(reinterpret_cast<float*>(w.vbuf.p), 4 * nv, 1.0f, s);
-    else
-        fill<double>(reinterpret_cast<double*>(w.vbuf.p), 4 * nv, 1.0, s);
-    for (uint32_t k = 0; k < 3; ++k) launch(k);
     SA_HIP(hipEventRecord(e0, s));
     for (uint32_t k = 0; k < reps; ++k) launch(k);
     SA_HIP(hipEventRecord(e1, s));
@@ -550,7 +735,9 @@ double time_dir_kernel(const DevCsr<double>& A, const float* val32, bool fp32, u
     SA_HIP(hipEventElapsedTime(&ms, e0, e1));
     SA_HIP(hipEventDestroy(e0));
     SA_HIP(hipEventDestroy(e1));
-    return (double)ms / reps;
+    return reps ? (double)ms / reps : 0.0;
 }
+template double time_dir_kernel<float>(const Sell<float>&, bool, uint32_t, CgWork&, hipStream_t);
+template double time_dir_kernel<double>(const Sell<double>&, bool, uint32_t, CgWork&, hipStream_t);
 
 }  // namespace stream_amd

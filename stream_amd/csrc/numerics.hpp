@@ -4,17 +4,26 @@
 // linear_system.cpp:27-43, PrecJacobi preconditioner_jacobi.cpp:29-60,
 // ConjugateGradient conjugate_gradient.cpp:44-208).
 //
-// CG on the GPU keeps every scalar on the device.  One iteration is two
-// kernels:
-//   dir    : d_new = z + beta*d_old for the own rows, q = A d_new with the
-//            neighbour entries of d_new recomputed from (z, d_old), and the
-//            partial sums of d_new.q;
+// Matrix layout for the iterations: SELL-64.  The off-diagonal entries of
+// 64 consecutive rows (one wavefront, one row per lane) form a slice stored
+// column-major: entry k of row r (slice s = r/64, lane l = r%64) lives at
+// off[s] + 64k + l, so the k-th load of a wave is one contiguous 256 B
+// (FP32) or 512 B (FP64) access.  Slices are as wide as their longest row;
+// per-row lengths predicate the loads, so padding costs memory but no
+// traffic.  The diagonal is a separate array.  CSR stays the interchange
+// format (reference DeviceCSR, assembly output, FP64 true residuals).
+//
+// CG keeps every scalar on the device.  One iteration is two kernels:
+//   dir    : d = z + beta d_old on the own row, q = A d with neighbour
+//            entries of d recomputed from the interleaved {z, d_old} pair
+//            (one load per neighbour), partial d.q;
 //   update : x += alpha d, r -= alpha q, z = M^-1 r, partial r.z and r.r.
-// Each kernel folds its block partials with a last-block-done reduction
-// (partials exchanged through coherent stores), so the scalars alpha, beta
-// and the convergence flag are produced on the device; the host reads them
-// once per batch of iterations.  Dots accumulate in FP64 whatever the
-// vector precision.
+// The two vectors z and d live interleaved in two ping-pong buffers:
+// iteration k reads P[k&1] = {z_k, d_{k-1}} and writes P[(k+1)&1] =
+// {z_{k+1}, d_k}.  Each kernel folds its block partials with a
+// last-block-done reduction, so alpha, beta and the convergence flag never
+// leave the device; the host reads them once per batch of iterations.
+// Dots accumulate in FP64 whatever the vector precision.
 #pragma once
 #include "geometry.hpp"
 #include "primitives.hpp"
@@ -28,6 +37,36 @@ template <class V> struct DevCsr {
     DBuf<V> val;
 };
 
+constexpr uint32_t kSlice = 64;
+
+template <class V> struct Sell {
+    uint32_t n = 0;
+    uint32_t nslices = 0;
+    uint32_t entries = 0;    // allocated slots (with padding)
+    DBuf<uint32_t> off;      // [nslices + 1]
+    DBuf<uint16_t> width;    // [nslices]
+    DBuf<uint16_t> len;      // [n]
+    DBuf<uint32_t> col;      // [entries]
+    DBuf<V> val;             // [entries]
+    DBuf<V> diag;            // [n]
+    DBuf<uint32_t> tmp;      // build scratch
+    DBuf<uint64_t> scan_ws;
+};
+
+// Value forms of a SELL copy.
+enum SellForm : int32_t {
+    kSellStandard = 0,  // off-diagonals and diagonal as in the CSR
+    kSellRowSum = 1,    // off-diagonals; diagonal slot = FP64 row sum (difference form)
+};
+enum SellPart : int32_t { kSellOffDiag = 0, kSellLower = 1, kSellUpper = 2 };
+
+// Build a SELL copy of a CSR matrix (values from val64 or val32, one of them
+// non-null).  part selects all off-diagonals, the strict lower or the strict
+// upper triangle.  Synchronises once (allocation size).
+template <class V>
+void build_sell(uint32_t n, const uint32_t* row, const uint32_t* col, const double* val64, const float* val32,
+                SellForm form, SellPart part, Sell<V>& S, hipStream_t s);
+
 // Device-resident CG scalars (one 128-byte record).
 struct CgScalars {
     double rz;      // r.z of the current iterate
@@ -36,7 +75,7 @@ struct CgScalars {
     double rr;      // r.r (recursive residual)
     double alpha;
     double beta;
-    double tol2;    // stop when rr <= tol2 (= rtol^2 * bb)
+    double tol2;    // stop when the criterion <= tol2 (= rtol^2 * bb)
     double aux;     // r.r of the last true-residual (replacement) pass
     double bb;      // b.b
     uint32_t iter;
@@ -59,18 +98,18 @@ struct CgWork {
 // Preconditioner kind of the fused solvers.
 enum PrecKind : int32_t { kPrecNone = 0, kPrecJacobi = 1, kPrecLLt = 2 };
 
-// Learned factor: M^-1 = L L^T with L lower triangular on A's pattern.
-// lo: strictly lower entries (j < i) of each row, up: strictly upper entries
-// (j > i) holding L_ji, diag: L_ii.  Both halves in CSR of their own.
+// Factor of M^-1 = L L^T, L lower triangular on A's pattern: lo holds the
+// strictly lower entries L_ij (j < i) by row, up the strictly upper entries
+// of row i holding L_ji (the strict upper part of L^T), diag L_ii.
 template <class V> struct LowerFactor {
     uint32_t n = 0;
-    DBuf<uint32_t> lo_row, lo_col, up_row, up_col;
-    DBuf<V> lo_val, up_val, diag;
+    Sell<V> lo, up;
+    DBuf<V> diag;
 };
 
 struct SolveStats {
-    uint32_t iters = 0;        // inner iterations
-    uint32_t replacements = 0; // FP64 residual replacements (mixed)
+    uint32_t iters = 0;        // iterations
+    uint32_t replacements = 0; // FP64 residual replacements
     double true_rel = 0.0;     // ||b - A x|| / ||b|| in FP64
     double ms = 0.0;
 };
@@ -82,43 +121,30 @@ struct CgOptions {
     uint32_t crit = 0;           // 0: r.r, 1: r.z
     bool verify = true;          // FP64 true-residual verification at convergence
     bool min_one = false;        // run at least one iteration (reference loop)
-    double replace_factor = 0.0; // mixed: replace when ||r|| fell by this factor
+    double replace_factor = 0.0; // replace when ||r|| fell by this factor
     uint32_t batch = 32;         // iterations between host checks
 };
 
 // Process-wide solver knobs (set through Stream_tune).
 struct Tuning {
-    double replace_factor = 1e-1;  // mixed: residual replacement when ||r|| fell by this factor
+    double replace_factor = 3e-1;  // mixed: residual replacement when ||r|| fell by this factor
     uint32_t batch = 32;           // CG iterations between host checks
 };
 Tuning& tuning();
 
-// FP64 Jacobi-PCG to ||b - Ax|| / ||b|| <= rtol (true residual verified;
-// restarts from the true residual when the recursive one drifted).
-void cg_fp64(const DevCsr<double>& A, const double* idiag, const double* b, double* x, double rtol, uint32_t maxit,
-             CgWork& w, SolveStats& st, hipStream_t s);
-// The same driver with FP32 vectors and matrix values (A32val on A's
-// pattern) under explicit options.  rowsum_form: A32val holds the
-// off-diagonals and, in the diagonal slot, the row sum (Laplacian
-// difference form of the product).
-void cg_fp32_opts(const DevCsr<double>& A64, const float* A32val, const float* idiag32, int32_t prec,
-                  const double* b, double* x, const CgOptions& o, CgWork& w, SolveStats& st, hipStream_t s,
-                  bool rowsum_form = false);
-
-// Mixed precision: FP32 matrix, FP32 vectors and preconditioner, FP64 dots,
-// x accumulated in FP64 and the residual replaced by the FP64 true residual
-// b - A x whenever the recursive residual has dropped by `replace_factor`
-// since the last replacement (and at convergence).  prec: kPrecJacobi uses
-// idiag32; kPrecLLt uses L.
-void cg_mixed(const DevCsr<double>& A64, const DevCsr<float>& A32, const float* idiag32, const LowerFactor<float>* L,
-              int32_t prec, const double* b, double* x, double rtol, uint32_t maxit, CgWork& w, SolveStats& st,
+// CG on SELL matrix S (values V) with vectors in precision V.  A64 (CSR,
+// FP64) gives the true residuals.  x: FP64 solution (in/out).  With V =
+// float the updates accumulate in an FP32 correction merged into x at
+// every residual replacement.  diff: S holds the row-sum form.
+template <class V>
+void cg_solve(const DevCsr<double>& A64, const Sell<V>& S, bool diff, int32_t prec, const V* idiag,
+              const LowerFactor<V>* L, const double* b, double* x, const CgOptions& o, CgWork& w, SolveStats& st,
               hipStream_t s);
 
-// ||b - A x||_2 and ||b||_2 in FP64.
+// ||b - A x||_2^2 and ||b||_2^2 in FP64.
 void true_residual(const DevCsr<double>& A, const double* b, const double* x, double* out2, CgWork& w, hipStream_t s);
 
 // Average duration (ms) of the CG direction kernel over `reps` launches.
-double time_dir_kernel(const DevCsr<double>& A, const float* val32, bool fp32, uint32_t reps, CgWork& w,
-                       hipStream_t s);
+template <class V> double time_dir_kernel(const Sell<V>& S, bool diff, uint32_t reps, CgWork& w, hipStream_t s);
 
 }  // namespace stream_amd

@@ -20,6 +20,7 @@
 
 #include "ashape2d.hpp"
 #include "numerics.hpp"
+#include "precond.hpp"
 
 namespace stream_amd {
 
@@ -46,7 +47,6 @@ struct P1Args {
     uint32_t* arow;
     uint32_t* acol;
     double* aval;
-    float* aval32;
     double* idiag;
     float* idiag32;
     double* b;
@@ -139,7 +139,6 @@ __global__ __launch_bounds__(256) void p1_assemble_kernel(P1Args a) {
         }
         diag -= val[k];
     }
-    double rowsum = 0.0;
     for (uint32_t k = 0; k < cnt; ++k) {
         double v;
         if (k == dpos)
@@ -147,12 +146,7 @@ __global__ __launch_bounds__(256) void p1_assemble_kernel(P1Args a) {
         else
             v = (bi || a.bnd[col[k]]) ? 0.0 : val[k];
         val[k] = v;
-        rowsum += v;
-        a.aval32[base + k] = (float)v;
     }
-    // FP32 copy in row-sum form: the diagonal slot holds the row sum of the
-    // final row (zero on interior rows away from the Dirichlet set)
-    a.aval32[base + dpos] = (float)rowsum;
     const double dg = bi ? 1.0 : diag;
     const double id = 1.0 / dg;
     a.idiag[i] = id;
@@ -195,7 +189,10 @@ struct Step2D {
     DBuf<Sph4<double>> pts;  // input order
     DBuf<double> f_in, f_m, b, x, p_out;
     DevCsr<double> A;
-    DBuf<float> val32, idiag32;
+    Sell<double> S64;  // FP64 iteration matrix (standard form)
+    Sell<float> S32;   // FP32 iteration matrix (row-sum form)
+    LowerFactor<float> L32;  // M^-1 = L L^T (FSAI or learned)
+    DBuf<float> idiag32;
     DBuf<double> idiag;
     CgWork work;
     SolveStats last;
@@ -218,7 +215,7 @@ struct Step2D {
     }
 
     // mesh (LBVH + alpha-shape), permute the source, assemble
-    void mesh_and_assemble(int bits) {
+    void mesh_and_assemble(int bits, int solver) {
         hipStream_t s = ctx().stream;
         hipEvent_t ev[3];
         for (auto& e : ev) SA_HIP(hipEventCreate(&e));
@@ -229,6 +226,7 @@ struct Step2D {
         mesh.compute(s);
         SA_HIP(hipEventRecord(ev[1], s));
         assemble(s);
+        prepare(solver, s);
         SA_HIP(hipEventRecord(ev[2], s));
         SA_HIP(hipEventSynchronize(ev[2]));
         float a = 0, c = 0;
@@ -245,7 +243,6 @@ struct Step2D {
         A.row.resize(n + 1);
         A.col.resize(cap);
         A.val.resize(cap);
-        val32.resize(cap);
         idiag.resize(n);
         idiag32.resize(n);
         b.resize(n);
@@ -266,29 +263,42 @@ struct Step2D {
         a.arow = A.row.p;
         a.acol = A.col.p;
         a.aval = A.val.p;
-        a.aval32 = val32.p;
         a.idiag = idiag.p;
         a.idiag32 = idiag32.p;
         a.b = b.p;
         a.cap = cap;
         a.flags = ctx().d_flags;
         hipLaunchKernelGGL(p1_assemble_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, a);
+        A.nnz = read_scalar(A.row.p + n);
     }
 
-    // solver: 0 = FP64 Jacobi-PCG, 1 = mixed FP32/FP64 Jacobi-PCG
+    // SELL copy for the chosen solver (FP64 standard or FP32 row-sum form)
+    void prepare(int solver, hipStream_t s) {
+        if (solver == 0) {
+            build_sell<double>(n, A.row.p, A.col.p, A.val.p, nullptr, kSellStandard, kSellOffDiag, S64, s);
+        } else {
+            build_sell<float>(n, A.row.p, A.col.p, A.val.p, nullptr, kSellRowSum, kSellOffDiag, S32, s);
+            if (solver == 2) fsai_factor<float>(A, L32, s);
+        }
+    }
+
+    // solver: 0 = FP64 Jacobi-PCG, 1 = mixed FP32/FP64 Jacobi-PCG,
+    // 2 = mixed PCG with the FSAI L L^T factor
     void solve(int solver, double rtol, uint32_t maxit) {
         hipStream_t s = ctx().stream;
         fill<double>(x.p, n, 0.0, s);
-        A.nnz = n + mesh.cap_edges;
+        CgOptions o;
+        o.rtol = rtol;
+        o.maxit = maxit;
+        o.batch = tuning().batch;
         if (solver == 0) {
-            cg_fp64(A, idiag.p, b.p, x.p, rtol, maxit, work, last, s);
+            cg_solve<double>(A, S64, false, kPrecJacobi, idiag.p, nullptr, b.p, x.p, o, work, last, s);
         } else if (solver == 1) {
-            CgOptions o;
-            o.rtol = rtol;
-            o.maxit = maxit;
             o.replace_factor = tuning().replace_factor;
-            o.batch = tuning().batch;
-            cg_fp32_opts(A, val32.p, idiag32.p, kPrecJacobi, b.p, x.p, o, work, last, s, true);
+            cg_solve<float>(A, S32, true, kPrecJacobi, idiag32.p, nullptr, b.p, x.p, o, work, last, s);
+        } else if (solver == 2) {
+            o.replace_factor = tuning().replace_factor;
+            cg_solve<float>(A, S32, true, kPrecLLt, nullptr, &L32, b.p, x.p, o, work, last, s);
         } else {
             throw std::invalid_argument("unknown solver kind");
         }
@@ -323,6 +333,26 @@ struct d_CSR {
     DevCsr<double> A;  // row/col always; val when f64
     DBuf<float> val32;
     bool f64 = false;
+    Sell<float> S32;   // standard form, from the FP32 values
+    Sell<double> S64;  // standard form, from the FP64 values
+    bool s32_ok = false, s64_ok = false;
+    const Sell<float>& sell32() {
+        if (!s32_ok) {
+            build_sell<float>(A.n, A.row.p, A.col.p, nullptr, val32.p, kSellStandard, kSellOffDiag, S32,
+                              ctx().stream);
+            s32_ok = true;
+        }
+        return S32;
+    }
+    const Sell<double>& sell64() {
+        ensure_f64();
+        if (!s64_ok) {
+            build_sell<double>(A.n, A.row.p, A.col.p, A.val.p, nullptr, kSellStandard, kSellOffDiag, S64,
+                               ctx().stream);
+            s64_ok = true;
+        }
+        return S64;
+    }
     void ensure_f64() {
         if (f64) return;
         // widen the FP32 values exactly
@@ -447,6 +477,7 @@ void d_csr_h2d(d_CSR* d, const h_CSR* h) {
         const auto v32 = h_vec_f32(h);
         h2d(d->val32.p, v32.data(), nnz);
         d->f64 = h->f64;
+        d->s32_ok = d->s64_ok = false;
         if (h->f64) {
             d->A.val.resize(nnz ? nnz : 1);
             h2d(d->A.val.p, h->val64.data(), nnz);
@@ -522,15 +553,16 @@ static uint32_t ref_solve(SolverCG* sol, const LinSys* sys, const PrecJacobi_st*
         sol->x.resize(n ? n : 1);
         std::vector<double> xd(x, x + n);
         h2d(sol->x.p, xd.data(), n);
-        const_cast<d_CSR*>(A)->ensure_f64();
+        d_CSR* Am = const_cast<d_CSR*>(A);
+        Am->ensure_f64();
         CgOptions o;
         o.abs_tol2 = 1e-12;
         o.maxit = 10001;
         o.crit = prec ? 1u : 0u;
         o.verify = false;
         o.min_one = true;
-        cg_fp32_opts(A->A, A->val32.p, prec ? prec->idiag32.p : nullptr, prec ? kPrecJacobi : kPrecNone, sys->b.p,
-                     sol->x.p, o, sol->work, sol->last, ctx().stream);
+        cg_solve<float>(Am->A, Am->sell32(), false, prec ? kPrecJacobi : kPrecNone, prec ? prec->idiag32.p : nullptr,
+                        nullptr, sys->b.p, sol->x.p, o, sol->work, sol->last, ctx().stream);
         d2h(xd.data(), sol->x.p, n);
         for (uint32_t i = 0; i < n; ++i) x[i] = (float)xd[i];
         ret = sol->last.iters >= 10001 ? 10001u : sol->last.iters - 1;
@@ -559,7 +591,13 @@ int32_t SolverCG_solve_rel_f64(SolverCG* sol, const LinSys* sys, const PrecJacob
         const_cast<d_CSR*>(A)->ensure_f64();
         sol->x.resize(n ? n : 1);
         h2d(sol->x.p, x, n);
-        cg_fp64(A->A, prec->idiag.p, sys->b.p, sol->x.p, rtol, maxit, sol->work, sol->last, ctx().stream);
+        CgOptions o;
+        o.rtol = rtol;
+        o.maxit = maxit;
+        o.batch = tuning().batch;
+        d_CSR* Am = const_cast<d_CSR*>(A);
+        cg_solve<double>(Am->A, Am->sell64(), false, kPrecJacobi, prec->idiag.p, nullptr, sys->b.p, sol->x.p, o,
+                         sol->work, sol->last, ctx().stream);
         d2h(x, sol->x.p, n);
         if (stats) {
             stats[0] = sol->last.iters;
@@ -585,8 +623,9 @@ int32_t SolverCG_solve_mixed(SolverCG* sol, const LinSys* sys, const PrecJacobi_
         o.maxit = maxit;
         o.replace_factor = tuning().replace_factor;
         o.batch = tuning().batch;
-        cg_fp32_opts(A->A, A->val32.p, prec->idiag32.p, kPrecJacobi, sys->b.p, sol->x.p, o, sol->work, sol->last,
-                     ctx().stream);
+        d_CSR* Am = const_cast<d_CSR*>(A);
+        cg_solve<float>(Am->A, Am->sell32(), false, kPrecJacobi, prec->idiag32.p, nullptr, sys->b.p, sol->x.p, o,
+                        sol->work, sol->last, ctx().stream);
         d2h(x, sol->x.p, n);
         if (stats) {
             stats[0] = sol->last.iters;
@@ -650,11 +689,11 @@ int32_t PfemStep2D_set_morton_bits(PfemStep2D* h, int32_t bits) {
 }
 
 // One step: mesh + assemble + solve.  solver: 0 = FP64 Jacobi-PCG,
-// 1 = mixed FP32/FP64 Jacobi-PCG.  stats [8]: mesh ms, assembly ms, solve
+// 1 = mixed FP32/FP64 Jacobi-PCG, 2 = mixed PCG with the FSAI L L^T factor.  stats [8]: mesh ms, assembly ms, solve
 // ms, iterations, true relative residual, replacements, nnz, n_elems.
 int32_t PfemStep2D_run(PfemStep2D* h, int32_t solver, double rtol, uint32_t maxit, double* stats) {
     return g([&] {
-        h->s.mesh_and_assemble(h->bits);
+        h->s.mesh_and_assemble(h->bits, solver);
         h->s.solve(solver, rtol, maxit);
         if (stats) {
             stats[0] = h->s.t_mesh;
@@ -682,7 +721,11 @@ int32_t PfemStep2D_run(PfemStep2D* h, int32_t solver, double rtol, uint32_t maxi
 int32_t PfemStep2D_time_dir_kernel(PfemStep2D* h, int32_t fp32, uint32_t reps, double* ms) {
     return g([&] {
         CgWork w;
-        *ms = time_dir_kernel(h->s.A, h->s.val32.p, fp32 != 0, reps, w, ctx().stream);
+        h->s.prepare(fp32 ? 1 : 0, ctx().stream);
+        if (fp32)
+            *ms = time_dir_kernel<float>(h->s.S32, true, reps, w, ctx().stream);
+        else
+            *ms = time_dir_kernel<double>(h->s.S64, false, reps, w, ctx().stream);
     });
 }
 

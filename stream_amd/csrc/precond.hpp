@@ -1,0 +1,32 @@
+// Factored approximate-inverse preconditioners M^-1 = L L^T with L lower
+// triangular on the pattern of A (no counterpart in the reference, whose
+// only preconditioner is Jacobi: include/numerics/preconditioner.hpp:60-72).
+//
+// Two producers of L:
+//   * classical FSAI (Kolotilina-Yeremin) on the upper pattern: for row i,
+//     P = {i} U {j > i : A_ij in pattern}; solve A[P,P] g = e_i exactly in
+//     FP64 (dense Cholesky, |P| <= kFsaiMax) and scale g by 1/sqrt(g_i).
+//     Then G (upper, rows g) satisfies diag(G A G^T) = 1 and L = G^T.
+//   * the learned message-passing network (mpnn.hip), which writes the same
+//     slots.
+// Storage (LowerFactor): up = strict upper entries of row i (L_ji, j > i),
+// lo = strict lower entries of row i (L_ij, j < i), diag = L_ii; both as
+// SELL-64 copies of A's pattern, so the two half products of the apply are
+// coalesced like the SpMV.
+#pragma once
+#include "numerics.hpp"
+
+namespace stream_amd {
+
+constexpr uint32_t kFsaiMax = 16;
+
+// Build the lo/up SELL structures of L on A's pattern (values zeroed).
+template <class V> void factor_structure(const DevCsr<double>& A, LowerFactor<V>& L, hipStream_t s);
+
+// Fill L with the classical FSAI factor of A (FP64 solves, stored in V).
+template <class V> void fsai_factor(const DevCsr<double>& A, LowerFactor<V>& L, hipStream_t s);
+
+// Copy the up values (and diag) into their transposed lo slots.
+template <class V> void mirror_lower(LowerFactor<V>& L, hipStream_t s);
+
+}  // namespace stream_amd

@@ -272,6 +272,11 @@ bool radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, ui
     return (passes % 2) == 1;
 }
 
+__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ a, uint4* __restrict__ b, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
 }  // namespace stream_amd
 
 using namespace stream_amd;
@@ -302,6 +307,34 @@ int32_t Stream_sort_pairs_u64(uint32_t n, const uint64_t* keys, const uint32_t* 
         d2h(keys_out, in1 ? k1.p : k0.p, n);
         d2h(vals_out, in1 ? v1.p : v0.p, n);
         if (read_scalar(ctx().d_flags + kFlagError) != 0) throw std::runtime_error("device error flag raised");
+    });
+}
+
+// Plain device copy of `nbytes` (multiple of 16) repeated `reps` times: the
+// calibration kernel for hardware byte counters (its algorithmic traffic is
+// exactly nbytes read + nbytes written per launch).  Returns the average ms.
+int32_t Stream_copy_calibration(uint64_t nbytes, uint32_t reps, double* ms) {
+    return guarded([&] {
+        if (nbytes % 16 != 0 || nbytes == 0) throw std::invalid_argument("nbytes must be a positive multiple of 16");
+        hipStream_t s = ctx().stream;
+        DBuf<uint4> a, b;
+        const size_t n = nbytes / 16;
+        a.resize(n);
+        b.resize(n);
+        fill<uint4>(a.p, n, make_uint4(1, 2, 3, 4), s);
+        hipEvent_t e0, e1;
+        SA_HIP(hipEventCreate(&e0));
+        SA_HIP(hipEventCreate(&e1));
+        SA_HIP(hipEventRecord(e0, s));
+        for (uint32_t k = 0; k < reps; ++k)
+            hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, a.p, b.p, n);
+        SA_HIP(hipEventRecord(e1, s));
+        SA_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        SA_HIP(hipEventElapsedTime(&t, e0, e1));
+        SA_HIP(hipEventDestroy(e0));
+        SA_HIP(hipEventDestroy(e1));
+        *ms = reps ? (double)t / reps : 0.0;
     });
 }
 

@@ -31,8 +31,10 @@ PfemStep2D_time_dir_kernel = bind(lib, "PfemStep2D_time_dir_kernel", i32,
 
 SOLVER_FP64_JACOBI = 0
 SOLVER_MIXED_JACOBI = 1
-SOLVER_MIXED_LEARNED = 2
-SOLVER_NAMES = {"fp64": SOLVER_FP64_JACOBI, "mixed": SOLVER_MIXED_JACOBI, "learned": SOLVER_MIXED_LEARNED}
+SOLVER_MIXED_FSAI = 2
+SOLVER_MIXED_LEARNED = 3
+SOLVER_NAMES = {"fp64": SOLVER_FP64_JACOBI, "mixed": SOLVER_MIXED_JACOBI, "fsai": SOLVER_MIXED_FSAI,
+                "learned": SOLVER_MIXED_LEARNED}
 
 
 class PfemStep2D:
