@@ -69,10 +69,22 @@ void PfemStep2D_destroy(PfemStep2D* step);
 int32_t PfemStep2D_set_points(PfemStep2D* step, uint32_t n, const double* xy, const double* alpha, const double* f);
 int32_t PfemStep2D_set_morton_bits(PfemStep2D* step, int32_t bits);
 /* solver 0 = FP64 Jacobi-PCG, 1 = mixed FP32/FP64 Jacobi-PCG, 2 = mixed PCG
- * with the FSAI L L^T factor.  stats [8]:
+ * with the FSAI L L^T factor, 3 = mixed PCG with the learned L L^T factor
+ * (stream_learned.h; Jacobi when the factor is rejected).  stats [10]:
  * mesh ms, assembly ms, solve ms, iterations, true relative residual,
- * replacements, nnz, triangles. */
+ * replacements, nnz, triangles, preconditioner set-up ms, 1 if the learned
+ * factor was rejected. */
 int32_t PfemStep2D_run(PfemStep2D* step, int32_t solver, double rtol, uint32_t maxit, double* stats);
+/* Mesh + assemble only; stats [2] (nullable): mesh ms, assembly ms. */
+int32_t PfemStep2D_assemble(PfemStep2D* step, double* stats);
+/* Device source f = sum (ax kx + ay ky) cos(kx x + ky y + phi), modes
+ * [n][5] rows (kx, ky, phi, ax, ay), n <= 32, re-evaluated after advection. */
+int32_t PfemStep2D_set_source_modes(PfemStep2D* step, uint32_t n_modes, const double* modes);
+/* `steps` forward-Euler steps of u = (dpsi/dy, -dpsi/dx), psi = sum a sin(kx x
+ * + ky y + phi), modes [n][4] rows (kx, ky, phi, a), n <= 32. */
+int32_t PfemStep2D_advect(PfemStep2D* step, uint32_t n_modes, const double* modes, double dt, uint32_t steps);
+/* current points [N][2], input order */
+int32_t PfemStep2D_get_points(PfemStep2D* step, double* xy);
 int32_t PfemStep2D_get_solution(PfemStep2D* step, double* p);
 int32_t PfemStep2D_get_solution_morton(PfemStep2D* step, double* p);
 uint32_t PfemStep2D_nnz(PfemStep2D* step);

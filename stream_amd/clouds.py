@@ -104,3 +104,35 @@ def divergence_source(xy: np.ndarray, modes: np.ndarray) -> np.ndarray:
     for kx, ky, phi, ax, ay in modes:
         f += (ax * kx + ay * ky) * np.cos(kx * xy[:, 0] + ky * xy[:, 1] + phi)
     return f
+
+
+ADVECT_SEED, ADVECT_MODES, ADVECT_DT, ADVECT_STEPS = 33, 16, 1e-3, 200
+
+
+def stream_modes(seed: int = ADVECT_SEED, n_modes: int = ADVECT_MODES, speed: float = 0.25):
+    """Stream function psi = sum_m a_m sin(k_m . x + phi_m) for the advection
+    sequence: wave numbers pi*(1 + floor(4u)) per axis, phases 2*pi*u and
+    a_m = speed / (n_modes |k_m|), so each mode's velocity amplitude is
+    speed / n_modes and |u| <= speed.  Rows (kx, ky, phi, a)."""
+    u = uniform(seed, 3 * n_modes).reshape(n_modes, 3)
+    kx = np.pi * (1.0 + np.floor(4.0 * u[:, 0]))
+    ky = np.pi * (1.0 + np.floor(4.0 * u[:, 1]))
+    phi = 2.0 * np.pi * u[:, 2]
+    a = speed / (n_modes * np.hypot(kx, ky))
+    return np.stack([kx, ky, phi, a], axis=1)
+
+
+def advect(xy: np.ndarray, modes: np.ndarray, dt: float, steps: int = 1) -> np.ndarray:
+    """Forward Euler in u = (dpsi/dy, -dpsi/dx) (host restatement of the
+    device advection, for tests)."""
+    p = np.array(xy, np.float64, copy=True)
+    for _ in range(steps):
+        ux = np.zeros(p.shape[0])
+        uy = np.zeros(p.shape[0])
+        for kx, ky, phi, a in modes:
+            c = a * np.cos(kx * p[:, 0] + ky * p[:, 1] + phi)
+            ux += ky * c
+            uy -= kx * c
+        p[:, 0] += dt * ux
+        p[:, 1] += dt * uy
+    return p

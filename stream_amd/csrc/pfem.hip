@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "ashape2d.hpp"
+#include "mpnn.hpp"
 #include "numerics.hpp"
 #include "precond.hpp"
 
@@ -179,6 +180,43 @@ __global__ void pack_spheres_kernel(const double* __restrict__ xy, const double*
     out[i] = s;
 }
 
+// Fourier modes on the device: stream function psi = sum a sin(kx x + ky y
+// + phi) (advection) and source f = sum (ax kx + ay ky) cos(kx x + ky y + phi).
+struct Modes {
+    uint32_t n = 0;
+    double m[32][5];
+};
+
+// One forward-Euler step x += dt u(x), u = (dpsi/dy, -dpsi/dx).
+__global__ __launch_bounds__(256) void advect_kernel(uint32_t n, Modes md, double dt, Sph4<double>* __restrict__ pts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Sph4<double> p = pts[i];
+    double ux = 0.0, uy = 0.0;
+    for (uint32_t k = 0; k < md.n; ++k) {
+        const double kx = md.m[k][0], ky = md.m[k][1], ph = md.m[k][2], a = md.m[k][3];
+        const double c = a * cos(kx * p.c[0] + ky * p.c[1] + ph);
+        ux += ky * c;
+        uy -= kx * c;
+    }
+    p.c[0] += dt * ux;
+    p.c[1] += dt * uy;
+    pts[i] = p;
+}
+
+__global__ __launch_bounds__(256) void source_kernel(uint32_t n, Modes md, const Sph4<double>* __restrict__ pts,
+                                                     double* __restrict__ f) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Sph4<double> p = pts[i];
+    double s = 0.0;
+    for (uint32_t k = 0; k < md.n; ++k) {
+        const double kx = md.m[k][0], ky = md.m[k][1], ph = md.m[k][2];
+        s += (md.m[k][3] * kx + md.m[k][4] * ky) * cos(kx * p.c[0] + ky * p.c[1] + ph);
+    }
+    f[i] = s;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -196,8 +234,37 @@ struct Step2D {
     DBuf<double> idiag;
     CgWork work;
     SolveStats last;
-    double t_mesh = 0, t_asm = 0, t_solve = 0;
+    double t_mesh = 0, t_asm = 0, t_prep = 0, t_solve = 0;
     uint32_t n = 0;
+    PrecLearned* learned = nullptr;  // network of solver 3 (not owned)
+    bool learned_fallback = false;   // the last learned factor was rejected
+    Modes source;                    // device source f(x) (n = 0: host-given f)
+
+    // Point the network's graph at this step's system (after assembly).
+    void bind_graph(MpnnGraph& g, hipStream_t s) {
+        g.n = n;
+        g.nnz = A.nnz;
+        g.row = A.row.p;
+        g.col = A.col.p;
+        g.val = A.val.p;
+        g.xy = reinterpret_cast<const double*>(mesh.lbvh.obj_m.p);  // Sph4: c[0], c[1] first
+        g.xy_stride = sizeof(Sph4<double>) / sizeof(double);
+        g.bnd = mesh.node_is_bnd.p;
+        mpnn_prepare(g, s);
+    }
+
+    void advect(const Modes& md, double dt, uint32_t steps) {
+        hipStream_t s = ctx().stream;
+        for (uint32_t k = 0; k < steps; ++k)
+            hipLaunchKernelGGL(advect_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, md, dt, pts.p);
+        if (source.n) update_source(s);
+        SA_HIP(hipStreamSynchronize(s));
+    }
+
+    void update_source(hipStream_t s) {
+        f_in.resize(n ? n : 1);
+        hipLaunchKernelGGL(source_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, source, pts.p, f_in.p);
+    }
 
     void set_points(uint32_t n_, const double* h_xy, const double* h_alpha, const double* h_f) {
         n = n_;
@@ -214,10 +281,11 @@ struct Step2D {
         SA_HIP(hipStreamSynchronize(s));
     }
 
-    // mesh (LBVH + alpha-shape), permute the source, assemble
+    // mesh (LBVH + alpha-shape), permute the source, assemble, and build the
+    // solver's matrix copy and preconditioner (solver < 0: none)
     void mesh_and_assemble(int bits, int solver) {
         hipStream_t s = ctx().stream;
-        hipEvent_t ev[3];
+        hipEvent_t ev[4];
         for (auto& e : ev) SA_HIP(hipEventCreate(&e));
         SA_HIP(hipEventRecord(ev[0], s));
         mesh.bits = bits;
@@ -226,14 +294,17 @@ struct Step2D {
         mesh.compute(s);
         SA_HIP(hipEventRecord(ev[1], s));
         assemble(s);
-        prepare(solver, s);
         SA_HIP(hipEventRecord(ev[2], s));
-        SA_HIP(hipEventSynchronize(ev[2]));
-        float a = 0, c = 0;
+        if (solver >= 0) prepare(solver, s);
+        SA_HIP(hipEventRecord(ev[3], s));
+        SA_HIP(hipEventSynchronize(ev[3]));
+        float a = 0, c = 0, d = 0;
         SA_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
         SA_HIP(hipEventElapsedTime(&c, ev[1], ev[2]));
+        SA_HIP(hipEventElapsedTime(&d, ev[2], ev[3]));
         t_mesh = a;
         t_asm = c;
+        t_prep = d;
         for (auto& e : ev) SA_HIP(hipEventDestroy(e));
     }
 
@@ -273,18 +344,36 @@ struct Step2D {
     }
 
     // SELL copy for the chosen solver (FP64 standard or FP32 row-sum form)
+    // and its factor.  The learned factor is rejected (Jacobi is used
+    // instead) when any of its values is non-finite or its diagonal is not
+    // positive.
     void prepare(int solver, hipStream_t s) {
+        learned_fallback = false;
         if (solver == 0) {
             build_sell<double>(n, A.row.p, A.col.p, A.val.p, nullptr, kSellStandard, kSellOffDiag, S64, s);
         } else {
             build_sell<float>(n, A.row.p, A.col.p, A.val.p, nullptr, kSellRowSum, kSellOffDiag, S32, s);
             if (solver == 2) fsai_factor<float>(A, L32, s);
+            if (solver == 3) {
+                if (!learned) throw std::invalid_argument("solver 3 needs a network (PfemStep2D_set_learned)");
+                factor_structure<float>(A, L32, s);
+                bind_graph(learned->graph, s);
+                learned->net.forward(learned->graph, s);
+                learned->net.pack(learned->graph, L32, s);
+                const uint32_t bad = read_scalar(ctx().d_flags + kFlagLearned);
+                if (bad) {
+                    fill<uint32_t>(ctx().d_flags + kFlagLearned, 1, 0u, s);
+                    learned_fallback = true;
+                }
+            }
         }
     }
 
     // solver: 0 = FP64 Jacobi-PCG, 1 = mixed FP32/FP64 Jacobi-PCG,
-    // 2 = mixed PCG with the FSAI L L^T factor
+    // 2 = mixed PCG with the FSAI L L^T factor, 3 = mixed PCG with the
+    // learned L L^T factor (Jacobi when the factor was rejected)
     void solve(int solver, double rtol, uint32_t maxit) {
+        if (solver == 3 && learned_fallback) solver = 1;
         hipStream_t s = ctx().stream;
         fill<double>(x.p, n, 0.0, s);
         CgOptions o;
@@ -296,7 +385,7 @@ struct Step2D {
         } else if (solver == 1) {
             o.replace_factor = tuning().replace_factor;
             cg_solve<float>(A, S32, true, kPrecJacobi, idiag32.p, nullptr, b.p, x.p, o, work, last, s);
-        } else if (solver == 2) {
+        } else if (solver == 2 || solver == 3) {
             o.replace_factor = tuning().replace_factor;
             cg_solve<float>(A, S32, true, kPrecLLt, nullptr, &L32, b.p, x.p, o, work, last, s);
         } else {
@@ -688,11 +777,92 @@ int32_t PfemStep2D_set_morton_bits(PfemStep2D* h, int32_t bits) {
     });
 }
 
+// Fourier modes [n_modes][5] for the device source f = sum (ax kx + ay ky)
+// cos(kx x + ky y + phi), rows (kx, ky, phi, ax, ay); evaluated now at the
+// current points and again after every advection.  n_modes = 0 returns to
+// the host-given f of PfemStep2D_set_points.
+int32_t PfemStep2D_set_source_modes(PfemStep2D* h, uint32_t n_modes, const double* modes) {
+    return g([&] {
+        if (n_modes > 32 || (n_modes && !modes)) throw std::invalid_argument("at most 32 source modes");
+        h->s.source.n = n_modes;
+        std::memcpy(h->s.source.m, modes, sizeof(double) * 5 * n_modes);
+        if (n_modes) {
+            h->s.update_source(ctx().stream);
+            SA_HIP(hipStreamSynchronize(ctx().stream));
+        }
+    });
+}
+
+// Move the points `steps` forward-Euler steps of dt in the divergence-free
+// velocity u = (dpsi/dy, -dpsi/dx) of psi = sum a sin(kx x + ky y + phi),
+// modes [n_modes][4] rows (kx, ky, phi, a).
+int32_t PfemStep2D_advect(PfemStep2D* h, uint32_t n_modes, const double* modes, double dt, uint32_t steps) {
+    return g([&] {
+        if (n_modes > 32 || (n_modes && !modes)) throw std::invalid_argument("at most 32 velocity modes");
+        Modes md;
+        md.n = n_modes;
+        for (uint32_t k = 0; k < n_modes; ++k) {
+            for (int c = 0; c < 4; ++c) md.m[k][c] = modes[4 * k + c];
+            md.m[k][4] = 0.0;
+        }
+        h->s.advect(md, dt, steps);
+    });
+}
+
+// Current point coordinates [N][2] (input order).
+int32_t PfemStep2D_get_points(PfemStep2D* h, double* xy) {
+    return g([&] {
+        std::vector<Sph4<double>> t(h->s.n);
+        d2h(t.data(), h->s.pts.p, h->s.n);
+        for (uint32_t i = 0; i < h->s.n; ++i) {
+            xy[2 * i] = t[i].c[0];
+            xy[2 * i + 1] = t[i].c[1];
+        }
+    });
+}
+
+// Network used by solver 3 (not owned; null detaches).
+int32_t PfemStep2D_set_learned(PfemStep2D* h, PrecLearned* net) {
+    return g([&] { h->s.learned = net; });
+}
+
+// Mesh and assemble without solving (the system is then available to
+// PfemStep2D_get_system and PrecLearned_bind_step).  stats [2] (nullable):
+// mesh ms, assembly ms.
+int32_t PfemStep2D_assemble(PfemStep2D* h, double* stats) {
+    return g([&] {
+        h->s.mesh_and_assemble(h->bits, -1);
+        if (stats) {
+            stats[0] = h->s.t_mesh;
+            stats[1] = h->s.t_asm;
+        }
+        const uint32_t fl = read_scalar(ctx().d_flags + kFlagError);
+        if (fl) {
+            fill<uint32_t>(ctx().d_flags, 1, 0u, ctx().stream);
+            throw std::runtime_error("device error flag raised during the step");
+        }
+    });
+}
+
+// Bind the network's training/inference graph to the step's current
+// system (device-resident, no copies).
+int32_t PrecLearned_bind_step(PrecLearned* net, PfemStep2D* h) {
+    return g([&] {
+        if (!net || !h || !h->s.A.nnz) throw std::invalid_argument("PrecLearned_bind_step: no assembled system");
+        h->s.bind_graph(net->graph, ctx().stream);
+        mpnn_check_flags();
+    });
+}
+
 // One step: mesh + assemble + solve.  solver: 0 = FP64 Jacobi-PCG,
-// 1 = mixed FP32/FP64 Jacobi-PCG, 2 = mixed PCG with the FSAI L L^T factor.  stats [8]: mesh ms, assembly ms, solve
-// ms, iterations, true relative residual, replacements, nnz, n_elems.
+// 1 = mixed FP32/FP64 Jacobi-PCG, 2 = mixed PCG with the FSAI L L^T factor,
+// 3 = mixed PCG with the learned L L^T factor.  stats [10]: mesh ms,
+// assembly ms, solve ms, iterations, true relative residual, replacements,
+// nnz, n_elems, preconditioner set-up ms (matrix copy + factor), 1 when the
+// learned factor was rejected and Jacobi used.
 int32_t PfemStep2D_run(PfemStep2D* h, int32_t solver, double rtol, uint32_t maxit, double* stats) {
     return g([&] {
+        if (solver < 0 || solver > 3) throw std::invalid_argument("unknown solver kind");
         h->s.mesh_and_assemble(h->bits, solver);
         h->s.solve(solver, rtol, maxit);
         if (stats) {
@@ -704,6 +874,8 @@ int32_t PfemStep2D_run(PfemStep2D* h, int32_t solver, double rtol, uint32_t maxi
             stats[5] = h->s.last.replacements;
             stats[6] = read_scalar(h->s.A.row.p + h->s.n);
             stats[7] = h->s.mesh.n_elems();
+            stats[8] = h->s.t_prep;
+            stats[9] = h->s.learned_fallback ? 1.0 : 0.0;
         }
         const uint32_t fl = read_scalar(ctx().d_flags + kFlagError);
         if (fl) {

@@ -26,6 +26,12 @@ PfemStep2D_get_system = bind(lib, "PfemStep2D_get_system", i32,
 PfemStep2D_get_mesh = bind(lib, "PfemStep2D_get_mesh", i32,
                            [_p, opt(np.uint32), opt(np.uint8), opt(np.float64), opt(np.uint32)])
 
+PfemStep2D_assemble = bind(lib, "PfemStep2D_assemble", i32, [_p, opt(np.float64)])
+PfemStep2D_set_source_modes = bind(lib, "PfemStep2D_set_source_modes", i32, [_p, u32, opt(np.float64, False)])
+PfemStep2D_advect = bind(lib, "PfemStep2D_advect", i32, [_p, u32, opt(np.float64, False), f64, u32])
+PfemStep2D_get_points = bind(lib, "PfemStep2D_get_points", i32, [_p, arr(np.float64, 2, writeable=True)])
+PfemStep2D_set_learned = bind(lib, "PfemStep2D_set_learned", i32, [_p, _p])
+
 PfemStep2D_time_dir_kernel = bind(lib, "PfemStep2D_time_dir_kernel", i32,
                                   [_p, i32, u32, ctypes.POINTER(ctypes.c_double)])
 
@@ -54,11 +60,41 @@ class PfemStep2D:
             self.h = None
 
     def run(self, solver="fp64", rtol=1e-8, maxit=200000):
-        st = np.zeros(8)
+        st = np.zeros(10)
         kind = SOLVER_NAMES[solver] if isinstance(solver, str) else int(solver)
         check(PfemStep2D_run(self.h, kind, rtol, maxit, st))
         return dict(mesh_ms=st[0], assemble_ms=st[1], solve_ms=st[2], iters=int(st[3]), true_rel=st[4],
-                    replacements=int(st[5]), nnz=int(st[6]), n_elems=int(st[7]))
+                    replacements=int(st[5]), nnz=int(st[6]), n_elems=int(st[7]), prep_ms=st[8],
+                    fallback=bool(st[9]))
+
+    def assemble(self):
+        """Mesh and assemble without solving; returns (mesh ms, assembly ms)."""
+        st = np.zeros(2)
+        check(PfemStep2D_assemble(self.h, st))
+        return float(st[0]), float(st[1])
+
+    def set_source_modes(self, modes):
+        """Device source f from Fourier modes rows (kx, ky, phi, ax, ay)."""
+        modes = np.ascontiguousarray(modes, np.float64).reshape(-1, 5)
+        check(PfemStep2D_set_source_modes(self.h, modes.shape[0], modes.ravel() if modes.size else None))
+
+    def advect(self, modes, dt, steps=1):
+        """Forward-Euler advection in the stream-function field of modes rows (kx, ky, phi, a)."""
+        modes = np.ascontiguousarray(modes, np.float64).reshape(-1, 4)
+        check(PfemStep2D_advect(self.h, modes.shape[0], modes.ravel(), dt, steps))
+
+    def points(self):
+        xy = np.empty((self.n, 2))
+        check(PfemStep2D_get_points(self.h, xy))
+        return xy
+
+    def set_learned(self, net):
+        """Network used by solver "learned" (kept alive by this object)."""
+        self._learned = net
+        check(PfemStep2D_set_learned(self.h, net.h if net is not None else None))
+
+    def nnz(self):
+        return int(PfemStep2D_nnz(self.h))
 
     def time_dir_kernel(self, fp32: bool, reps: int = 200) -> float:
         """Average ms of the CG direction kernel on the current system."""

@@ -43,7 +43,7 @@ def test_library_exports_every_declared_symbol(libname):
 
 def test_headers_compile_as_c(tmp_path):
     src = tmp_path / "t.c"
-    src.write_text('#include "stream_geometry.h"\n#include "stream_mesh.h"\n#include "stream_numerics.h"\n'
+    src.write_text('#include "stream_geometry.h"\n#include "stream_mesh.h"\n#include "stream_numerics.h"\n#include "stream_learned.h"\n'
                    "int main(void){return 0;}\n")
     r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", INC, str(src), "-o", str(tmp_path / "t")],
                        capture_output=True, text=True)

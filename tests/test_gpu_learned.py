@@ -1,0 +1,211 @@
+"""Learned L L^T preconditioner on the GPU: forward and gradient parity with
+the CPU oracle (oracle/mpnn.py), training determinism, the Jacobi fallback,
+device advection, and the iteration gain of the trained network over the
+method without it (Jacobi-PCG) at the same verified FP64 residual."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from oracle import mpnn
+from stream_amd import clouds
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(n_side=20, seed=0):
+    pts, al = clouds.jittered_square(n_side, seed=seed)
+    r = ol.ashape2d(pts, al, bits=32)
+    xy = r["ordered"]
+    f = clouds.divergence_source(xy, clouds.fourier_modes(11))
+    row, col, val, b = ol.assemble_p1(xy, r["elem"], r["node_is_bnd"], f)
+    return row, col, val, b, xy, r["node_is_bnd"]
+
+
+@pytest.fixture(scope="module")
+def small():
+    return _system()
+
+
+def _params(seed=3, scale=0.05):
+    """Oracle-initialised weights with non-zero output heads, as float32."""
+    P = mpnn.unflatten(mpnn.init_params(7))
+    rng = np.random.default_rng(seed)
+    for k in ("o_w", "d_w", "de_b", "dd_b", "m_b0", "m_b2", "u_b1", "enc_b"):
+        P[k][:] = scale * rng.standard_normal(P[k].shape)
+    return mpnn.flatten(P).astype(np.float32)
+
+
+def _gpu_layout(row, col):
+    """CSR positions of the oracle's off-diagonal entries and of the diagonal."""
+    n = len(row) - 1
+    rid = np.repeat(np.arange(n), np.diff(row.astype(np.int64)))
+    off = np.flatnonzero(rid != col)
+    diag = np.flatnonzero(rid == col)
+    assert len(diag) == n
+    return off, diag
+
+
+def test_param_count_matches_oracle():
+    from stream_amd.learned import PrecLearned, param_layout
+
+    net = PrecLearned()
+    assert net.count == mpnn.n_params()
+    assert [n for n, _ in param_layout()] == [n for n, _ in mpnn.param_layout()]
+    assert np.array_equal(net.params(), mpnn.init_params(7).astype(np.float32))
+
+
+def test_forward_matches_oracle(small):
+    from stream_amd.learned import PrecLearned
+
+    row, col, val, b, xy, bnd = small
+    p = _params()
+    net = PrecLearned(p)
+    net.set_system(row, col, val, xy, bnd)
+    loss, ghat = net.forward(with_ghat=True)
+    g = mpnn.Graph(row, col, val, xy, bnd)
+    gup, gdiag, _ = mpnn.forward(mpnn.unflatten(p.astype(np.float64)), g, emu=True)
+    ref_loss = mpnn.loss_and_grad_ghat(g, gup, gdiag)[0]
+    off, diag = _gpu_layout(row, col)
+    up = g.up
+    gpu_up = ghat[off][up].astype(np.float64)
+    gpu_d = ghat[diag].astype(np.float64)
+    scale = max(np.abs(gup[up]).max(), 1e-30)
+    err_up = np.abs(gpu_up - gup[up]).max() / scale
+    err_d = np.abs(gpu_d - gdiag).max() / np.abs(gdiag).max()
+    print(f"forward: max rel err upper {err_up:.2e} diag {err_d:.2e} loss {loss:.8f} vs {ref_loss:.8f}")
+    assert np.abs(gup[up]).max() > 1e-3  # the head is active
+    assert err_up < 2e-4 and err_d < 2e-4
+    assert abs(loss - ref_loss) <= 1e-4 * max(1.0, abs(ref_loss))
+
+
+def test_zero_head_is_jacobi(small):
+    from stream_amd.learned import PrecLearned
+
+    row, col, val, b, xy, bnd = small
+    net = PrecLearned()  # zero output weights
+    net.set_system(row, col, val, xy, bnd)
+    loss, ghat = net.forward(with_ghat=True)
+    off, diag = _gpu_layout(row, col)
+    rid = np.repeat(np.arange(len(row) - 1), np.diff(row.astype(np.int64)))
+    upper = off[col[off] > rid[off]]
+    assert np.all(ghat[upper] == 0.0)
+    assert np.all(ghat[diag] == 1.0)
+    assert abs(loss) < 1e-6
+
+
+def test_gradient_matches_oracle(small):
+    from stream_amd.learned import PrecLearned
+
+    row, col, val, b, xy, bnd = small
+    p = _params()
+    net = PrecLearned(p)
+    net.set_system(row, col, val, xy, bnd)
+    loss, grad = net.gradient()
+    g = mpnn.Graph(row, col, val, xy, bnd)
+    ref_loss, ref = mpnn.backward(mpnn.unflatten(p.astype(np.float64)), g, emu=True)
+    o = 0
+    worst = 0.0
+    for name, shape in mpnn.param_layout():
+        k = int(np.prod(shape))
+        a = grad[o:o + k].astype(np.float64)
+        r = ref[name].ravel()
+        rel = np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-12)
+        worst = max(worst, rel)
+        assert rel < 1e-3, (name, rel, np.linalg.norm(r))
+        o += k
+    print(f"gradient: worst per-tensor relative error {worst:.2e}")
+    assert abs(loss - ref_loss) <= 1e-4 * max(1.0, abs(ref_loss))
+
+
+def test_training_is_deterministic(small):
+    from stream_amd.learned import PrecLearned
+
+    row, col, val, b, xy, bnd = small
+    out = []
+    for _ in range(2):
+        net = PrecLearned()
+        net.set_system(row, col, val, xy, bnd)
+        losses = [net.train_step(3e-3) for _ in range(6)]
+        out.append((net.params(), losses))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+    assert out[0][1][-1] < out[0][1][0]
+
+
+def test_device_advection_matches_host():
+    from stream_amd.pfem import PfemStep2D
+
+    pts, al = clouds.jittered_square(32, seed=2)
+    step = PfemStep2D(pts, al, np.ones(len(pts)))
+    modes = clouds.stream_modes()
+    step.advect(modes, 1e-3, 5)
+    ref = clouds.advect(pts, modes, 1e-3, 5)
+    assert np.abs(step.points() - ref).max() < 1e-14
+    assert np.abs(step.points() - pts).max() > 1e-5
+
+
+def test_device_source_matches_host():
+    from stream_amd.pfem import PfemStep2D
+
+    pts, al = clouds.dam_break_scaled(5000)
+    src = clouds.fourier_modes(11)
+    a = PfemStep2D(pts, al, clouds.divergence_source(pts, src))
+    a.assemble()
+    ra = a.system()
+    b = PfemStep2D(pts, al, np.zeros(len(pts)))
+    b.set_source_modes(src)
+    b.assemble()
+    rb = b.system()
+    assert np.array_equal(ra[0], rb[0]) and np.array_equal(ra[1], rb[1]) and np.array_equal(ra[2], rb[2])
+    assert np.allclose(ra[3], rb[3], rtol=1e-12, atol=1e-15)
+
+
+def test_nan_weights_fall_back_to_jacobi():
+    from stream_amd.learned import PrecLearned
+    from stream_amd.pfem import PfemStep2D
+
+    pts, al = clouds.jittered_square(48, seed=4)
+    step = PfemStep2D(pts, al, np.ones(len(pts)))
+    p = _params()
+    p[-64:] = np.nan  # d_w
+    net = PrecLearned(p)
+    step.set_learned(net)
+    r = step.run("learned")
+    rj = step.run("mixed")
+    assert r["fallback"] and r["true_rel"] <= 1e-8
+    assert r["iters"] == rj["iters"]
+
+
+def test_trained_network_beats_jacobi():
+    """Train on an advected dam-break sequence (device mesh + assembly every
+    snapshot), evaluate on a later, unseen snapshot."""
+    from stream_amd.learned import PrecLearned
+    from stream_amd.pfem import PfemStep2D
+
+    pts, al = clouds.dam_break_scaled(20000)
+    step = PfemStep2D(pts, al, np.zeros(len(pts)))
+    step.set_source_modes(clouds.fourier_modes(11))
+    modes = clouds.stream_modes()
+    net = PrecLearned()
+    first = last = None
+    for snap in range(6):
+        step.assemble()
+        net.bind_step(step)
+        for it in range(40):
+            loss = net.train_step(2e-3, read_loss=(it == 0 or it == 39))
+            if first is None:
+                first = loss
+        last = loss
+        step.advect(modes, 1e-3, 20)
+    step.advect(modes, 1e-3, 20)
+    step.set_learned(net)
+    rj = step.run("mixed")
+    rl = step.run("learned")
+    rf = step.run("fsai")
+    r64 = step.run("fp64")
+    print(f"loss {first:.4f} -> {last:.4f}; iterations: fp64 Jacobi {r64['iters']}, mixed Jacobi {rj['iters']}, "
+          f"learned {rl['iters']}, FSAI {rf['iters']}; true rel {rl['true_rel']:.2e}")
+    assert last < first - 0.05
+    assert not rl["fallback"] and rl["true_rel"] <= 1e-8
+    assert rl["iters"] < 0.75 * rj["iters"]
+    assert rl["iters"] < 0.8 * r64["iters"]

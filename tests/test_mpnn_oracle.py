@@ -1,0 +1,156 @@
+"""Pins of the learned-preconditioner oracle (oracle/mpnn.py) on the CPU.
+
+The network has no counterpart in the reference (its only preconditioner is
+Jacobi, src/numerics/preconditioner_jacobi.cpp:29-60), so the oracle is pinned
+by mathematics: finite differences of every parameter gradient, the
+Jacobi closed form at zero output weights, an independent dense loss, and
+Kaporin's theorem (the loss minimiser over the pattern is the FSAI factor).
+"""
+import numpy as np
+import pytest
+import scipy.optimize as so
+import scipy.sparse as sp
+
+import oracle_lib as ol
+from oracle import mpnn
+from stream_amd import clouds
+
+
+def _system(n_side=20, seed=0, source="ones"):
+    pts, al = clouds.jittered_square(n_side, seed=seed)
+    r = ol.ashape2d(pts, al, bits=32)
+    xy = r["ordered"]
+    f = np.ones(len(pts)) if source == "ones" else clouds.divergence_source(xy, clouds.fourier_modes(11))
+    row, col, val, b = ol.assemble_p1(xy, r["elem"], r["node_is_bnd"], f)
+    return row, col, val, b, xy, r["node_is_bnd"]
+
+
+@pytest.fixture(scope="module")
+def graph():
+    row, col, val, b, xy, bnd = _system()
+    return mpnn.Graph(row, col, val, xy, bnd)
+
+
+def _random_params(seed=3):
+    p = mpnn.init_params(7)
+    rng = np.random.default_rng(seed)
+    P = mpnn.unflatten(p)
+    for k in ("o_w", "d_w", "de_b", "dd_b", "m_b0", "u_b1"):
+        P[k][:] = 0.03 * rng.standard_normal(P[k].shape)
+    return mpnn.flatten(P)
+
+
+def test_param_count():
+    # 4x64+64 + 3 x (64x128 + 3x64 + 64 + 128x64 + 64) + 64x192 + 3x64 + 4x64
+    assert mpnn.n_params() == 320 + 3 * 16704 + 12288 + 192 + 256
+
+
+def test_bf16_rounding_kat():
+    x = np.array([1.0, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, 1.0 + 2.0 ** -7, -2.5, 3.0e38, 1.0 + 2.0 ** -9])
+    want = np.array([1.0, 1.0, 1.0 + 2.0 ** -6, 1.0 + 2.0 ** -7, -2.5, 2.9908441e38, 1.0])
+    got = mpnn.bf16(x)
+    assert got[:5].tolist() == want[:5].tolist()
+    assert got[6] == 1.0
+    # every result is a bf16 value: low 16 bits of the float32 pattern clear
+    assert np.all(got.astype(np.float32).view(np.uint32) & 0xFFFF == 0)
+
+
+def test_zero_output_weights_is_jacobi(graph):
+    p = mpnn.init_params(7)
+    gup, gdiag, _ = mpnn.forward(mpnn.unflatten(p), graph, emu=True)
+    assert np.all(gup == 0.0)
+    assert np.allclose(gdiag, 1.0, rtol=0, atol=1e-15)
+    loss, _, _, T = mpnn.loss_and_grad_ghat(graph, gup, gdiag)
+    assert abs(loss) < 1e-12 and abs(T - graph.n) < 1e-9
+
+
+def test_loss_dense(graph):
+    p = _random_params()
+    gup, gdiag, _ = mpnn.forward(mpnn.unflatten(p), graph, emu=False)
+    loss, _, _, _ = mpnn.loss_and_grad_ghat(graph, gup, gdiag)
+    G = mpnn.ghat_matrix(graph, gup, gdiag).toarray()
+    assert np.allclose(G, np.triu(G))
+    L = mpnn.factor_csr(graph, gup, gdiag).toarray()
+    A = graph.ahat.toarray() / np.outer(graph.sd, graph.sd)
+    X = L.T @ A @ L
+    n = graph.n
+    # log det X = log det A + 2 sum log L_ii with L_ii = sd_i Ghat_ii
+    dense = (np.log(np.trace(X) / n) - np.linalg.slogdet(X)[1] / n + np.linalg.slogdet(A)[1] / n
+             + 2.0 / n * np.log(graph.sd).sum())
+    assert abs(dense - loss) < 1e-9 * max(1.0, abs(loss))
+
+
+@pytest.mark.parametrize("emu", [False])
+def test_gradient_finite_differences(graph, emu):
+    p = _random_params()
+    _, grad = mpnn.backward(mpnn.unflatten(p), graph, emu=emu)
+    rng = np.random.default_rng(5)
+    offs = 0
+    worst = 0.0
+    for name, shape in mpnn.param_layout():
+        k = int(np.prod(shape))
+        d = np.zeros_like(p)
+        d[offs:offs + k] = rng.standard_normal(k)
+        eps = 1e-7
+        lp = mpnn.loss(mpnn.unflatten(p + eps * d), graph, emu)
+        lm = mpnn.loss(mpnn.unflatten(p - eps * d), graph, emu)
+        fd = (lp - lm) / (2 * eps)
+        an = float(grad[name].ravel() @ d[offs:offs + k])
+        err = abs(fd - an) / max(1e-8, abs(fd), abs(an))
+        worst = max(worst, err)
+        assert err < 5e-4, (name, fd, an)
+        offs += k
+    print("worst relative FD error", worst)
+
+
+def _fsai_dense(A):
+    n = A.shape[0]
+    G = np.zeros((n, n))
+    for i in range(n):
+        P = [i] + [j for j in range(i + 1, n) if A[i, j] != 0.0]
+        g = np.linalg.solve(A[np.ix_(P, P)], np.eye(len(P))[0])
+        G[i, P] = g / np.sqrt(g[0])
+    return G
+
+
+def test_kaporin_minimiser_is_fsai():
+    row, col, val, b, xy, bnd = _system(n_side=8)
+    g = mpnn.Graph(row, col, val, xy, bnd)
+    sel = g.mask
+    idx = np.flatnonzero(sel)
+    n = g.n
+
+    def f(v):
+        gup = np.zeros(len(g.ei))
+        gup[idx] = v[:len(idx)]
+        gd = np.where(g.bnd != 0, 1.0, v[len(idx):])
+        if np.any(gd <= 0):
+            return 1e30, np.zeros_like(v)
+        L, dgup, dgd, _ = mpnn.loss_and_grad_ghat(g, gup, gd)
+        return L, np.concatenate([dgup[idx], np.where(g.bnd != 0, 0.0, dgd)])
+
+    v0 = np.concatenate([np.zeros(len(idx)), np.ones(n)])
+    res = so.minimize(f, v0, jac=True, method="L-BFGS-B", options=dict(maxiter=2000, gtol=1e-12, ftol=1e-15))
+    gup = np.zeros(len(g.ei))
+    gup[idx] = res.x[:len(idx)]
+    gd = np.where(g.bnd != 0, 1.0, res.x[len(idx):])
+    G = mpnn.ghat_matrix(g, gup, gd).toarray()
+    Ah = g.ahat.toarray()
+    act = g.bnd == 0
+    # rows of the minimiser all have ghat_k Ahat ghat_k = T/n; FSAI has 1
+    Gs = G[np.ix_(act, act)]
+    Ga = Gs / np.sqrt(np.diag(Gs @ Ah[np.ix_(act, act)] @ Gs.T))[:, None]
+    F = _fsai_dense(Ah[np.ix_(act, act)])
+    assert np.max(np.abs(Ga - F)) < 1e-5
+
+
+def test_training_lowers_loss(graph):
+    p = mpnn.init_params(7)
+    opt = mpnn.Adam(len(p), lr=2e-3)
+    losses = []
+    for _ in range(25):
+        L, grad = mpnn.backward(mpnn.unflatten(p), graph, emu=True)
+        losses.append(L)
+        p = opt.step(p, mpnn.flatten(grad))
+    assert abs(losses[0]) < 1e-12
+    assert losses[-1] < -0.02
