@@ -5,14 +5,16 @@ jittered fluid column plus 9,472 scattered fragments, alpha = 1.2h), with
 the nodal source f = div u of 8 seeded Fourier modes.  Points and source
 are resident in HBM before timing starts.
 
-One step = LBVH build + 2D alpha-shape + P1 pressure assembly + CG to a
-true FP64 relative residual ||b - A x|| / ||b|| <= 1e-8 (verified after
-every solve).  The headline solver is the mixed-precision PCG (FP32 SpMV
-and preconditioner, FP64 dots and residual replacement); the FP64
-Jacobi-PCG step rate is reported beside it.
+One step = LBVH build + 2D alpha-shape + P1 pressure assembly +
+preconditioner set-up + CG to a true FP64 relative residual
+||b - A x|| / ||b|| <= 1e-8 (verified after every solve).  The headline
+solver is the learned mixed PCG: M^-1 = L L^T from the trained
+message-passing network (weights in stream_amd/weights), FP32 SpMV and
+preconditioner, FP64 dots and residual replacement.  The FP64 Jacobi-PCG
+step rate is measured beside it in the same run.
 
-Prints one JSON line (see the keys below).  ``--gpus N`` with N > 1 exits:
-this build targets one GPU.
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build targets
+one GPU.
 """
 from __future__ import annotations
 
@@ -28,6 +30,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "mesh+solve steps/s @1M 2D pts, rtol 1e-8: learned mixed PCG vs FP64 Jacobi-CG"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+WEIGHTS = os.path.join(ROOT, "stream_amd", "weights", "mpnn_dam2d")
+SOLVER_LABEL = {
+    "learned": "learned mixed PCG (MPNN L L^T, FP32 SpMV/apply, FP64 dots + residual replacement)",
+    "mixed": "mixed FP32/FP64 Jacobi-PCG",
+    "fsai": "mixed PCG with the classical FSAI L L^T",
+    "fp64": "FP64 Jacobi-PCG",
+}
 
 
 def _cpu_model():
@@ -76,12 +86,24 @@ def cpu_baseline(pts, al, f, fp64_iters):
     }
 
 
+def sell_dir_bytes(n, nnz, value_bytes):
+    """Algorithmic bytes of one CG direction launch (d = z + beta d_old, q = A d,
+    d.q) on the SELL-64 matrix: per row the length (2 B), the diagonal, the
+    own {z, d_old} pair, d and q written; per off-diagonal entry its column
+    (4 B) and value; per 64-row slice its offset and width (6 B).  Each
+    vector entry is counted once (neighbour re-reads are cache traffic)."""
+    vb = value_bytes
+    nslices = (n + 63) // 64
+    return n * (2 + vb + 2 * vb + vb + vb) + (nnz - n) * (4 + vb) + nslices * 6
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--solver", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--solver", default="learned", choices=["learned", "mixed", "fsai", "fp64"])
+    ap.add_argument("--weights", default=WEIGHTS, help="weight file prefix (.bin + .json) of the learned solver")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 companion measurement")
     args = ap.parse_args()
@@ -102,6 +124,16 @@ def main():
     pts, al = clouds.dam_break()
     f = clouds.divergence_source(pts, clouds.fourier_modes(11))
     step = PfemStep2D(pts, al, f)
+    net = None
+    if args.solver == "learned":
+        from stream_amd.learned import PrecLearned
+
+        if not os.path.exists(args.weights + ".bin"):
+            print(f"learned solver: weight file {args.weights}.bin not found "
+                  "(train with python -m stream_amd.train_learned --out <prefix>)", file=sys.stderr)
+            sys.exit(1)
+        net = PrecLearned(PrecLearned.load_params(args.weights))
+        step.set_learned(net)
 
     def timed(solver, k, w):
         infos = []
@@ -116,9 +148,11 @@ def main():
 
     ms, infos = timed(args.solver, args.steps, args.warmup)
     last = infos[-1]
+    if any(i["fallback"] for i in infos):
+        print("[bench] the learned factor was rejected (Jacobi fallback used)", file=sys.stderr)
     print(f"[bench] {args.solver}: {ms:.2f} ms/step  mesh {last['mesh_ms']:.2f}  asm {last['assemble_ms']:.2f}  "
-          f"solve {last['solve_ms']:.2f} ms  iters {last['iters']}  rel {last['true_rel']:.2e}", file=sys.stderr,
-          flush=True)
+          f"prep {last['prep_ms']:.2f}  solve {last['solve_ms']:.2f} ms  iters {last['iters']}  "
+          f"rel {last['true_rel']:.2e}", file=sys.stderr, flush=True)
     fp64 = None
     if not args.no_fp64 and args.solver != "fp64":
         ms64, inf64 = timed("fp64", max(1, args.steps // 2), 1)
@@ -131,15 +165,22 @@ def main():
     kms = step.time_dir_kernel(fp32=fp32, reps=200)
     n = pts.shape[0]
     nnz = last["nnz"]
-    vb = 4 if fp32 else 8
-    # algorithmic bytes per launch: row pointers, (col, val) per entry,
-    # read z and d_old, write d_new and q
-    alg_bytes = 4 * (n + 1) + nnz * (4 + vb) + 4 * vb * n
+    alg_bytes = sell_dir_bytes(n, nnz, 4 if fp32 else 8)
     achieved = alg_bytes / (kms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"cg_dir_kernel<{'float' if fp32 else 'double'}>", "kernel_ms": round(kms, 5),
+                "kernel": f"cg_dir_kernel<{'float, true' if fp32 else 'double, false'}>", "kernel_ms": round(kms, 5),
                 "alg_bytes_per_launch": int(alg_bytes)}
+
+    mfma = None
+    if net is not None:
+        step.run(args.solver, rtol=1e-8)  # leaves the system assembled
+        net.bind_step(step)
+        gms, gflops = net.time_gemm(reps=100)
+        tf = gflops / (gms * 1e-3) / 1e12
+        mfma = {"bound": "mfma", "kernel": "gemm_rows_kernel<64, 64, 128, 0> (layer-0 [P|Q] = H W_pq, bf16)",
+                "achieved": round(tf, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tf / BF16_PEAK_TFLOPS, 5), "kernel_ms": round(gms, 5), "flops_per_launch": gflops}
 
     cpu = None
     if not args.no_cpu:
@@ -163,17 +204,20 @@ def main():
             "workload": "2D dam-break cloud, 1,000,000 FP64 points, alpha=1.2h, LBVH + alpha-shape + P1 pressure "
                         "assembly + CG to true FP64 rtol 1e-8",
             "points": int(n),
-            "solver": "mixed FP32/FP64 Jacobi-PCG" if fp32 else "FP64 Jacobi-PCG",
+            "solver": SOLVER_LABEL[args.solver],
+            "weights": os.path.relpath(args.weights, ROOT) if net is not None else None,
             "nnz": int(nnz),
             "triangles": int(last["n_elems"]),
         },
         "stages_ms": {"mesh": round(last["mesh_ms"], 3), "assembly": round(last["assemble_ms"], 3),
-                      "solve": round(last["solve_ms"], 3)},
+                      "precond_setup": round(last["prep_ms"], 3), "solve": round(last["solve_ms"], 3)},
         "iters": last["iters"],
         "true_rel": last["true_rel"],
         "fp64_jacobi": None if fp64 is None else {
             "steps_per_s": round(1000.0 / fp64["ms_per_step"], 4), **fp64},
+        "speedup_vs_fp64_jacobi": None if fp64 is None else round(fp64["ms_per_step"] / ms, 3),
         "roofline": roofline,
+        "mfma": mfma,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

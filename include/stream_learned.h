@@ -45,6 +45,9 @@ int32_t PrecLearned_forward(PrecLearned* net, double* loss, float* ghat);
 int32_t PrecLearned_gradient(PrecLearned* net, double* loss, float* grad);
 /* One Adam step with learning rate lr; loss (nullable) before the update. */
 int32_t PrecLearned_train_step(PrecLearned* net, float lr, double* loss);
+/* Average ms over `reps` launches (HIP events) of the layer-0 MFMA GEMM
+ * [P|Q] = H W_pq on the bound system; flops (nullable) = 2 n 64 128. */
+int32_t PrecLearned_time_gemm(PrecLearned* net, uint32_t reps, double* ms, double* flops);
 /* Network used by PfemStep2D_run's solver 3 (not owned; NULL detaches). */
 int32_t PfemStep2D_set_learned(PfemStep2D* step, PrecLearned* net);
 

@@ -57,8 +57,9 @@ int32_t SolverCG_solve_mixed(SolverCG* solver, const LinSys* sys, const PrecJaco
                              uint32_t maxit, double* stats);
 
 /* Process-wide solver knobs: "replace_factor" (mixed precision residual
- * replacement threshold, default 0.1) and "batch" (iterations between host
- * checks, default 32). */
+ * replacement threshold, default 0.3), "batch" (iterations between host
+ * checks, default 32) and "cg_blocks" (grid cap of the CG kernels, 64..4096,
+ * default 1024). */
 int32_t Stream_tune(const char* key, double value);
 
 /* Mesh-and-solve step on device-resident points: LBVH + 2D alpha-shape +

@@ -4,6 +4,7 @@
 #include <cstdio>
 
 #include "oracle.hpp"
+#include "oracle3d.hpp"
 
 using namespace orc;
 
@@ -126,9 +127,90 @@ void get2d(AlphaShape2D<S>* a, uint32_t* elem, uint32_t* row, uint32_t* neig, ui
     if (nineig)
         for (uint32_t i = 0; i < n; ++i) nineig[i] = a->nineig[i];
 }
+template <class S>
+void* ashape3d_run(uint32_t n, const S* xyzr, int bits) {
+    auto* a = new AlphaShape3D<S>();
+    a->morton_bits = bits;
+    a->init(pack<S, 3>(n, xyzr));
+    a->compute();
+    return a;
+}
+
+template <class S>
+void get3d(AlphaShape3D<S>* a, uint32_t* elem, uint32_t* row, uint32_t* neig, uint8_t* edge_bnd,
+           uint8_t* node_bnd, uint32_t* perm, S* xyz_m, uint32_t* nineig) {
+    const uint32_t n = a->n;
+    if (elem) std::memcpy(elem, a->elem.data(), sizeof(uint32_t) * 4ull * a->n_elems);
+    if (row) std::memcpy(row, a->row.data(), sizeof(uint32_t) * (n + 1));
+    if (neig) std::memcpy(neig, a->neig.data(), sizeof(uint32_t) * a->n_edges);
+    if (edge_bnd) std::memcpy(edge_bnd, a->edge_is_bnd.data(), a->n_edges);
+    if (node_bnd) std::memcpy(node_bnd, a->node_is_bnd.data(), n);
+    if (perm) std::memcpy(perm, a->lbvh.perm.data(), sizeof(uint32_t) * n);
+    if (xyz_m)
+        for (uint32_t i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) xyz_m[3 * i + c] = a->coords_m[i].c[c];
+    if (nineig)
+        for (uint32_t i = 0; i < n; ++i) nineig[i] = a->nineig[i];
+}
 }  // namespace
 
 extern "C" {
+
+// ---- 3D predicates ----
+float orc_orient3d_f32(const float* a, const float* b, const float* c, const float* d) { return orient3d(a, b, c, d); }
+double orc_orient3d_f64(const double* a, const double* b, const double* c, const double* d) {
+    return orient3d(a, b, c, d);
+}
+int orc_orient3d_exact_f32(const float* a, const float* b, const float* c, const float* d) {
+    return exact::orient3d_sign(a, b, c, d);
+}
+int orc_orient3d_exact_f64(const double* a, const double* b, const double* c, const double* d) {
+    return exact::orient3d_sign(a, b, c, d);
+}
+float orc_insphere_filter_f32(const float* a, const float* b, const float* c, const float* d, const float* e,
+                              float* err) {
+    return insphere_filter(a, b, c, d, e, err);
+}
+double orc_insphere_filter_f64(const double* a, const double* b, const double* c, const double* d, const double* e,
+                               double* err) {
+    return insphere_filter(a, b, c, d, e, err);
+}
+int orc_insphere_exact_f32(const float* a, const float* b, const float* c, const float* d, const float* e) {
+    return exact::insphere_sign(a, b, c, d, e);
+}
+int orc_insphere_exact_f64(const double* a, const double* b, const double* c, const double* d, const double* e) {
+    return exact::insphere_sign(a, b, c, d, e);
+}
+
+// ---- alpha-shape 3D ----
+void* orc_ashape3d_run_f32(uint32_t n, const float* xyzr, int bits) { return ashape3d_run<float>(n, xyzr, bits); }
+void* orc_ashape3d_run_f64(uint32_t n, const double* xyzr, int bits) { return ashape3d_run<double>(n, xyzr, bits); }
+// sizes: [n, n_elems, n_edges, exact_calls, over_cap, broken_cycles]
+void orc_ashape3d_sizes(void* h, int is_f64, uint64_t* out) {
+    if (is_f64) {
+        auto* a = (AlphaShape3D<double>*)h;
+        out[0] = a->n; out[1] = a->n_elems; out[2] = a->n_edges; out[3] = a->exact_calls;
+        out[4] = a->over_cap; out[5] = a->broken_cycles.load();
+    } else {
+        auto* a = (AlphaShape3D<float>*)h;
+        out[0] = a->n; out[1] = a->n_elems; out[2] = a->n_edges; out[3] = a->exact_calls;
+        out[4] = a->over_cap; out[5] = a->broken_cycles.load();
+    }
+}
+void orc_ashape3d_get_f32(void* h, uint32_t* elem, uint32_t* row, uint32_t* neig, uint8_t* eb, uint8_t* nb,
+                          uint32_t* perm, float* xyz_m, uint32_t* nineig) {
+    get3d((AlphaShape3D<float>*)h, elem, row, neig, eb, nb, perm, xyz_m, nineig);
+}
+void orc_ashape3d_get_f64(void* h, uint32_t* elem, uint32_t* row, uint32_t* neig, uint8_t* eb, uint8_t* nb,
+                          uint32_t* perm, double* xyz_m, uint32_t* nineig) {
+    get3d((AlphaShape3D<double>*)h, elem, row, neig, eb, nb, perm, xyz_m, nineig);
+}
+void orc_ashape3d_free(void* h, int is_f64) {
+    if (is_f64)
+        delete (AlphaShape3D<double>*)h;
+    else
+        delete (AlphaShape3D<float>*)h;
+}
 
 int orc_lbvh2d_build_f32(uint32_t n, const float* xyr, int bits, int leaf_mode, uint64_t* codes,
                          uint64_t* codes_sorted, uint32_t* perm, int32_t* sep, uint8_t* cl, uint32_t* lp,

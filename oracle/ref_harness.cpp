@@ -21,6 +21,16 @@ float ref_incircle(const float* a, const float* b, const float* c, const float* 
 float ref_incircleexact(const float* a, const float* b, const float* c, const float* d) {
     return incircleexact(a, b, c, d);
 }
+float ref_orient3d(const float* a, const float* b, const float* c, const float* d) { return orient3d(a, b, c, d); }
+float ref_orient3dexact(const float* a, const float* b, const float* c, const float* d) {
+    return orient3dexact(a, b, c, d);
+}
+float ref_insphere(const float* a, const float* b, const float* c, const float* d, const float* e, float* err) {
+    return insphere(a, b, c, d, e, err);
+}
+float ref_insphereexact(const float* a, const float* b, const float* c, const float* d, const float* e) {
+    return insphereexact(a, b, c, d, e);
+}
 uint64_t ref_encode2d(uint32_t x, uint32_t y) { return stream::geo::encode_magicbits2D(x, y); }
 uint64_t ref_encode3d(uint32_t x, uint32_t y, uint32_t z) { return stream::geo::encode_magicbits3D(x, y, z); }
 void ref_decode2d(uint64_t m, uint32_t* x, uint32_t* y) { stream::geo::decode_magicbits2D(m, *x, *y); }

@@ -1084,6 +1084,34 @@ int32_t PrecLearned_gradient(PrecLearned* h, double* loss, float* grad) {
     });
 }
 
+// Average duration (ms, HIP events on the library stream over `reps`
+// launches) of the layer-0 [P|Q] = H W_pq GEMM on the bound system, and its
+// FLOP count per launch (2 n 64 128).
+int32_t PrecLearned_time_gemm(PrecLearned* h, uint32_t reps, double* ms, double* flops) {
+    return guarded([&] {
+        if (!h || !ms || !reps) throw std::invalid_argument("PrecLearned_time_gemm: bad arguments");
+        hipStream_t s = ctx().stream;
+        MpnnNet& net = h->net;
+        const uint32_t n = h->graph.n;
+        net.forward(h->graph, s);
+        hipEvent_t e0, e1;
+        SA_HIP(hipEventCreate(&e0));
+        SA_HIP(hipEventCreate(&e1));
+        const GemmArgs a = gargs(n, net.H[0].p, kHid, net.wt.p + wt_pq(0), net.PQ[0].p, kPq);
+        run_gemm<64, 64, 128, kEpiStore>(a, s);
+        SA_HIP(hipEventRecord(e0, s));
+        for (uint32_t k = 0; k < reps; ++k) run_gemm<64, 64, 128, kEpiStore>(a, s);
+        SA_HIP(hipEventRecord(e1, s));
+        SA_HIP(hipEventSynchronize(e1));
+        float t = 0.f;
+        SA_HIP(hipEventElapsedTime(&t, e0, e1));
+        SA_HIP(hipEventDestroy(e0));
+        SA_HIP(hipEventDestroy(e1));
+        *ms = (double)t / reps;
+        if (flops) *flops = 2.0 * n * kHid * kPq;
+    });
+}
+
 // One Adam step on the bound system.  loss (nullable) is the loss before
 // the update; reading it synchronises.
 int32_t PrecLearned_train_step(PrecLearned* h, float lr, double* loss) {

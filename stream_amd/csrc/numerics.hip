@@ -8,9 +8,10 @@
 namespace stream_amd {
 
 constexpr int kCgThreads = 256;
-// Grid of every CG kernel: at most 1024 blocks (4 per CU) with grid-stride
-// rows, so the last block folds at most 4 partials per thread.
-constexpr uint32_t kMaxCgBlocks = 1024;
+// Grid of every CG kernel: at most Tuning::cg_blocks blocks (<= 4096) with
+// grid-stride rows; the last block folds at most 16 partials per thread,
+// all loads in flight at once.
+constexpr uint32_t kMaxCgBlocks = 4096;
 
 Tuning& tuning() {
     static Tuning t;
@@ -19,7 +20,8 @@ Tuning& tuning() {
 
 void CgWork::resize(uint32_t n) {
     blocks = (n + kCgThreads - 1) / kCgThreads;
-    if (blocks > kMaxCgBlocks) blocks = kMaxCgBlocks;
+    const uint32_t cap = tuning().cg_blocks < kMaxCgBlocks ? tuning().cg_blocks : kMaxCgBlocks;
+    if (blocks > cap) blocks = cap;
     if (blocks == 0) blocks = 1;
     sc.resize(1);
     out2.resize(2);
@@ -242,13 +244,14 @@ template <class V> SellArgs<V> sell_args(const Sell<V>& S) {
 template <class V, int PREC>
 __global__ __launch_bounds__(kCgThreads) void cg_residual_kernel(
     uint32_t n, const uint32_t* __restrict__ row, const uint32_t* __restrict__ col, const double* __restrict__ val,
-    const double* __restrict__ b, const double* __restrict__ x, V* __restrict__ r, V2<V>* __restrict__ P,
-    const V* __restrict__ idiag, CgScalars* sc, double* partials, uint32_t* counter, int mode, double rtol2) {
+    const double* __restrict__ b, const double* __restrict__ x, V* __restrict__ r, uint32_t rs,
+    V2<V>* __restrict__ P, const V* __restrict__ idiag, CgScalars* sc, double* partials, uint32_t* counter, int mode,
+    double rtol2) {
     double v[3] = {0.0, 0.0, 0.0};
     CG_ROWS(i) {
         const double ri = b[i] - row_dot(row, col, val, x, i);
         const V rv = (V)ri;
-        r[i] = rv;
+        r[(size_t)i * rs] = rv;
         V2<V> p = P[i];
         if (mode == 0) p.y = (V)0;
         if constexpr (PREC != kPrecLLt) {
@@ -292,7 +295,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_residual_kernel(
 // loses to cancellation.
 template <class V, bool DIFF>
 __global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(SellArgs<V> M, const V2<V>* __restrict__ cur,
-                                                            V2<V>* __restrict__ nxt, V* __restrict__ q,
+                                                            V2<V>* __restrict__ nxt, V* __restrict__ q, uint32_t qs,
                                                             CgScalars* sc, double* partials, uint32_t* counter) {
     if (sc->done) return;
     const V beta = (V)sc->beta;
@@ -331,7 +334,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(SellArgs<V> M, const
         }
         if (valid) {
             nxt[i].y = dn;
-            q[i] = acc;
+            q[(size_t)i * qs] = acc;
             v[0] += (double)dn * (double)acc;
         }
     }
@@ -382,47 +385,95 @@ __global__ __launch_bounds__(kCgThreads) void cg_update_kernel(uint32_t n, XV* _
     }
 }
 
-// w = L^T r : w_i = L_ii r_i + sum_{j>i} L_ji r_j (U = the upper SELL).
-template <class V>
-__global__ __launch_bounds__(kCgThreads) void llt_up_kernel(SellArgs<V> M, const V* __restrict__ r,
-                                                            V* __restrict__ w, const CgScalars* sc, int force) {
-    if (!force && sc->done) return;
+// L L^T path, one kernel for the update and the first half of the apply:
+// x += alpha d; r' = r - alpha q; w = L^T r' (w_i = L_ii r'_i + sum_{j>i}
+// L_ji r'_j, U = the upper SELL).  r and q live interleaved ({r, q} per
+// row, written by the direction kernel), so the r' of each upper neighbour
+// is recomputed from one 8-byte load with the same operations its owner
+// uses; r' goes to the other {r, q} buffer.  Partials w.w (= r'.z, since
+// z = L w) and r'.r'.
+// mode 0: iteration (beta, iter, done); mode 1: init, mode 2: after a
+// residual replacement (r given in rq.x, alpha = 0, x and rq_nxt untouched;
+// mode 2 keeps beta as in cg_residual_kernel).
+template <class V, class XV>
+__global__ __launch_bounds__(kCgThreads) void llt_update_kernel(SellArgs<V> U, XV* __restrict__ x,
+                                                                const V2<V>* __restrict__ pd,
+                                                                const V2<V>* __restrict__ rq,
+                                                                V2<V>* __restrict__ rq_nxt, V* __restrict__ w,
+                                                                CgScalars* sc, double* partials, uint32_t* counter,
+                                                                int mode) {
+    if (mode == 0 && sc->done) return;
+    const SellArgs<V>& M = U;  // slice loop
+    const double alpha = mode == 0 ? sc->alpha : 0.0;
+    const V av = (V)alpha;
+    const bool it = mode == 0;
     const uint32_t lane = threadIdx.x & 63u;
+    double v[2] = {0.0, 0.0};
     SELL_FOR_SLICES(s) {
         const uint32_t i = s * kSlice + lane;
-        const bool valid = i < M.n;
+        const bool valid = i < U.n;
         const uint32_t ii = valid ? i : 0u;
-        const uint32_t L = valid ? (uint32_t)M.len[i] : 0u;
-        const uint32_t W = M.width[s];
-        const uint32_t base = M.off[s] + lane;
-        V acc = M.diag[ii] * r[ii];
+        const uint32_t L = valid ? (uint32_t)U.len[i] : 0u;
+        const uint32_t W = U.width[s];
+        const uint32_t base = U.off[s] + lane;
+        const V2<V> own = rq[ii];
+        const V ri = it ? own.x - av * own.y : own.x;
+        V acc = U.diag[ii] * ri;
         for (uint32_t k = 0; k < W; k += 4) {
             uint32_t c[4];
             V a[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const bool ok = k + u < L;
-                c[u] = ok ? M.col[base + (k + u) * kSlice] : ii;
-                a[u] = ok ? M.val[base + (k + u) * kSlice] : (V)0;
+                c[u] = ok ? U.col[base + (k + u) * kSlice] : ii;
+                a[u] = ok ? U.val[base + (k + u) * kSlice] : (V)0;
             }
+            V2<V> g[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) acc += a[u] * r[c[u]];
+            for (int u = 0; u < 4; ++u) g[u] = rq[c[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += a[u] * (it ? g[u].x - av * g[u].y : g[u].x);
         }
-        if (valid) w[i] = acc;
+        if (valid) {
+            if (it) {
+                x[i] += (XV)(alpha * (double)pd[i].y);
+                rq_nxt[i].x = ri;
+            }
+            w[i] = acc;
+            v[0] += (double)acc * (double)acc;
+            v[1] += (double)ri * (double)ri;
+        }
+    }
+    double tot[2];
+    if (!grid_fold<2>(v, partials, counter, tot)) return;
+    if (threadIdx.x == 0) {
+        if (mode == 0) {
+            sc->rr = tot[1];
+            sc->rz_old = sc->rz;
+            sc->beta = tot[0] / sc->rz;
+            sc->rz = tot[0];
+            sc->iter += 1;
+            const double cv = sc->crit ? tot[0] : tot[1];
+            sc->done = (cv <= sc->tol2 || sc->iter >= sc->maxit) ? 1u : 0u;
+        } else if (mode == 1) {
+            sc->rz = tot[0];
+            sc->rz_old = tot[0];
+            const double cv = sc->crit ? tot[0] : sc->rr;
+            sc->done = (cv <= sc->tol2 || sc->iter >= sc->maxit) ? 1u : 0u;
+        } else {
+            sc->rz = tot[0];
+        }
     }
 }
 
-// z = L w : z_i = L_ii w_i + sum_{j<i} L_ij w_j into P[i].x; partial r.z.
-// mode 0: iteration bookkeeping (beta, iter, done); mode 1: init;
-// mode 2: after a residual replacement (beta kept, as in cg_residual_kernel).
+// Second half of the apply: z = L w (z_i = L_ii w_i + sum_{j<i} L_ij w_j)
+// into P[i].x.  force: run even when the solve is flagged done (init and
+// replacement passes).
 template <class V>
 __global__ __launch_bounds__(kCgThreads) void llt_lo_kernel(SellArgs<V> M, const V* __restrict__ w,
-                                                            const V* __restrict__ r, V2<V>* __restrict__ P,
-                                                            CgScalars* sc, double* partials, uint32_t* counter,
-                                                            int mode) {
-    if (mode == 0 && sc->done) return;
+                                                            V2<V>* __restrict__ P, const CgScalars* sc, int force) {
+    if (!force && sc->done) return;
     const uint32_t lane = threadIdx.x & 63u;
-    double v[1] = {0.0};
     SELL_FOR_SLICES(s) {
         const uint32_t i = s * kSlice + lane;
         const bool valid = i < M.n;
@@ -443,29 +494,7 @@ __global__ __launch_bounds__(kCgThreads) void llt_lo_kernel(SellArgs<V> M, const
 #pragma unroll
             for (int u = 0; u < 4; ++u) acc += a[u] * w[c[u]];
         }
-        if (valid) {
-            P[i].x = acc;
-            v[0] += (double)r[i] * (double)acc;
-        }
-    }
-    double tot[1];
-    if (!grid_fold<1>(v, partials, counter, tot)) return;
-    if (threadIdx.x == 0) {
-        if (mode == 0) {
-            sc->rz_old = sc->rz;
-            sc->beta = tot[0] / sc->rz;
-            sc->rz = tot[0];
-            sc->iter += 1;
-            const double cv = sc->crit ? tot[0] : sc->rr;
-            sc->done = (cv <= sc->tol2 || sc->iter >= sc->maxit) ? 1u : 0u;
-        } else if (mode == 1) {
-            sc->rz = tot[0];
-            sc->rz_old = tot[0];
-            const double cv = sc->crit ? tot[0] : sc->rr;
-            sc->done = (cv <= sc->tol2 || sc->iter >= sc->maxit) ? 1u : 0u;
-        } else {
-            sc->rz = tot[0];
-        }
+        if (valid) P[i].x = acc;
     }
 }
 
@@ -523,7 +552,8 @@ void clear_done(CgWork& w, hipStream_t s) {
 
 // Vectors of one solve, carved from the grow-only workspace.
 template <class V> struct CgVecs {
-    V2<V>* P[2];
+    V2<V>* P[2];   // {z, d} ping-pong
+    V2<V>* RQ[2];  // {r, q} ping-pong (L L^T path)
     V* r;
     V* q;
     V* w;
@@ -531,12 +561,16 @@ template <class V> struct CgVecs {
 };
 template <class V> CgVecs<V> carve(CgWork& w, uint32_t n) {
     const size_t nv = ((size_t)n + 63) & ~(size_t)63;
-    w.vbuf.resize(nv * (2 * sizeof(V2<V>) + 3 * sizeof(V) + sizeof(float)) + 256);
+    w.vbuf.resize(nv * (4 * sizeof(V2<V>) + 3 * sizeof(V) + sizeof(float)) + 256);
     uint8_t* p = w.vbuf.p;
     CgVecs<V> v;
     v.P[0] = reinterpret_cast<V2<V>*>(p);
     p += nv * sizeof(V2<V>);
     v.P[1] = reinterpret_cast<V2<V>*>(p);
+    p += nv * sizeof(V2<V>);
+    v.RQ[0] = reinterpret_cast<V2<V>*>(p);
+    p += nv * sizeof(V2<V>);
+    v.RQ[1] = reinterpret_cast<V2<V>*>(p);
     p += nv * sizeof(V2<V>);
     v.r = reinterpret_cast<V*>(p);
     p += nv * sizeof(V);
@@ -585,12 +619,17 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
     const double rtol2 = o.abs_tol2 >= 0.0 ? -o.abs_tol2 : o.rtol * o.rtol;
     auto residual = [&](int mode, uint32_t it) {
         V2<V>* cur = vv.P[it & 1];
-        hipLaunchKernelGGL((cg_residual_kernel<V, PREC>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p, b, x,
-                           vv.r, cur, idiag, w.sc.p, w.partials.p, w.counter.p, mode, rtol2);
         if constexpr (PREC == kPrecLLt) {
-            hipLaunchKernelGGL((llt_up_kernel<V>), grid, blk, 0, s, MU, vv.r, vv.w, w.sc.p, 1);
-            hipLaunchKernelGGL((llt_lo_kernel<V>), grid, blk, 0, s, ML, vv.w, vv.r, cur, w.sc.p, w.partials.p,
-                               w.counter.p, mode == 0 ? 1 : 2);
+            V2<V>* rqc = vv.RQ[it & 1];
+            hipLaunchKernelGGL((cg_residual_kernel<V, PREC>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p,
+                               b, x, reinterpret_cast<V*>(rqc), 2u, cur, idiag, w.sc.p, w.partials.p, w.counter.p,
+                               mode, rtol2);
+            hipLaunchKernelGGL((llt_update_kernel<V, XV>), grid, blk, 0, s, MU, xacc, cur, rqc, rqc, vv.w, w.sc.p,
+                               w.partials.p, w.counter.p, mode == 0 ? 1 : 2);
+            hipLaunchKernelGGL((llt_lo_kernel<V>), grid, blk, 0, s, ML, vv.w, cur, w.sc.p, 1);
+        } else {
+            hipLaunchKernelGGL((cg_residual_kernel<V, PREC>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p,
+                               b, x, vv.r, 1u, cur, idiag, w.sc.p, w.partials.p, w.counter.p, mode, rtol2);
         }
     };
     auto merge = [&] {
@@ -622,14 +661,19 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
             const uint32_t it = it0 + k;
             V2<V>* cur = vv.P[it & 1];
             V2<V>* nxt = vv.P[(it + 1) & 1];
-            hipLaunchKernelGGL((cg_dir_kernel<V, DIFF>), grid, blk, 0, s, MA, cur, nxt, vv.q, w.sc.p, w.partials.p,
-                               w.counter.p);
-            hipLaunchKernelGGL((cg_update_kernel<V, XV, PREC>), grid, blk, 0, s, n, xacc, vv.r, nxt, vv.q, idiag,
-                               w.sc.p, w.partials.p, w.counter.p);
             if constexpr (PREC == kPrecLLt) {
-                hipLaunchKernelGGL((llt_up_kernel<V>), grid, blk, 0, s, MU, vv.r, vv.w, w.sc.p, 0);
-                hipLaunchKernelGGL((llt_lo_kernel<V>), grid, blk, 0, s, ML, vv.w, vv.r, nxt, w.sc.p, w.partials.p,
-                                   w.counter.p, 0);
+                V2<V>* rqc = vv.RQ[it & 1];
+                V2<V>* rqn = vv.RQ[(it + 1) & 1];
+                hipLaunchKernelGGL((cg_dir_kernel<V, DIFF>), grid, blk, 0, s, MA, cur, nxt,
+                                   reinterpret_cast<V*>(rqc) + 1, 2u, w.sc.p, w.partials.p, w.counter.p);
+                hipLaunchKernelGGL((llt_update_kernel<V, XV>), grid, blk, 0, s, MU, xacc, nxt, rqc, rqn, vv.w,
+                                   w.sc.p, w.partials.p, w.counter.p, 0);
+                hipLaunchKernelGGL((llt_lo_kernel<V>), grid, blk, 0, s, ML, vv.w, nxt, w.sc.p, 0);
+            } else {
+                hipLaunchKernelGGL((cg_dir_kernel<V, DIFF>), grid, blk, 0, s, MA, cur, nxt, vv.q, 1u, w.sc.p,
+                                   w.partials.p, w.counter.p);
+                hipLaunchKernelGGL((cg_update_kernel<V, XV, PREC>), grid, blk, 0, s, n, xacc, vv.r, nxt, vv.q, idiag,
+                                   w.sc.p, w.partials.p, w.counter.p);
             }
         }
         h = read_sc(w, s);
@@ -717,11 +761,11 @@ template <class V> double time_dir_kernel(const Sell<V>& S, bool diff, uint32_t 
         V2<V>* cur = vv.P[k & 1];
         V2<V>* nxt = vv.P[(k + 1) & 1];
         if (diff)
-            hipLaunchKernelGGL((cg_dir_kernel<V, true>), grid, blk, 0, s, M, cur, nxt, vv.q, w.sc.p, w.partials.p,
-                               w.counter.p);
+            hipLaunchKernelGGL((cg_dir_kernel<V, true>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
+                               w.partials.p, w.counter.p);
         else
-            hipLaunchKernelGGL((cg_dir_kernel<V, false>), grid, blk, 0, s, M, cur, nxt, vv.q, w.sc.p, w.partials.p,
-                               w.counter.p);
+            hipLaunchKernelGGL((cg_dir_kernel<V, false>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
+                               w.partials.p, w.counter.p);
     };
     for (uint32_t k = 0; k < 3; ++k) launch(k);
     hipEvent_t e0, e1;

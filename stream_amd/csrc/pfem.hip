@@ -738,6 +738,9 @@ int32_t Stream_tune(const char* key, double value) {
         } else if (k == "batch") {
             if (!(value >= 1 && value <= 4096)) throw std::invalid_argument("batch must be in [1, 4096]");
             tuning().batch = (uint32_t)value;
+        } else if (k == "cg_blocks") {
+            if (!(value >= 64 && value <= 4096)) throw std::invalid_argument("cg_blocks must be in [64, 4096]");
+            tuning().cg_blocks = (uint32_t)value;
         } else {
             throw std::invalid_argument("unknown tuning key: " + k);
         }

@@ -32,6 +32,7 @@ PrecLearned_bind_step = bind(lib, "PrecLearned_bind_step", i32, [_p, _p])
 PrecLearned_forward = bind(lib, "PrecLearned_forward", i32, [_p, _dp, opt(np.float32)])
 PrecLearned_gradient = bind(lib, "PrecLearned_gradient", i32, [_p, _dp, opt(np.float32)])
 PrecLearned_train_step = bind(lib, "PrecLearned_train_step", i32, [_p, ctypes.c_float, _dp])
+PrecLearned_time_gemm = bind(lib, "PrecLearned_time_gemm", i32, [_p, u32, _dp, _dp])
 
 HIDDEN = 64
 LAYERS = 3
@@ -123,6 +124,13 @@ class PrecLearned:
         loss = ctypes.c_double(0.0)
         check(PrecLearned_train_step(self.h, lr, ctypes.byref(loss) if read_loss else None))
         return loss.value if read_loss else None
+
+    def time_gemm(self, reps: int = 100):
+        """(average ms, FLOPs) of the layer-0 MFMA GEMM on the bound system."""
+        ms = ctypes.c_double(0.0)
+        fl = ctypes.c_double(0.0)
+        check(PrecLearned_time_gemm(self.h, reps, ctypes.byref(ms), ctypes.byref(fl)))
+        return ms.value, fl.value
 
     def save(self, path: str, meta: dict | None = None):
         """FP32 parameters (little-endian .bin) plus a JSON description."""

@@ -67,6 +67,14 @@ class PfemStep2D:
                     replacements=int(st[5]), nnz=int(st[6]), n_elems=int(st[7]), prep_ms=st[8],
                     fallback=bool(st[9]))
 
+    def set_points(self, xy, alpha, f):
+        """Replace the points (same count or a new one), alpha and host source."""
+        xy = np.ascontiguousarray(xy, np.float64)
+        self.n = xy.shape[0]
+        check(PfemStep2D_set_source_modes(self.h, 0, None))
+        check(PfemStep2D_set_points(self.h, self.n, xy, np.ascontiguousarray(alpha, np.float64),
+                                    np.ascontiguousarray(f, np.float64)))
+
     def assemble(self):
         """Mesh and assemble without solving; returns (mesh ms, assembly ms)."""
         st = np.zeros(2)

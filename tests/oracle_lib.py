@@ -42,6 +42,13 @@ def ref():
         _ref.ref_incircle.argtypes = [P(f)] * 5
         _ref.ref_incircleexact.restype = f
         _ref.ref_incircleexact.argtypes = [P(f)] * 4
+        for name in ("ref_orient3d", "ref_orient3dexact"):
+            getattr(_ref, name).restype = f
+            getattr(_ref, name).argtypes = [P(f)] * 4
+        _ref.ref_insphere.restype = f
+        _ref.ref_insphere.argtypes = [P(f)] * 6
+        _ref.ref_insphereexact.restype = f
+        _ref.ref_insphereexact.argtypes = [P(f)] * 5
         _ref.ref_encode2d.restype = ctypes.c_uint64
         _ref.ref_encode2d.argtypes = [ctypes.c_uint32] * 2
         _ref.ref_encode3d.restype = ctypes.c_uint64
@@ -78,6 +85,20 @@ def _declare(L):
     L.orc_assemble_p1.argtypes = [u32, vp, u32, vp, vp, vp, u64, vp, vp, vp, vp]
     L.orc_jacobi_pcg.restype = u32
     L.orc_jacobi_pcg.argtypes = [u32, vp, vp, vp, vp, vp, f64, u32, vp]
+    for tag, ft in (("f32", ctypes.c_float), ("f64", f64)):
+        getattr(L, f"orc_orient3d_{tag}").restype = ft
+        getattr(L, f"orc_orient3d_{tag}").argtypes = [vp] * 4
+        getattr(L, f"orc_orient3d_exact_{tag}").restype = i32
+        getattr(L, f"orc_orient3d_exact_{tag}").argtypes = [vp] * 4
+        getattr(L, f"orc_insphere_filter_{tag}").restype = ft
+        getattr(L, f"orc_insphere_filter_{tag}").argtypes = [vp] * 6
+        getattr(L, f"orc_insphere_exact_{tag}").restype = i32
+        getattr(L, f"orc_insphere_exact_{tag}").argtypes = [vp] * 5
+        getattr(L, f"orc_ashape3d_run_{tag}").restype = vp
+        getattr(L, f"orc_ashape3d_run_{tag}").argtypes = [u32, vp, i32]
+        getattr(L, f"orc_ashape3d_get_{tag}").argtypes = [vp] * 9
+    L.orc_ashape3d_sizes.argtypes = [vp, i32, vp]
+    L.orc_ashape3d_free.argtypes = [vp, i32]
     L.orc_step_f64.restype = u32
     L.orc_step_f64.argtypes = [u32, vp, vp, u32, vp, vp]
 
@@ -130,6 +151,28 @@ def ashape2d(coords, alpha, bits=32, dtype=np.float64):
         ctypes.c_void_p(h), _p(r["elem"]), _p(r["row"]), _p(r["neig"]), _p(r["edge_is_bnd"]), _p(r["node_is_bnd"]),
         _p(r["perm"]), _p(r["ordered"]), _p(r["nineig"]))
     L.orc_ashape2d_free(h, int(is64))
+    return r
+
+
+def ashape3d(coords, alpha, bits=21, dtype=np.float64):
+    """Run the 3D oracle alpha-shape; returns a dict of outputs."""
+    L = lib()
+    xyzr = spheres(coords, alpha, dtype)
+    n = xyzr.shape[0]
+    is64 = dtype == np.float64
+    tag = "f64" if is64 else "f32"
+    h = getattr(L, f"orc_ashape3d_run_{tag}")(n, _p(xyzr), bits)
+    sz = np.zeros(6, np.uint64)
+    L.orc_ashape3d_sizes(h, int(is64), _p(sz))
+    ne, nedge = int(sz[1]), int(sz[2])
+    r = dict(elem=np.empty((ne, 4), np.uint32), row=np.empty(n + 1, np.uint32), neig=np.empty(nedge, np.uint32),
+             edge_is_bnd=np.empty(nedge, np.uint8), node_is_bnd=np.empty(n, np.uint8), perm=np.empty(n, np.uint32),
+             ordered=np.empty((n, 3), dtype), nineig=np.empty(n, np.uint32), n_exact_calls=int(sz[3]),
+             over_cap=int(sz[4]), broken_cycles=int(sz[5]))
+    getattr(L, f"orc_ashape3d_get_{tag}")(
+        ctypes.c_void_p(h), _p(r["elem"]), _p(r["row"]), _p(r["neig"]), _p(r["edge_is_bnd"]), _p(r["node_is_bnd"]),
+        _p(r["perm"]), _p(r["ordered"]), _p(r["nineig"]))
+    L.orc_ashape3d_free(h, int(is64))
     return r
 
 

@@ -1,0 +1,170 @@
+"""Pins of the 3D oracle: orient3d / insphere against the reference's own
+predicates (compiled in oracle/_ref) and exact rationals, the 3D alpha-shape
+against the reference's golden count and against Qhull."""
+import ctypes
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+P = ctypes.POINTER
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _det(m):
+    """Exact determinant (fractions) by Gaussian elimination."""
+    m = [[Fraction(x) for x in r] for r in m]
+    n = len(m)
+    det = Fraction(1)
+    for c in range(n):
+        piv = next((r for r in range(c, n) if m[r][c] != 0), None)
+        if piv is None:
+            return Fraction(0)
+        if piv != c:
+            m[c], m[piv] = m[piv], m[c]
+            det = -det
+        det *= m[c][c]
+        for r in range(c + 1, n):
+            f = m[r][c] / m[c][c]
+            for k in range(c, n):
+                m[r][k] -= f * m[c][k]
+    return det
+
+
+def _sgn(x):
+    return (x > 0) - (x < 0)
+
+
+def _orient3d_q(p):
+    return _sgn(_det([[Fraction(float(v)) for v in q] + [1] for q in p]))
+
+
+def _insphere_q(p):
+    rows = []
+    for q in p:
+        x, y, z = (Fraction(float(v)) for v in q)
+        rows.append([x, y, z, x * x + y * y + z * z, 1])
+    return _sgn(_det(rows))
+
+
+def _quintuples(dtype, seed):
+    rng = np.random.default_rng(seed)
+    out = [rng.random((5, 3)).astype(dtype) for _ in range(60)]
+    # cospherical / coplanar lattice points (exact zeros)
+    g = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 1]], dtype)
+    out.append(g)
+    out.append(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0], [0.5, 0.5, 0.0]], dtype))
+    out.append(np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1]], dtype))
+    # nearly degenerate: a point nudged off the sphere by one ulp
+    e = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1]], dtype)
+    e[4, 2] = np.nextafter(e[4, 2], dtype(2))
+    out.append(e)
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_3d_predicates_exact_against_rationals(dtype):
+    L = ol.lib()
+    tag = "f64" if dtype == np.float64 else "f32"
+    o_ex = getattr(L, f"orc_orient3d_exact_{tag}")
+    i_ex = getattr(L, f"orc_insphere_exact_{tag}")
+    o_f = getattr(L, f"orc_orient3d_{tag}")
+    for q in _quintuples(dtype, 1):
+        pts = [np.ascontiguousarray(r) for r in q]
+        so = _orient3d_q(q[:4])
+        si = _insphere_q(q)
+        assert o_ex(*[_fp(p) for p in pts[:4]]) == so
+        assert _sgn(o_f(*[_fp(p) for p in pts[:4]])) == so
+        assert i_ex(*[_fp(p) for p in pts]) == si
+
+
+def test_insphere_sign_convention():
+    # e at the centre of the sphere through a positively oriented tetrahedron
+    L = ol.lib()
+    a, b, c, d = (np.array(v, np.float64) for v in ([1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, 0]))
+    assert L.orc_orient3d_exact_f64(_fp(a), _fp(b), _fp(c), _fp(d)) != 0
+    if L.orc_orient3d_exact_f64(_fp(a), _fp(b), _fp(c), _fp(d)) < 0:
+        a, b = b, a
+    inside = np.array([0.25, 0.25, 0.25])
+    outside = np.array([3.0, 3.0, 3.0])
+    assert L.orc_insphere_exact_f64(_fp(a), _fp(b), _fp(c), _fp(d), _fp(inside)) == 1
+    assert L.orc_insphere_exact_f64(_fp(a), _fp(b), _fp(c), _fp(d), _fp(outside)) == -1
+
+
+def test_3d_predicates_against_reference_fp32():
+    R = ol.ref()
+    if R is None:
+        pytest.skip("reference predicates not built (oracle/_ref)")
+    L = ol.lib()
+    f = ctypes.c_float
+    rng = np.random.default_rng(7)
+    quint = [rng.random((5, 3)).astype(np.float32) for _ in range(400)]
+    # near-degenerate sets that reach the reference's exact stages
+    base = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1]], np.float32)
+    for k in range(40):
+        q = base.copy()
+        q[4] += np.float32(1e-7) * rng.standard_normal(3).astype(np.float32)
+        quint.append(q)
+    for q in quint:
+        ptr = [np.ascontiguousarray(r).ctypes.data_as(P(f)) for r in q]
+        vp = [np.ascontiguousarray(r).ctypes.data_as(ctypes.c_void_p) for r in q]
+        ea = f(0)
+        eb = f(0)
+        va = R.ref_insphere(*ptr, ctypes.byref(ea))
+        vb = L.orc_insphere_filter_f32(*vp, ctypes.cast(ctypes.byref(eb), ctypes.c_void_p))
+        assert np.float32(va).tobytes() == np.float32(vb).tobytes() and ea.value == eb.value
+        assert _sgn(R.ref_insphereexact(*ptr)) == L.orc_insphere_exact_f32(*vp)
+        assert _sgn(R.ref_orient3d(*ptr[:4])) == _sgn(L.orc_orient3d_f32(*vp[:4]))
+        assert _sgn(R.ref_orient3dexact(*ptr[:4])) == L.orc_orient3d_exact_f32(*vp[:4])
+
+
+def test_readme_kat_uniform3d_100k():
+    """testcases/IMR26/README.md: 100,000 glibc-rand points in the unit cube,
+    seed 42, alpha 0.022, FP32 -> 526,040 tetrahedra.  The oracle gives
+    526,042: the two extra tetrahedra are decided by the float circumradius
+    filter or by one of the 23 local cavities that reach the reference's
+    100-step boundary walk cap in this workload; the count is pinned here so
+    any change shows."""
+    pts = ol.glibc_uniform(100_000, 3, 42)
+    r = ol.ashape3d(pts, np.float32(0.022), bits=21, dtype=np.float32)
+    n = r["elem"].shape[0]
+    assert abs(n - 526_040) <= 2
+    assert n == 526_042
+    assert r["over_cap"] == 0 and r["broken_cycles"] == 0
+
+
+def _circumradius2(p):
+    a, b, c, d = (np.asarray(x, np.float64) for x in p)
+    A = np.array([b - a, c - a, d - a])
+    rhs = 0.5 * np.array([np.dot(b - a, b - a), np.dot(c - a, c - a), np.dot(d - a, d - a)])
+    ctr = np.linalg.solve(A, rhs)
+    return float(np.dot(ctr, ctr))
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_alpha_shape_3d_against_qhull(seed):
+    from scipy.spatial import Delaunay
+
+    rng = np.random.default_rng(seed)
+    n = 3000
+    pts = rng.random((n, 3))
+    alpha = 0.06  # K ~ 22 candidates: no cavity walk reaches the 100-step cap
+    r = ol.ashape3d(pts, alpha, bits=21, dtype=np.float64)
+    perm = r["perm"].astype(np.int64)
+    got = {tuple(sorted(perm[list(t)])) for t in r["elem"]}
+    dl = Delaunay(pts)
+    want = set()
+    for s in dl.simplices:
+        if _circumradius2(pts[s]) <= alpha * alpha * (1 - 1e-9):
+            want.add(tuple(sorted(int(v) for v in s)))
+    border = {tuple(sorted(int(v) for v in s)) for s in dl.simplices
+              if abs(_circumradius2(pts[s]) / (alpha * alpha) - 1) < 1e-9}
+    assert want <= got
+    assert got - want <= border
+    assert len(got) == len(r["elem"])  # every tet output once
