@@ -57,6 +57,46 @@ uint32_t AlphaShape2D_get_nexact(const AlphaShape2D* ashape);
 /* status of the last call on the handle (0 = ok) */
 int32_t AlphaShape2D_last_status(const AlphaShape2D* ashape);
 
+/* ======================== 3D alpha-shape ======================== */
+typedef struct AlphaShape3D AlphaShape3D;
+
+/* include/mesh/DirectAlphaShape3D.hpp:47 */
+AlphaShape3D* AlphaShape3D_create(void);
+/* include/mesh/DirectAlphaShape3D.hpp:48 */
+void AlphaShape3D_destroy(AlphaShape3D* ashape);
+/* include/mesh/DirectAlphaShape3D.hpp:52, src/mesh/DirectAlphaShape3D.cpp:760-772
+ * coords row-major [3N], alpha [N] */
+void AlphaShape3D_set_nodes(AlphaShape3D* ashape, uint32_t nnodes, const float* coords, const float* alpha);
+/* include/mesh/DirectAlphaShape3D.hpp:55 */
+void AlphaShape3D_init(AlphaShape3D* ashape);
+/* include/mesh/DirectAlphaShape3D.hpp:58  compute + compress */
+void AlphaShape3D_compute(AlphaShape3D* ashape);
+/* include/mesh/DirectAlphaShape3D.hpp:61 */
+uint32_t AlphaShape3D_get_nelem(const AlphaShape3D* ashape);
+/* include/mesh/DirectAlphaShape3D.hpp:64  tetrahedra [4*nelem], Morton ids */
+void AlphaShape3D_get_elem(const AlphaShape3D* ashape, uint32_t* elems);
+/* include/mesh/DirectAlphaShape3D.hpp:66  nodes [3N], Morton order */
+void AlphaShape3D_get_ordered_nodes(AlphaShape3D* ashape, float* nodes);
+
+/* ---- extensions ---- */
+/* Morton bits per axis (1..21, default 21 as lbvh.hpp:186 for dim 3). */
+int32_t AlphaShape3D_set_morton_bits(AlphaShape3D* ashape, int32_t bits);
+int32_t AlphaShape3D_set_nodes_f64(AlphaShape3D* ashape, uint32_t nnodes, const double* coords, const double* alpha);
+/* FP64 nodes already in device memory: rows (x, y, z, alpha). */
+int32_t AlphaShape3D_set_nodes_device_f64(AlphaShape3D* ashape, uint32_t nnodes, const double* d_xyzr);
+int32_t AlphaShape3D_get_ordered_nodes_f64(AlphaShape3D* ashape, double* nodes);
+/* AlphaShape3D::getEdge / getBoundaryNodes / getPermutation,
+ * src/mesh/DirectAlphaShape3D.cpp:75-105; edge_is_bnd = count <= 3 */
+uint32_t AlphaShape3D_get_nedge(const AlphaShape3D* ashape);
+int32_t AlphaShape3D_get_edges(const AlphaShape3D* ashape, uint32_t* row, uint32_t* neig, uint8_t* edge_is_bnd,
+                               uint8_t* node_is_bnd);
+int32_t AlphaShape3D_get_permutation(const AlphaShape3D* ashape, uint32_t* perm);
+/* nodes that needed the exact stage; nodes whose candidate list was clamped
+ * to 124 (the reference's 128 entries with the 4 infinity nodes, cpp:369) */
+uint32_t AlphaShape3D_get_nexact(const AlphaShape3D* ashape);
+uint32_t AlphaShape3D_get_nclamped(const AlphaShape3D* ashape);
+int32_t AlphaShape3D_last_status(const AlphaShape3D* ashape);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,22 @@ int32_t PfemStep2D_set_source_modes(PfemStep2D* step, uint32_t n_modes, const do
 int32_t PfemStep2D_advect(PfemStep2D* step, uint32_t n_modes, const double* modes, double dt, uint32_t steps);
 /* current points [N][2], input order */
 int32_t PfemStep2D_get_points(PfemStep2D* step, double* xy);
+
+/* 3D mesh-and-solve step: LBVH + alpha-shape tetrahedra + P1 pressure
+ * assembly on tetrahedra + CG (solvers 0, 1, 2 as above). */
+typedef struct PfemStep3D PfemStep3D;
+PfemStep3D* PfemStep3D_create(void);
+void PfemStep3D_destroy(PfemStep3D* step);
+/* xyz [3N], alpha [N], nodal source f [N], input order */
+int32_t PfemStep3D_set_points(PfemStep3D* step, uint32_t n, const double* xyz, const double* alpha, const double* f);
+int32_t PfemStep3D_set_morton_bits(PfemStep3D* step, int32_t bits);
+int32_t PfemStep3D_run(PfemStep3D* step, int32_t solver, double rtol, uint32_t maxit, double* stats);
+int32_t PfemStep3D_assemble(PfemStep3D* step, double* stats);
+int32_t PfemStep3D_get_solution(PfemStep3D* step, double* p);
+uint32_t PfemStep3D_nnz(PfemStep3D* step);
+uint32_t PfemStep3D_get_nelem(PfemStep3D* step);
+int32_t PfemStep3D_get_system(PfemStep3D* step, uint32_t* row, uint32_t* col, double* val, double* b);
+int32_t PfemStep3D_get_mesh(PfemStep3D* step, uint32_t* elem, uint8_t* node_is_bnd, double* xyz, uint32_t* perm);
 int32_t PfemStep2D_get_solution(PfemStep2D* step, double* p);
 int32_t PfemStep2D_get_solution_morton(PfemStep2D* step, double* p);
 uint32_t PfemStep2D_nnz(PfemStep2D* step);

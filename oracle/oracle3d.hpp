@@ -495,4 +495,93 @@ template <class S> struct AlphaShape3D {
     }
 };
 
+// --------------------------------------------------------------------------
+// P1 pressure Laplacian on tetrahedra (no reference counterpart).  Per tet,
+// with its vertices in ascending global id p0..p3: e_k = p_k - p0,
+// g1 = e2 x e3, g2 = e3 x e1, g3 = e1 x e2, g0 = -(g1 + g2 + g3),
+// det = e1 . g1, K_ab = (g_a . g_b) / (6 |det|) (= V grad phi_a . grad phi_b),
+// mass |det|/24 per vertex.  Entries are summed in increasing tet index (the
+// element order), so both endpoints of an edge produce the same bits;
+// diagonal = -(sum of the row's off-diagonals in ascending column order);
+// Dirichlet rows identity, columns zeroed; b_i = f_i * mass_i.
+// --------------------------------------------------------------------------
+inline void tet_stiffness(const double* p0, const double* p1, const double* p2, const double* p3, double K[6],
+                          double* vol) {
+    const double e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    const double e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+    const double e3[3] = {p3[0] - p0[0], p3[1] - p0[1], p3[2] - p0[2]};
+    auto cross = [](const double* a, const double* b, double* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    double g[4][3];
+    cross(e2, e3, g[1]);
+    cross(e3, e1, g[2]);
+    cross(e1, e2, g[3]);
+    for (int c = 0; c < 3; ++c) g[0][c] = -((g[1][c] + g[2][c]) + g[3][c]);
+    const double det = e1[0] * g[1][0] + e1[1] * g[1][1] + e1[2] * g[1][2];
+    const double ad = std::fabs(det);
+    const double den = 6.0 * ad;
+    static const int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {1, 2, 3, 2, 3, 3};
+    for (int k = 0; k < 6; ++k) {
+        const double* a = g[pa[k]];
+        const double* b = g[pb[k]];
+        K[k] = (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) / den;
+    }
+    *vol = ad / 6.0;
+}
+
+inline void assemble_p1_tet(uint32_t n, const double* xyz, uint32_t ne, const uint32_t* elem, const uint8_t* bnd,
+                            const double* f, CsrD& A, std::vector<double>& b) {
+    std::vector<std::vector<std::pair<uint32_t, double>>> rows(n);
+    std::vector<double> mass(n, 0.0);
+    static const int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {1, 2, 3, 2, 3, 3};
+    for (uint32_t t = 0; t < ne; ++t) {
+        uint32_t v[4] = {elem[4 * t], elem[4 * t + 1], elem[4 * t + 2], elem[4 * t + 3]};
+        std::sort(v, v + 4);
+        double K[6], vol;
+        tet_stiffness(xyz + 3 * v[0], xyz + 3 * v[1], xyz + 3 * v[2], xyz + 3 * v[3], K, &vol);
+        for (int k = 0; k < 6; ++k) {
+            rows[v[pa[k]]].push_back({v[pb[k]], K[k]});
+            rows[v[pb[k]]].push_back({v[pa[k]], K[k]});
+        }
+        for (int k = 0; k < 4; ++k) mass[v[k]] += vol / 4.0;
+    }
+    A.n = n;
+    A.row.assign(n + 1, 0);
+    A.col.clear();
+    A.val.clear();
+    b.assign(n, 0.0);
+    for (uint32_t i = 0; i < n; ++i) {
+        auto& r = rows[i];
+        std::stable_sort(r.begin(), r.end(), [](auto& x, auto& y) { return x.first < y.first; });
+        std::vector<std::pair<uint32_t, double>> m;
+        for (auto& e : r) {
+            if (!m.empty() && m.back().first == e.first)
+                m.back().second += e.second;
+            else
+                m.push_back(e);
+        }
+        double diag = 0.0;
+        for (auto& e : m) diag -= e.second;
+        bool placed = false;
+        for (auto& e : m) {
+            if (!placed && e.first > i) {
+                A.col.push_back(i);
+                A.val.push_back(bnd[i] ? 1.0 : diag);
+                placed = true;
+            }
+            A.col.push_back(e.first);
+            A.val.push_back((bnd[i] || bnd[e.first]) ? 0.0 : e.second);
+        }
+        if (!placed) {
+            A.col.push_back(i);
+            A.val.push_back(bnd[i] ? 1.0 : diag);
+        }
+        A.row[i + 1] = (uint32_t)A.col.size();
+        b[i] = bnd[i] ? 0.0 : f[i] * mass[i];
+    }
+}
+
 }  // namespace orc

@@ -182,6 +182,20 @@ int orc_insphere_exact_f64(const double* a, const double* b, const double* c, co
     return exact::insphere_sign(a, b, c, d, e);
 }
 
+// ---- P1 on tetrahedra: returns nnz; row/col/val/b must hold n+1 / cap / cap / n ----
+int64_t orc_assemble_p1_tet(uint32_t n, const double* xyz, uint32_t ne, const uint32_t* elem, const uint8_t* bnd,
+                            const double* f, uint64_t cap, uint32_t* row, uint32_t* col, double* val, double* b) {
+    CsrD A;
+    std::vector<double> bb;
+    assemble_p1_tet(n, xyz, ne, elem, bnd, f, A, bb);
+    if (A.col.size() > cap) return -(int64_t)A.col.size();
+    std::memcpy(row, A.row.data(), sizeof(uint32_t) * (n + 1));
+    std::memcpy(col, A.col.data(), sizeof(uint32_t) * A.col.size());
+    std::memcpy(val, A.val.data(), sizeof(double) * A.val.size());
+    std::memcpy(b, bb.data(), sizeof(double) * n);
+    return (int64_t)A.col.size();
+}
+
 // ---- alpha-shape 3D ----
 void* orc_ashape3d_run_f32(uint32_t n, const float* xyzr, int bits) { return ashape3d_run<float>(n, xyzr, bits); }
 void* orc_ashape3d_run_f64(uint32_t n, const double* xyzr, int bits) { return ashape3d_run<double>(n, xyzr, bits); }

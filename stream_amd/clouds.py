@@ -106,6 +106,39 @@ def divergence_source(xy: np.ndarray, modes: np.ndarray) -> np.ndarray:
     return f
 
 
+def jittered_grid3(nx: int, ny: int, nz: int, h: float, jitter: float, seed: int) -> np.ndarray:
+    """Cell-centred nx x ny x nz grid, index k = (kz*ny + ky)*nx + kx, each
+    coordinate moved by U(-jitter*h, jitter*h)."""
+    kx = np.tile(np.arange(nx, dtype=np.float64), ny * nz)
+    ky = np.tile(np.repeat(np.arange(ny, dtype=np.float64), nx), nz)
+    kz = np.repeat(np.arange(nz, dtype=np.float64), nx * ny)
+    u = uniform(seed, 3 * nx * ny * nz).reshape(-1, 3)
+    return np.stack([(kx + 0.5) * h + (2.0 * u[:, 0] - 1.0) * jitter * h,
+                     (ky + 0.5) * h + (2.0 * u[:, 1] - 1.0) * jitter * h,
+                     (kz + 0.5) * h + (2.0 * u[:, 2] - 1.0) * jitter * h], axis=1)
+
+
+TANK_NX, TANK_NY, TANK_NZ = 215, 108, 86
+
+
+def sloshing_tank(nx: int = TANK_NX, ny: int = TANK_NY, nz: int = TANK_NZ, seed: int = 2):
+    """3D sloshing-tank cloud: a jittered cubic grid (+-0.25h) filling
+    [0,1] x [0,0.5] x [0,0.4] (h = 1/nx), then z += 0.05 sin(2 pi x) z/0.4,
+    which moves the top surface by 0.05 sin(2 pi x) and keeps the bottom;
+    alpha = 1.3h.  Defaults give 1,996,920 points."""
+    h = 1.0 / nx
+    p = jittered_grid3(nx, ny, nz, h, 0.25, seed)
+    p[:, 2] += 0.05 * np.sin(2.0 * np.pi * p[:, 0]) * (p[:, 2] / 0.4)
+    return p, np.full(p.shape[0], 1.3 * h)
+
+
+def sloshing_tank_scaled(n_target: int, seed: int = 2):
+    """Smaller tank cloud with the same aspect ratio and spacing rule."""
+    s = (n_target / float(TANK_NX * TANK_NY * TANK_NZ)) ** (1.0 / 3.0)
+    return sloshing_tank(max(4, int(round(TANK_NX * s))), max(3, int(round(TANK_NY * s))),
+                         max(3, int(round(TANK_NZ * s))), seed)
+
+
 ADVECT_SEED, ADVECT_MODES, ADVECT_DT, ADVECT_STEPS = 33, 16, 1e-3, 200
 
 

@@ -1,0 +1,492 @@
+// P1 pressure Laplacian on the 3D alpha-shape tetrahedra and the 3D
+// mesh-and-solve step (no reference counterpart: the reference has no FEM
+// module and never connects mesh and solve).
+//
+// Row i is assembled by node i's thread from the tetrahedra incident to i,
+// visited in increasing element index, so the contributions to entry (i, j)
+// are summed in the same order by rows i and j and the matrix is bitwise
+// symmetric.  Per tetrahedron, with its vertices in ascending id p0..p3:
+//   e_k = p_k - p0, g1 = e2 x e3, g2 = e3 x e1, g3 = e1 x e2,
+//   g0 = -(g1 + g2 + g3), det = e1 . g1,
+//   K_ab = (g_a . g_b) / (6 |det|)   (= V grad phi_a . grad phi_b),
+//   lumped mass |det| / 24 per vertex.
+// Diagonal = -(sum of the row's off-diagonals in ascending column order);
+// Dirichlet p = 0 on free-surface nodes (identity rows, zeroed columns);
+// b_i = f_i * mass_i.  The incident-element lists are a counting sort of
+// the element array (per-node lists sorted after an atomic fill).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ashape3d.hpp"
+#include "numerics.hpp"
+#include "precond.hpp"
+
+namespace stream_amd {
+
+namespace {
+
+__global__ __launch_bounds__(256) void tet_count_kernel(uint32_t ne, const uint32_t* __restrict__ elem,
+                                                        uint32_t* __restrict__ cnt) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ne) return;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) atomicAdd(&cnt[elem[4ull * t + k]], 1u);
+}
+
+__global__ __launch_bounds__(256) void tet_fill_kernel(uint32_t ne, const uint32_t* __restrict__ elem,
+                                                       const uint32_t* __restrict__ toff, uint32_t* __restrict__ cur,
+                                                       uint32_t* __restrict__ inc) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ne) return;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t v = elem[4ull * t + k];
+        inc[toff[v] + atomicAdd(&cur[v], 1u)] = t;
+    }
+}
+
+__device__ __forceinline__ void sort4(uint32_t* v) {
+    uint32_t t;
+    if (v[0] > v[1]) { t = v[0]; v[0] = v[1]; v[1] = t; }
+    if (v[2] > v[3]) { t = v[2]; v[2] = v[3]; v[3] = t; }
+    if (v[0] > v[2]) { t = v[0]; v[0] = v[2]; v[2] = t; }
+    if (v[1] > v[3]) { t = v[1]; v[1] = v[3]; v[3] = t; }
+    if (v[1] > v[2]) { t = v[1]; v[1] = v[2]; v[2] = t; }
+}
+
+// Sort node i's incident elements and build its sorted unique column list
+// (i and the vertices of its elements) in tmp[3*toff[i] + i ...].
+__global__ __launch_bounds__(256) void tet_cols_kernel(uint32_t n, const uint32_t* __restrict__ elem,
+                                                       const uint32_t* __restrict__ toff, uint32_t* __restrict__ inc,
+                                                       uint32_t* __restrict__ tmp, uint32_t* __restrict__ ncol) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t b0 = toff[i], b1 = toff[i + 1];
+    for (uint32_t k = b0 + 1; k < b1; ++k) {
+        const uint32_t x = inc[k];
+        uint32_t p = k;
+        while (p > b0 && inc[p - 1] > x) {
+            inc[p] = inc[p - 1];
+            --p;
+        }
+        inc[p] = x;
+    }
+    uint32_t* c = tmp + 3ull * b0 + i;
+    uint32_t cnt = 1;
+    c[0] = i;
+    for (uint32_t k = b0; k < b1; ++k) {
+        const uint32_t t = inc[k];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t v = elem[4ull * t + q];
+            // sorted insert if absent
+            uint32_t lo = 0, hi = cnt;
+            while (lo < hi) {
+                const uint32_t m = (lo + hi) >> 1;
+                if (c[m] < v)
+                    lo = m + 1;
+                else
+                    hi = m;
+            }
+            if (lo < cnt && c[lo] == v) continue;
+            for (uint32_t p = cnt; p > lo; --p) c[p] = c[p - 1];
+            c[lo] = v;
+            ++cnt;
+        }
+    }
+    ncol[i] = cnt;
+}
+
+struct P1TetArgs {
+    uint32_t n;
+    const Sph4<double>* P;
+    const uint32_t* elem;
+    const uint32_t* toff;
+    const uint32_t* inc;
+    const uint32_t* tmp;
+    const uint8_t* bnd;
+    const double* f;
+    uint32_t* arow;  // filled by the scan
+    uint32_t* acol;
+    double* aval;
+    double* idiag;
+    float* idiag32;
+    double* b;
+};
+
+__global__ __launch_bounds__(256) void p1_tet_kernel(P1TetArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t r0 = a.arow[i], cnt = a.arow[i + 1] - r0;
+    const uint32_t* src = a.tmp + 3ull * a.toff[i] + i;
+    uint32_t* col = a.acol + r0;
+    double* val = a.aval + r0;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        col[k] = src[k];
+        val[k] = 0.0;
+    }
+    auto find = [&](uint32_t v) -> uint32_t {
+        uint32_t lo = 0, hi = cnt;
+        while (hi - lo > 1) {
+            const uint32_t m = (lo + hi) >> 1;
+            if (col[m] <= v)
+                lo = m;
+            else
+                hi = m;
+        }
+        return lo;
+    };
+    constexpr int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {1, 2, 3, 2, 3, 3};
+    double mass = 0.0;
+    for (uint32_t k = a.toff[i]; k < a.toff[i + 1]; ++k) {
+        const uint32_t t = a.inc[k];
+        uint32_t v[4] = {a.elem[4ull * t], a.elem[4ull * t + 1], a.elem[4ull * t + 2], a.elem[4ull * t + 3]};
+        sort4(v);
+        double p[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const Sph4<double> s = a.P[v[q]];
+            p[q][0] = s.c[0];
+            p[q][1] = s.c[1];
+            p[q][2] = s.c[2];
+        }
+        double e[3][3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) e[q][c] = p[q + 1][c] - p[0][c];
+        double g[4][3];
+        auto cross = [](const double* x, const double* y, double* o) {
+            o[0] = x[1] * y[2] - x[2] * y[1];
+            o[1] = x[2] * y[0] - x[0] * y[2];
+            o[2] = x[0] * y[1] - x[1] * y[0];
+        };
+        cross(e[1], e[2], g[1]);
+        cross(e[2], e[0], g[2]);
+        cross(e[0], e[1], g[3]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g[0][c] = -((g[1][c] + g[2][c]) + g[3][c]);
+        const double det = e[0][0] * g[1][0] + e[0][1] * g[1][1] + e[0][2] * g[1][2];
+        const double ad = fabs(det);
+        const double den = 6.0 * ad;
+        int li = 0;
+#pragma unroll
+        for (int q = 1; q < 4; ++q)
+            if (v[q] == i) li = q;
+#pragma unroll
+        for (int kk = 0; kk < 6; ++kk) {
+            if (pa[kk] != li && pb[kk] != li) continue;
+            const double* x = g[pa[kk]];
+            const double* y = g[pb[kk]];
+            const double kv = (x[0] * y[0] + x[1] * y[1] + x[2] * y[2]) / den;
+            const uint32_t other = v[pa[kk] == li ? pb[kk] : pa[kk]];
+            val[find(other)] += kv;
+        }
+        mass += (ad / 6.0) / 4.0;
+    }
+    const bool bi = a.bnd[i] != 0;
+    double diag = 0.0;
+    uint32_t dpos = 0;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        if (col[k] == i) {
+            dpos = k;
+            continue;
+        }
+        diag -= val[k];
+    }
+    for (uint32_t k = 0; k < cnt; ++k) {
+        double vv;
+        if (k == dpos)
+            vv = bi ? 1.0 : diag;
+        else
+            vv = (bi || a.bnd[col[k]]) ? 0.0 : val[k];
+        val[k] = vv;
+    }
+    const double dg = bi ? 1.0 : diag;
+    const double id = dg != 0.0 ? 1.0 / dg : 1.0;
+    a.idiag[i] = id;
+    a.idiag32[i] = (float)id;
+    a.b[i] = bi ? 0.0 : a.f[i] * mass;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void permute3_fwd_kernel(const T* __restrict__ in, const uint32_t* __restrict__ perm,
+                                                           uint32_t n, T* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[perm[i]];
+}
+template <class T>
+__global__ __launch_bounds__(256) void permute3_bwd_kernel(const T* __restrict__ in, const uint32_t* __restrict__ perm,
+                                                           uint32_t n, T* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[perm[i]] = in[i];
+}
+
+}  // namespace
+
+struct Step3D {
+    Ash3D<double> mesh;
+    DBuf<Sph4<double>> pts;  // input order
+    DBuf<double> f_in, f_m, b, x, p_out;
+    DevCsr<double> A;
+    DBuf<uint32_t> tcnt, toff, tcur, inc, tmp, ncol;
+    DBuf<uint64_t> scan_ws;
+    Sell<double> S64;
+    Sell<float> S32;
+    LowerFactor<float> L32;
+    DBuf<float> idiag32;
+    DBuf<double> idiag;
+    CgWork work;
+    SolveStats last;
+    double t_mesh = 0, t_asm = 0, t_prep = 0, t_solve = 0;
+    uint32_t n = 0;
+
+    void set_points(uint32_t n_, const double* h_xyz, const double* h_alpha, const double* h_f) {
+        n = n_;
+        std::vector<Sph4<double>> h(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            h[i].c[0] = h_xyz[3ull * i];
+            h[i].c[1] = h_xyz[3ull * i + 1];
+            h[i].c[2] = h_xyz[3ull * i + 2];
+            h[i].r = h_alpha[i];
+        }
+        pts.resize(n ? n : 1);
+        h2d(pts.p, h.data(), n);
+        f_in.resize(n ? n : 1);
+        h2d(f_in.p, h_f, n);
+    }
+
+    void assemble(hipStream_t s) {
+        const uint32_t ne = mesh.n_elems();
+        A.n = n;
+        tcnt.resize(n + 1);
+        toff.resize(n + 1);
+        tcur.resize(n + 1);
+        ncol.resize(n + 1);
+        A.row.resize(n + 1);
+        scan_ws.resize(scan_workspace_words(n + 1));
+        fill<uint32_t>(tcnt.p, n + 1, 0u, s);
+        fill<uint32_t>(tcur.p, n + 1, 0u, s);
+        if (ne) hipLaunchKernelGGL(tet_count_kernel, dim3(grid_for(ne, 256)), dim3(256), 0, s, ne, mesh.elemglob.p, tcnt.p);
+        exclusive_scan_u32(tcnt.p, toff.p, n, scan_ws.p, s);
+        inc.resize(4ull * ne + 1);
+        tmp.resize(12ull * ne + n + 1);
+        if (ne)
+            hipLaunchKernelGGL(tet_fill_kernel, dim3(grid_for(ne, 256)), dim3(256), 0, s, ne, mesh.elemglob.p, toff.p,
+                               tcur.p, inc.p);
+        hipLaunchKernelGGL(tet_cols_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, mesh.elemglob.p, toff.p,
+                           inc.p, tmp.p, ncol.p);
+        exclusive_scan_u32(ncol.p, A.row.p, n, scan_ws.p, s);
+        A.nnz = read_scalar(A.row.p + n);
+        A.col.resize(A.nnz ? A.nnz : 1);
+        A.val.resize(A.nnz ? A.nnz : 1);
+        idiag.resize(n);
+        idiag32.resize(n);
+        b.resize(n);
+        f_m.resize(n);
+        x.resize(n);
+        hipLaunchKernelGGL((permute3_fwd_kernel<double>), dim3(grid_for(n, 256)), dim3(256), 0, s, f_in.p,
+                           mesh.lbvh.perm, n, f_m.p);
+        P1TetArgs a;
+        a.n = n;
+        a.P = mesh.lbvh.obj_m.p;
+        a.elem = mesh.elemglob.p;
+        a.toff = toff.p;
+        a.inc = inc.p;
+        a.tmp = tmp.p;
+        a.bnd = mesh.node_is_bnd.p;
+        a.f = f_m.p;
+        a.arow = A.row.p;
+        a.acol = A.col.p;
+        a.aval = A.val.p;
+        a.idiag = idiag.p;
+        a.idiag32 = idiag32.p;
+        a.b = b.p;
+        hipLaunchKernelGGL(p1_tet_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, a);
+    }
+
+    void prepare(int solver, hipStream_t s) {
+        if (solver == 0) {
+            build_sell<double>(n, A.row.p, A.col.p, A.val.p, nullptr, kSellStandard, kSellOffDiag, S64, s);
+        } else {
+            build_sell<float>(n, A.row.p, A.col.p, A.val.p, nullptr, kSellRowSum, kSellOffDiag, S32, s);
+            if (solver == 2) fsai_factor<float>(A, L32, s);
+        }
+    }
+
+    void mesh_and_assemble(int bits, int solver) {
+        hipStream_t s = ctx().stream;
+        hipEvent_t ev[4];
+        for (auto& e : ev) SA_HIP(hipEventCreate(&e));
+        SA_HIP(hipEventRecord(ev[0], s));
+        mesh.bits = bits;
+        mesh.set_nodes_device(n, pts.p);
+        mesh.init(s);
+        mesh.compute(s);
+        SA_HIP(hipEventRecord(ev[1], s));
+        assemble(s);
+        SA_HIP(hipEventRecord(ev[2], s));
+        if (solver >= 0) prepare(solver, s);
+        SA_HIP(hipEventRecord(ev[3], s));
+        SA_HIP(hipEventSynchronize(ev[3]));
+        float t0 = 0, t1 = 0, t2 = 0;
+        SA_HIP(hipEventElapsedTime(&t0, ev[0], ev[1]));
+        SA_HIP(hipEventElapsedTime(&t1, ev[1], ev[2]));
+        SA_HIP(hipEventElapsedTime(&t2, ev[2], ev[3]));
+        t_mesh = t0;
+        t_asm = t1;
+        t_prep = t2;
+        for (auto& e : ev) SA_HIP(hipEventDestroy(e));
+    }
+
+    void solve(int solver, double rtol, uint32_t maxit) {
+        hipStream_t s = ctx().stream;
+        fill<double>(x.p, n, 0.0, s);
+        CgOptions o;
+        o.rtol = rtol;
+        o.maxit = maxit;
+        o.batch = tuning().batch;
+        if (solver == 0) {
+            cg_solve<double>(A, S64, false, kPrecJacobi, idiag.p, nullptr, b.p, x.p, o, work, last, s);
+        } else {
+            o.replace_factor = tuning().replace_factor;
+            if (solver == 1)
+                cg_solve<float>(A, S32, true, kPrecJacobi, idiag32.p, nullptr, b.p, x.p, o, work, last, s);
+            else
+                cg_solve<float>(A, S32, true, kPrecLLt, nullptr, &L32, b.p, x.p, o, work, last, s);
+        }
+        t_solve = last.ms;
+    }
+};
+
+}  // namespace stream_amd
+
+using namespace stream_amd;
+
+// ===========================================================================
+// PfemStep3D C-ABI (stream_numerics.h).
+// ===========================================================================
+struct PfemStep3D {
+    Step3D s;
+    int bits = 21;
+};
+
+namespace {
+template <class F> int32_t g3(F&& f) { return guarded(std::forward<F>(f)); }
+}  // namespace
+
+extern "C" {
+
+PfemStep3D* PfemStep3D_create(void) {
+    try {
+        (void)ctx();
+        return new PfemStep3D();
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return nullptr;
+    }
+}
+void PfemStep3D_destroy(PfemStep3D* h) { delete h; }
+
+// Points (row-major xyz [3N]), alpha [N], nodal source f [N], input order.
+int32_t PfemStep3D_set_points(PfemStep3D* h, uint32_t n, const double* xyz, const double* alpha, const double* f) {
+    return g3([&] { h->s.set_points(n, xyz, alpha, f); });
+}
+int32_t PfemStep3D_set_morton_bits(PfemStep3D* h, int32_t bits) {
+    return g3([&] {
+        if (bits < 1 || bits > 21) throw std::invalid_argument("3D Morton bits per axis must be in [1, 21]");
+        h->bits = bits;
+    });
+}
+
+// One 3D step: LBVH + alpha-shape tetrahedra + P1 assembly + CG.  solver:
+// 0 = FP64 Jacobi-PCG, 1 = mixed FP32/FP64 Jacobi-PCG, 2 = mixed PCG with the
+// FSAI L L^T factor.  stats [10] as PfemStep2D_run (entry 9 is 0).
+int32_t PfemStep3D_run(PfemStep3D* h, int32_t solver, double rtol, uint32_t maxit, double* stats) {
+    return g3([&] {
+        if (solver < 0 || solver > 2) throw std::invalid_argument("3D step solvers are 0, 1 and 2");
+        h->s.mesh_and_assemble(h->bits, solver);
+        h->s.solve(solver, rtol, maxit);
+        if (stats) {
+            stats[0] = h->s.t_mesh;
+            stats[1] = h->s.t_asm;
+            stats[2] = h->s.t_solve;
+            stats[3] = h->s.last.iters;
+            stats[4] = h->s.last.true_rel;
+            stats[5] = h->s.last.replacements;
+            stats[6] = h->s.A.nnz;
+            stats[7] = h->s.mesh.n_elems();
+            stats[8] = h->s.t_prep;
+            stats[9] = 0.0;
+        }
+        const uint32_t fl = read_scalar(ctx().d_flags + kFlagError);
+        if (fl) {
+            fill<uint32_t>(ctx().d_flags, 1, 0u, ctx().stream);
+            throw std::runtime_error("device error flag raised during the 3D step");
+        }
+        if (h->s.last.true_rel > rtol) throw std::runtime_error("CG did not reach the requested residual");
+    });
+}
+
+// Mesh and assemble only; stats [2] (nullable): mesh ms, assembly ms.
+int32_t PfemStep3D_assemble(PfemStep3D* h, double* stats) {
+    return g3([&] {
+        h->s.mesh_and_assemble(h->bits, -1);
+        if (stats) {
+            stats[0] = h->s.t_mesh;
+            stats[1] = h->s.t_asm;
+        }
+        const uint32_t fl = read_scalar(ctx().d_flags + kFlagError);
+        if (fl) {
+            fill<uint32_t>(ctx().d_flags, 1, 0u, ctx().stream);
+            throw std::runtime_error("device error flag raised during the 3D step");
+        }
+    });
+}
+
+int32_t PfemStep3D_get_solution(PfemStep3D* h, double* p) {
+    return g3([&] {
+        hipStream_t s = ctx().stream;
+        const uint32_t n = h->s.n;
+        h->s.p_out.resize(n ? n : 1);
+        hipLaunchKernelGGL((permute3_bwd_kernel<double>), dim3(grid_for(n, 256)), dim3(256), 0, s, h->s.x.p,
+                           h->s.mesh.lbvh.perm, n, h->s.p_out.p);
+        d2h(p, h->s.p_out.p, n);
+    });
+}
+
+uint32_t PfemStep3D_nnz(PfemStep3D* h) { return h ? h->s.A.nnz : 0u; }
+
+// Assembled system in Morton order: row [N+1], col/val [nnz], b [N].
+int32_t PfemStep3D_get_system(PfemStep3D* h, uint32_t* row, uint32_t* col, double* val, double* b) {
+    return g3([&] {
+        const uint32_t n = h->s.n, nnz = h->s.A.nnz;
+        if (row) d2h(row, h->s.A.row.p, n + 1);
+        if (col) d2h(col, h->s.A.col.p, nnz);
+        if (val) d2h(val, h->s.A.val.p, nnz);
+        if (b) d2h(b, h->s.b.p, n);
+    });
+}
+
+// Mesh outputs (Morton ids): tetrahedra [4*nelem], node_is_bnd [N],
+// Morton-ordered xyz [3N], permutation [N]; any may be null.
+int32_t PfemStep3D_get_mesh(PfemStep3D* h, uint32_t* elem, uint8_t* node_is_bnd, double* xyz, uint32_t* perm) {
+    return g3([&] {
+        auto& m = h->s.mesh;
+        const uint32_t n = m.n;
+        if (elem) d2h(elem, m.elemglob.p, 4ull * m.n_elems());
+        if (node_is_bnd) d2h(node_is_bnd, m.node_is_bnd.p, n);
+        if (perm) d2h(perm, m.lbvh.perm, n);
+        if (xyz) {
+            std::vector<Sph4<double>> v(n);
+            d2h(v.data(), m.lbvh.obj_m.p, n);
+            for (uint32_t i = 0; i < n; ++i)
+                for (int c = 0; c < 3; ++c) xyz[3 * i + c] = v[i].c[c];
+        }
+    });
+}
+
+uint32_t PfemStep3D_get_nelem(PfemStep3D* h) { return h ? h->s.mesh.n_elems() : 0u; }
+
+}  // extern "C"

@@ -86,9 +86,15 @@ __device__ __forceinline__ S incircle_filter(const S* pa, const S* pb, const S* 
 struct ExactAcc {
     double* h;  // per-lane workspace
     int n;
+    int cap = 1 << 30;  // workspace length (components)
+    bool over = false;  // a component would not fit: the sign is unreliable
 
     __device__ __forceinline__ void add(double b) {
         if (b == 0.0) return;
+        if (n + 1 > cap) {
+            over = true;
+            return;
+        }
         double q = b;
         int k = 0;
         for (int i = 0; i < n; ++i) {
@@ -102,6 +108,43 @@ struct ExactAcc {
         }
         if (q != 0.0 || k == 0) h[k++] = q;
         n = k;
+    }
+    // sign * f0 * f1 * f2 (exact, up to four components)
+    __device__ __forceinline__ void add_prod3(double f0, double f1, double f2, double sign) {
+        const double a = sign * f0;
+        const double p0 = a * f1;
+        const double p1 = fma(a, f1, -p0);
+        const double q0 = p0 * f2, q1 = fma(p0, f2, -q0);
+        const double q2 = p1 * f2, q3 = fma(p1, f2, -q2);
+        add(q0);
+        add(q1);
+        add(q2);
+        add(q3);
+    }
+    // sign * f0 * f1 * f2 * f3 * f4 (exact, up to sixteen components)
+    __device__ __forceinline__ void add_prod5(double f0, double f1, double f2, double f3, double f4,
+                                              double sign) {
+        double p[16], q[16];
+        const double a = sign * f0;
+        p[0] = a * f1;
+        p[1] = fma(a, f1, -p[0]);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            q[2 * k] = p[k] * f2;
+            q[2 * k + 1] = fma(p[k], f2, -q[2 * k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            p[2 * k] = q[k] * f3;
+            p[2 * k + 1] = fma(q[k], f3, -p[2 * k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            q[2 * k] = p[k] * f4;
+            q[2 * k + 1] = fma(p[k], f4, -q[2 * k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) add(q[k]);
     }
     // sign * f0 * f1 (exact, two components)
     __device__ __forceinline__ void add_prod2(double f0, double f1, double sign) {
@@ -175,6 +218,128 @@ __device__ __noinline__ int incircle_exact(const S* a, const S* b, const S* c, c
             for (int comp = 0; comp < 2; ++comp)
                 acc.add_prod4(P[k][comp], P[k][comp], m[t][0], m[t][1], cof * m[t][2]);
     }
+    return acc.sign();
+}
+
+// ---------------------------------------------------------------------------
+// 3D predicates (include/geometry/predicates.hpp:585-1100).
+// ---------------------------------------------------------------------------
+template <class S> struct PredBounds3;
+template <> struct PredBounds3<float> {
+    static constexpr float o3dA = (float)((7.0 + 56.0 * 5.9604644775390625e-08) * 5.9604644775390625e-08);
+    static constexpr float ispA = (float)((16.0 + 224.0 * 5.9604644775390625e-08) * 5.9604644775390625e-08);
+};
+template <> struct PredBounds3<double> {
+    static constexpr double o3dA = (7.0 + 56.0 * 1.1102230246251565e-16) * 1.1102230246251565e-16;
+    static constexpr double ispA = (16.0 + 224.0 * 1.1102230246251565e-16) * 1.1102230246251565e-16;
+};
+
+// orient3d filter (predicates.hpp:661-714): the determinant of the
+// differences to d; *certain = false when its sign is not certified.
+template <class S>
+__device__ __forceinline__ S orient3d_filter(const S* pa, const S* pb, const S* pc, const S* pd, bool* certain) {
+    const S adx = pa[0] - pd[0], bdx = pb[0] - pd[0], cdx = pc[0] - pd[0];
+    const S ady = pa[1] - pd[1], bdy = pb[1] - pd[1], cdy = pc[1] - pd[1];
+    const S adz = pa[2] - pd[2], bdz = pb[2] - pd[2], cdz = pc[2] - pd[2];
+    const S bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+    const S cdxady = cdx * ady, adxcdy = adx * cdy;
+    const S adxbdy = adx * bdy, bdxady = bdx * ady;
+    const S det = adz * (bdxcdy - cdxbdy) + bdz * (cdxady - adxcdy) + cdz * (adxbdy - bdxady);
+    const S permanent = (fabs(bdxcdy) + fabs(cdxbdy)) * fabs(adz) + (fabs(cdxady) + fabs(adxcdy)) * fabs(bdz) +
+                        (fabs(adxbdy) + fabs(bdxady)) * fabs(cdz);
+    const S errbound = PredBounds3<S>::o3dA * permanent;
+    *certain = (det > errbound) || (-det > errbound);
+    return det;
+}
+
+// insphere filter (predicates.hpp:985-1100): det and its error bound.
+template <class S>
+__device__ __forceinline__ S insphere_filter(const S* pa, const S* pb, const S* pc, const S* pd, const S* pe, S* err) {
+    const S aex = pa[0] - pe[0], bex = pb[0] - pe[0], cex = pc[0] - pe[0], dex = pd[0] - pe[0];
+    const S aey = pa[1] - pe[1], bey = pb[1] - pe[1], cey = pc[1] - pe[1], dey = pd[1] - pe[1];
+    const S aez = pa[2] - pe[2], bez = pb[2] - pe[2], cez = pc[2] - pe[2], dez = pd[2] - pe[2];
+    const S aexbey = aex * bey, bexaey = bex * aey, ab = aexbey - bexaey;
+    const S bexcey = bex * cey, cexbey = cex * bey, bc = bexcey - cexbey;
+    const S cexdey = cex * dey, dexcey = dex * cey, cd = cexdey - dexcey;
+    const S dexaey = dex * aey, aexdey = aex * dey, da = dexaey - aexdey;
+    const S aexcey = aex * cey, cexaey = cex * aey, ac = aexcey - cexaey;
+    const S bexdey = bex * dey, dexbey = dex * bey, bd = bexdey - dexbey;
+    const S abc = fma(aez, bc, fma(-bez, ac, cez * ab));
+    const S bcd = fma(bez, cd, fma(-cez, bd, dez * bc));
+    const S cda = fma(cez, da, fma(dez, ac, aez * cd));
+    const S dab = fma(dez, ab, fma(aez, bd, bez * da));
+    const S alift = fma(aex, aex, fma(aey, aey, aez * aez));
+    const S blift = fma(bex, bex, fma(bey, bey, bez * bez));
+    const S clift = fma(cex, cex, fma(cey, cey, cez * cez));
+    const S dlift = fma(dex, dex, fma(dey, dey, dez * dez));
+    const S det = fma(dlift, abc, fma(-clift, dab, fma(blift, cda, -alift * bcd)));
+    const S az = fabs(aez), bz = fabs(bez), cz = fabs(cez), dz = fabs(dez);
+    const S p_aexbey = fabs(aexbey), p_bexaey = fabs(bexaey), p_bexcey = fabs(bexcey), p_cexbey = fabs(cexbey),
+            p_cexdey = fabs(cexdey), p_dexcey = fabs(dexcey), p_dexaey = fabs(dexaey), p_aexdey = fabs(aexdey),
+            p_aexcey = fabs(aexcey), p_cexaey = fabs(cexaey), p_bexdey = fabs(bexdey), p_dexbey = fabs(dexbey);
+    const S permanent =
+        fma(alift, fma(bz, (p_cexdey + p_dexcey), fma(cz, (p_dexbey + p_bexdey), dz * (p_bexcey + p_cexbey))),
+            fma(blift, fma(cz, (p_dexaey + p_aexdey), fma(dz, (p_aexcey + p_cexaey), az * (p_cexdey + p_dexcey))),
+                fma(clift,
+                    fma(dz, (p_aexbey + p_bexaey), fma(az, (p_bexdey + p_dexbey), bz * (p_dexaey + p_aexdey))),
+                    dlift * (fma(az, (p_bexcey + p_cexbey), fma(bz, (p_cexaey + p_aexcey), cz * (p_aexbey + p_bexaey)))))));
+    *err = PredBounds3<S>::ispA * permanent;
+    return det;
+}
+
+constexpr int kPerm3[6][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {0, 2, 1}, {1, 0, 2}, {2, 1, 0}};
+
+// Exact sign of det | x y z 1 | over a, b, c, d (= sign of orient3d):
+// Laplace on the ones column, 24 monomials of degree 3.  *over: workspace
+// exhausted.
+template <class S>
+__device__ __noinline__ int orient3d_exact(const S* a, const S* b, const S* c, const S* d, double* ws, int cap,
+                                           bool* over) {
+    const double P[4][3] = {{(double)a[0], (double)a[1], (double)a[2]}, {(double)b[0], (double)b[1], (double)b[2]},
+                            {(double)c[0], (double)c[1], (double)c[2]}, {(double)d[0], (double)d[1], (double)d[2]}};
+    ExactAcc acc{ws, 0, cap};
+    for (int k = 0; k < 4; ++k) {
+        int r[3], m = 0;
+        for (int q = 0; q < 4; ++q)
+            if (q != k) r[m++] = q;
+        const double cof = (k % 2 == 0) ? -1.0 : 1.0;
+        for (int t = 0; t < 6; ++t)
+            acc.add_prod3(P[r[kPerm3[t][0]]][0], P[r[kPerm3[t][1]]][1], P[r[kPerm3[t][2]]][2],
+                          cof * (t < 3 ? 1.0 : -1.0));
+    }
+    *over |= acc.over;
+    return acc.sign();
+}
+
+// Exact sign of det | x y z (x^2+y^2+z^2) 1 | over a..e (= sign of
+// insphere): Laplace on the ones column, then on the lift column, the lift
+// split into its three squares; 360 monomials of degree 5.
+template <class S>
+__device__ __noinline__ int insphere_exact(const S* a, const S* b, const S* c, const S* d, const S* e, double* ws,
+                                           int cap, bool* over) {
+    const double P[5][3] = {{(double)a[0], (double)a[1], (double)a[2]}, {(double)b[0], (double)b[1], (double)b[2]},
+                            {(double)c[0], (double)c[1], (double)c[2]}, {(double)d[0], (double)d[1], (double)d[2]},
+                            {(double)e[0], (double)e[1], (double)e[2]}};
+    ExactAcc acc{ws, 0, cap};
+    for (int k = 0; k < 5; ++k) {
+        int R[4], m = 0;
+        for (int q = 0; q < 5; ++q)
+            if (q != k) R[m++] = q;
+        const double ck = (k % 2 == 0) ? 1.0 : -1.0;
+        for (int j = 0; j < 4; ++j) {
+            int r[3], t3 = 0;
+            for (int q = 0; q < 4; ++q)
+                if (q != j) r[t3++] = R[q];
+            const double cj = (j % 2 == 0) ? -1.0 : 1.0;
+            for (int t = 0; t < 6; ++t) {
+                const double st = t < 3 ? 1.0 : -1.0;
+                for (int comp = 0; comp < 3; ++comp)
+                    acc.add_prod5(P[R[j]][comp], P[R[j]][comp], P[r[kPerm3[t][0]]][0], P[r[kPerm3[t][1]]][1],
+                                  P[r[kPerm3[t][2]]][2], ck * cj * st);
+            }
+        }
+    }
+    *over |= acc.over;
     return acc.sign();
 }
 

@@ -131,3 +131,74 @@ class PfemStep2D:
         perm = np.empty(self.n, np.uint32)
         check(PfemStep2D_get_mesh(self.h, elem, bnd, xy, perm))
         return elem, bnd, xy, perm
+
+
+PfemStep3D_create = bind(lib, "PfemStep3D_create", _p, [])
+PfemStep3D_destroy = bind(lib, "PfemStep3D_destroy", None, [_p])
+PfemStep3D_set_points = bind(lib, "PfemStep3D_set_points", i32,
+                             [_p, u32, arr(np.float64, 2), arr(np.float64, 1), arr(np.float64, 1)])
+PfemStep3D_set_morton_bits = bind(lib, "PfemStep3D_set_morton_bits", i32, [_p, i32])
+PfemStep3D_run = bind(lib, "PfemStep3D_run", i32, [_p, i32, f64, u32, arr(np.float64, 1, writeable=True)])
+PfemStep3D_assemble = bind(lib, "PfemStep3D_assemble", i32, [_p, opt(np.float64)])
+PfemStep3D_get_solution = bind(lib, "PfemStep3D_get_solution", i32, [_p, arr(np.float64, 1, writeable=True)])
+PfemStep3D_nnz = bind(lib, "PfemStep3D_nnz", u32, [_p])
+PfemStep3D_get_nelem = bind(lib, "PfemStep3D_get_nelem", u32, [_p])
+PfemStep3D_get_system = bind(lib, "PfemStep3D_get_system", i32,
+                             [_p, opt(np.uint32), opt(np.uint32), opt(np.float64), opt(np.float64)])
+PfemStep3D_get_mesh = bind(lib, "PfemStep3D_get_mesh", i32,
+                           [_p, opt(np.uint32), opt(np.uint8), opt(np.float64), opt(np.uint32)])
+
+
+class PfemStep3D:
+    """The 3D step: LBVH + 3D alpha-shape + P1 on tetrahedra + CG (solvers
+    "fp64", "mixed", "fsai")."""
+
+    def __init__(self, xyz, alpha, f, morton_bits: int = 21):
+        self.h = PfemStep3D_create()
+        if not self.h:
+            raise StreamError(lib.Stream_last_error().decode())
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        self.n = xyz.shape[0]
+        check(PfemStep3D_set_morton_bits(self.h, morton_bits))
+        check(PfemStep3D_set_points(self.h, self.n, xyz, np.ascontiguousarray(alpha, np.float64),
+                                    np.ascontiguousarray(f, np.float64)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            PfemStep3D_destroy(self.h)
+            self.h = None
+
+    def run(self, solver="mixed", rtol=1e-8, maxit=200000):
+        st = np.zeros(10)
+        kind = SOLVER_NAMES[solver] if isinstance(solver, str) else int(solver)
+        check(PfemStep3D_run(self.h, kind, rtol, maxit, st))
+        return dict(mesh_ms=st[0], assemble_ms=st[1], solve_ms=st[2], iters=int(st[3]), true_rel=st[4],
+                    replacements=int(st[5]), nnz=int(st[6]), n_elems=int(st[7]), prep_ms=st[8])
+
+    def assemble(self):
+        st = np.zeros(2)
+        check(PfemStep3D_assemble(self.h, st))
+        return float(st[0]), float(st[1])
+
+    def solution(self):
+        p = np.empty(self.n)
+        check(PfemStep3D_get_solution(self.h, p))
+        return p
+
+    def system(self):
+        nnz = PfemStep3D_nnz(self.h)
+        row = np.empty(self.n + 1, np.uint32)
+        col = np.empty(nnz, np.uint32)
+        val = np.empty(nnz)
+        b = np.empty(self.n)
+        check(PfemStep3D_get_system(self.h, row, col, val, b))
+        return row, col, val, b
+
+    def mesh(self):
+        ne = PfemStep3D_get_nelem(self.h)
+        elem = np.empty((ne, 4), np.uint32)
+        bnd = np.empty(self.n, np.uint8)
+        xyz = np.empty((self.n, 3))
+        perm = np.empty(self.n, np.uint32)
+        check(PfemStep3D_get_mesh(self.h, elem, bnd, xyz, perm))
+        return elem, bnd, xyz, perm

@@ -98,6 +98,8 @@ def _declare(L):
         getattr(L, f"orc_ashape3d_run_{tag}").argtypes = [u32, vp, i32]
         getattr(L, f"orc_ashape3d_get_{tag}").argtypes = [vp] * 9
     L.orc_ashape3d_sizes.argtypes = [vp, i32, vp]
+    L.orc_assemble_p1_tet.restype = ctypes.c_int64
+    L.orc_assemble_p1_tet.argtypes = [u32, vp, u32, vp, vp, vp, u64, vp, vp, vp, vp]
     L.orc_ashape3d_free.argtypes = [vp, i32]
     L.orc_step_f64.restype = u32
     L.orc_step_f64.argtypes = [u32, vp, vp, u32, vp, vp]
@@ -189,6 +191,24 @@ def assemble_p1(xy, elem, bnd, f):
     val = np.empty(cap, np.float64)
     b = np.empty(n, np.float64)
     nnz = L.orc_assemble_p1(n, _p(xy), elem.shape[0], _p(elem), _p(bnd), _p(f), cap, _p(row), _p(col), _p(val), _p(b))
+    assert nnz >= 0
+    return row, col[:nnz].copy(), val[:nnz].copy(), b
+
+
+def assemble_p1_tet(xyz, elem, bnd, f):
+    L = lib()
+    n = xyz.shape[0]
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    elem = np.ascontiguousarray(elem, np.uint32)
+    bnd = np.ascontiguousarray(bnd, np.uint8)
+    f = np.ascontiguousarray(f, np.float64)
+    cap = n + 12 * elem.shape[0] + 16
+    row = np.empty(n + 1, np.uint32)
+    col = np.empty(cap, np.uint32)
+    val = np.empty(cap, np.float64)
+    b = np.empty(n, np.float64)
+    nnz = L.orc_assemble_p1_tet(n, _p(xyz), elem.shape[0], _p(elem), _p(bnd), _p(f), cap, _p(row), _p(col),
+                                _p(val), _p(b))
     assert nnz >= 0
     return row, col[:nnz].copy(), val[:nnz].copy(), b
 

@@ -1,0 +1,119 @@
+"""GPU 3D alpha-shape against the CPU oracle (oracle/oracle3d.hpp): every
+output array bit-exact, FP32 and FP64, on random clouds, the reference's
+README workload, the sloshing-tank cloud and an unjittered lattice whose
+cospherical quintuples exercise the exact stage and the simulation of
+simplicity."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from stream_amd import clouds
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu(pts, alpha, dtype, bits=21):
+    from stream_amd.mesh import AlphaShape3D
+
+    a = AlphaShape3D(dtype=dtype, morton_bits=bits)
+    a.set_nodes(np.asarray(pts, dtype), np.full(len(pts), alpha, dtype) if np.isscalar(alpha) else alpha)
+    a.compute()
+    row, neig, eb = a.get_edges()
+    return dict(elem=a.get_elem(), ordered=a.get_ordered_nodes(), perm=a.get_permutation(), row=row, neig=neig,
+                edge_is_bnd=eb, node_is_bnd=a.get_boundary_nodes(), nexact=a.get_nexact(),
+                nclamped=a.get_nclamped())
+
+
+def _compare(g, o, label):
+    print(f"{label}: {len(g['elem'])} tets, {len(g['neig'])} edges, exact nodes {g['nexact']}, "
+          f"clamped {g['nclamped']}", flush=True)
+    assert np.array_equal(g["perm"], o["perm"])
+    assert np.array_equal(g["ordered"], o["ordered"])
+    assert np.array_equal(g["row"], o["row"])
+    assert np.array_equal(g["neig"], o["neig"])
+    assert np.array_equal(g["edge_is_bnd"], o["edge_is_bnd"])
+    assert np.array_equal(g["node_is_bnd"], o["node_is_bnd"])
+    assert np.array_equal(g["elem"], o["elem"])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_random_cloud_bit_exact(dtype):
+    rng = np.random.default_rng(11)
+    pts = rng.random((20000, 3))
+    alpha = 0.045
+    o = ol.ashape3d(pts.astype(dtype), dtype(alpha), bits=21, dtype=dtype)
+    g = _gpu(pts, alpha, dtype)
+    _compare(g, o, f"random 20k {np.dtype(dtype).name}")
+
+
+def test_readme_uniform3d_100k_fp32():
+    pts = ol.glibc_uniform(100_000, 3, 42)
+    o = ol.ashape3d(pts, np.float32(0.022), bits=21, dtype=np.float32)
+    g = _gpu(pts, np.float32(0.022), np.float32)
+    _compare(g, o, "README uniform3D 100k")
+    assert len(g["elem"]) == 526_042  # README: 526,040 (see the oracle test)
+
+
+def test_sloshing_tank_scaled_fp64():
+    pts, al = clouds.sloshing_tank_scaled(60_000)
+    o = ol.ashape3d(pts, al, bits=21, dtype=np.float64)
+    g = _gpu(pts, al, np.float64)
+    _compare(g, o, f"tank {len(pts)}")
+
+
+def test_lattice_exact_path_fp64():
+    # unjittered cubic lattice: every unit cube is cospherical
+    k = np.arange(14, dtype=np.float64)
+    pts = np.stack(np.meshgrid(k, k, k, indexing="ij"), -1).reshape(-1, 3) * 0.1
+    o = ol.ashape3d(pts, 0.13, bits=21, dtype=np.float64)
+    g = _gpu(pts, 0.13, np.float64)
+    assert g["nexact"] > 0
+    _compare(g, o, "lattice 14^3")
+
+
+def test_full_tank_invariants():
+    pts, al = clouds.sloshing_tank()
+    g = _gpu(pts, al, np.float64)
+    n = len(pts)
+    e = g["elem"].astype(np.int64)
+    print(f"tank 3D {n} points: {len(e)} tets ({len(e) / n:.2f}/pt), {len(g['neig'])} edges, "
+          f"exact nodes {g['nexact']}, clamped {g['nclamped']}", flush=True)
+    assert n == 1_996_920
+    assert np.all(e[:, 0] < e[:, 1:].min(axis=1))  # owner = lowest id
+    assert np.all(e < n)
+    assert 5.0 < len(e) / n < 7.0
+    assert g["nclamped"] == 0
+
+
+def _source3(xyz):
+    return np.sin(3.0 * xyz[:, 0]) * np.cos(2.0 * xyz[:, 1]) + xyz[:, 2]
+
+
+def test_step3d_assembly_bit_exact_and_solve():
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+
+    from stream_amd.pfem import PfemStep3D
+
+    pts, al = clouds.sloshing_tank_scaled(30_000)
+    f = _source3(pts)
+    st = PfemStep3D(pts, al, f)
+    st.assemble()
+    elem, bnd, xyz, perm = st.mesh()
+    row, col, val, b = st.system()
+    o_row, o_col, o_val, o_b = ol.assemble_p1_tet(xyz, elem, bnd, f[perm])
+    assert np.array_equal(row, o_row) and np.array_equal(col, o_col)
+    assert np.array_equal(val, o_val)
+    assert np.array_equal(b, o_b)
+    n = len(pts)
+    A = sp.csr_matrix((val, col.astype(np.int64), row.astype(np.int64)), shape=(n, n))
+    xd = spl.spsolve(A.tocsc(), b)
+    for solver in ("fp64", "mixed", "fsai"):
+        r = st.run(solver, rtol=1e-8)
+        p = st.solution()
+        x = np.empty(n)
+        x[:] = p[perm]  # Morton order
+        err = np.linalg.norm(x - xd) / np.linalg.norm(xd)
+        print(f"3D tank {n}: {solver} iters={r['iters']} true_rel={r['true_rel']:.2e} err_vs_direct={err:.2e} "
+              f"mesh={r['mesh_ms']:.1f}ms asm={r['assemble_ms']:.2f}ms solve={r['solve_ms']:.1f}ms", flush=True)
+        assert r["true_rel"] <= 1e-8 and err < 1e-6

@@ -168,3 +168,45 @@ def test_alpha_shape_3d_against_qhull(seed):
     assert want <= got
     assert got - want <= border
     assert len(got) == len(r["elem"])  # every tet output once
+
+
+def test_p1_tet_closed_form_and_invariants():
+    """Reference tetrahedron: K = V grad(phi_a).grad(phi_b) with V = 1/6 and
+    gradients (-1,-1,-1), e_x, e_y, e_z.  On a tank mesh: bitwise symmetry,
+    zero row sums, and linear fields in the kernel of the interior rows
+    (patch test)."""
+    import scipy.sparse as sp
+
+    from stream_amd import clouds
+
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float64)
+    elem = np.array([[0, 1, 2, 3]], np.uint32)
+    row, col, val, b = ol.assemble_p1_tet(xyz, elem, np.zeros(4, np.uint8), np.ones(4))
+    K = np.zeros((4, 4))
+    for i in range(4):
+        for k in range(row[i], row[i + 1]):
+            K[i, col[k]] = val[k]
+    G = np.array([[-1, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float64)
+    assert np.allclose(K, G @ G.T / 6.0, rtol=0, atol=1e-15)
+    assert np.allclose(b, 1.0 / 24.0)
+
+    pts, al = clouds.sloshing_tank_scaled(8000)
+    r = ol.ashape3d(pts, al, bits=21, dtype=np.float64)
+    xyz = r["ordered"]
+    bnd = r["node_is_bnd"]
+    n = len(xyz)
+    row, col, val, b = ol.assemble_p1_tet(xyz, r["elem"], bnd, np.ones(n))
+    A = sp.csr_matrix((val, col.astype(np.int64), row.astype(np.int64)), shape=(n, n))
+    assert np.all((A - A.T).tocoo().data == 0.0)  # bitwise symmetric
+    row2, col2, val2, _ = ol.assemble_p1_tet(xyz, r["elem"], np.zeros(n, np.uint8), np.ones(n))
+    A2 = sp.csr_matrix((val2, col2.astype(np.int64), row2.astype(np.int64)), shape=(n, n))
+    lin = xyz @ np.array([0.3, -0.7, 1.1]) + 0.2
+    inner = np.flatnonzero(bnd == 0)
+    assert np.abs(A2 @ np.ones(n)).max() < 1e-12
+    # Interior rows whose star is complete pass the patch test exactly; the
+    # reference's local construction leaves a few stars inconsistent with
+    # their neighbours' (cavity walks stopped by the 100-step cap:
+    # DirectAlphaShape3D.cpp:519), so a small fraction of rows miss a tet.
+    res = np.abs((A2 @ lin)[inner])
+    assert np.sum(res > 1e-11) <= 0.01 * len(inner)
+    assert np.median(res) < 1e-15
