@@ -43,6 +43,30 @@ int32_t LBVH3D_morton_f64(uint32_t n, const double* xyzr, int32_t bits, uint64_t
 int32_t LBVH2D_collisions_f32(uint32_t n, const float* xyr, int32_t bits, uint64_t* total);
 int32_t LBVH2D_collisions_f64(uint32_t n, const double* xyr, int32_t bits, uint64_t* total);
 
+/* Agglomerative LBVH (include/geometry/lbvh_agglo.hpp:62-361, LBVHa::build):
+ * permutation [n], children [n-1][2], ranges [n-1][2] (internal node k has
+ * id k + n), parent [2n-1] (2n at the root), root, node boxes [2n-1] rows
+ * (min..., max...), split index [n-1].  Any output may be NULL. */
+int32_t LBVHa2D_build_f32(uint32_t n, const float* xyr, int32_t bits, int32_t leaf_mode, uint32_t* perm,
+                          uint32_t* children, uint32_t* range, uint32_t* parent, uint32_t* root, float* boxes,
+                          uint32_t* split);
+int32_t LBVHa2D_build_f64(uint32_t n, const double* xyr, int32_t bits, int32_t leaf_mode, uint32_t* perm,
+                          uint32_t* children, uint32_t* range, uint32_t* parent, uint32_t* root, double* boxes,
+                          uint32_t* split);
+int32_t LBVHa3D_build_f64(uint32_t n, const double* xyzr, int32_t bits, int32_t leaf_mode, uint32_t* perm,
+                          uint32_t* children, uint32_t* range, uint32_t* parent, uint32_t* root, double* boxes,
+                          uint32_t* split);
+/* overlaps with leaf buckets of up to leaf_size objects,
+ * tests/geometry/lbvh_agglo_simple_collision.cpp:100-165 */
+int32_t LBVHa2D_collisions_f32(uint32_t n, const float* xyr, int32_t bits, uint32_t leaf_size, uint64_t* total);
+int32_t LBVHa2D_collisions_f64(uint32_t n, const double* xyr, int32_t bits, uint32_t leaf_size, uint64_t* total);
+int32_t LBVHa3D_collisions_f64(uint32_t n, const double* xyzr, int32_t bits, uint32_t leaf_size, uint64_t* total);
+
+/* device TraversalMinHeap (include/geometry/traversal_stack.hpp:31-90): push
+ * n items (32 slots), pop all into out_idx/out_cost; *pushed = accepted. */
+int32_t Stream_heap_order(uint32_t n, const uint32_t* idx, const float* cost, uint32_t* out_idx, float* out_cost,
+                          int32_t* pushed);
+
 /* stable onesweep radix sort of (u64, u32) pairs over the low `bits` bits */
 int32_t Stream_sort_pairs_u64(uint32_t n, const uint64_t* keys, const uint32_t* vals, uint64_t* keys_out,
                               uint32_t* vals_out, int32_t bits);

@@ -244,6 +244,126 @@ void build_lbvh(Lbvh<S, D>& t, const std::vector<Sphere<S, D>>& obj, int bits, L
 }
 
 // --------------------------------------------------------------------------
+// Agglomerative LBVH (include/geometry/lbvh_agglo.hpp:62-361).  The leaf
+// climbs are run one after the other in leaf order; the topology depends
+// only on the deltas and the boxes only on min/max, so this gives the same
+// arrays as any parallel interleaving.
+// --------------------------------------------------------------------------
+template <class S, int D> struct LbvhA {
+    uint32_t n = 0;
+    Box<S, D> global;
+    std::vector<uint64_t> codes_sorted;
+    std::vector<uint32_t> perm;
+    std::vector<Sphere<S, D>> obj_m;
+    std::vector<uint32_t> children, range, parent, split;  // [n-1][2], [n-1][2], [2n-1], [n-1]
+    std::vector<Box<S, D>> data;                            // [2n-1]
+    uint32_t root = 0;
+};
+
+template <class S, int D, class LeafBox>
+void build_lbvha(LbvhA<S, D>& t, const std::vector<Sphere<S, D>>& obj, int bits, LeafBox leaf_box) {
+    const uint32_t n = (uint32_t)obj.size();
+    t.n = n;
+    t.global = global_box(obj);
+    std::vector<uint64_t> codes;
+    morton_codes(obj, t.global, bits, codes);
+    t.perm.resize(n);
+    for (uint32_t i = 0; i < n; ++i) t.perm[i] = i;
+    std::stable_sort(t.perm.begin(), t.perm.end(), [&](uint32_t a, uint32_t b) { return codes[a] < codes[b]; });
+    t.codes_sorted.resize(n);
+    t.obj_m.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        t.codes_sorted[i] = codes[t.perm[i]];
+        t.obj_m[i] = obj[t.perm[i]];
+    }
+    t.children.assign(2ull * (n - 1), 0);
+    t.range.assign(2ull * (n - 1), 0);
+    t.parent.assign(2ull * n - 1, 0);
+    t.split.assign(n - 1, 0);
+    t.data.assign(2ull * n - 1, Box<S, D>::empty());
+    std::vector<uint64_t> delta(n - 1);
+    for (uint32_t i = 0; i + 1 < n; ++i) delta[i] = t.codes_sorted[i] ^ t.codes_sorted[i + 1];
+    std::vector<uint32_t> touched(n - 1, 0);
+    for (uint32_t i = 0; i < n; ++i) t.data[i] = leaf_box(t.obj_m[i]);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t idx = i, imin = i, imax = i;
+        Box<S, D> data = t.data[i];
+        while (true) {
+            if (imin == 0 && imax == n - 1) {
+                t.parent[idx] = 2 * n;
+                t.root = idx;
+                break;
+            }
+            const bool right = imin == 0 || (imax != n - 1 && delta[imax] < delta[imin - 1]);
+            const uint32_t parent = right ? imax + n : imin + n - 1;
+            const uint32_t k = parent - n;
+            t.range[2 * k + (right ? 0 : 1)] = right ? imin : imax;
+            t.children[2 * k + (right ? 0 : 1)] = idx;
+            t.parent[idx] = parent;
+            if (touched[k]++) {
+                const uint32_t sib = t.children[2 * k + (right ? 1 : 0)];
+                data.combine(t.data[sib]);
+                t.data[parent] = data;
+                t.split[k] = right ? imax : imin - 1;
+                idx = parent;
+                imin = t.range[2 * k];
+                imax = t.range[2 * k + 1];
+            } else {
+                break;
+            }
+        }
+    }
+}
+
+// Collision count with leaf buckets (tests/geometry/
+// lbvh_agglo_simple_collision.cpp:105-160), self excluded.
+template <class S, int D> uint64_t lbvha_collisions(const LbvhA<S, D>& t, uint32_t leaf_size) {
+    const uint32_t n = t.n;
+    uint64_t total = 0;
+#pragma omp parallel for reduction(+ : total) schedule(dynamic, 256)
+    for (uint32_t tid = 0; tid < n; ++tid) {
+        uint32_t stack[64];
+        uint32_t ss = 0;
+        stack[ss++] = t.root;
+        const Sphere<S, D>& q = t.obj_m[tid];
+        while (ss) {
+            const uint32_t cur = stack[--ss];
+            if (cur < n) continue;
+            for (int ch = 0; ch < 2; ++ch) {
+                const uint32_t cid = t.children[2 * (cur - n) + ch];
+                uint32_t rmin = cid, rmax = cid;
+                if (cid >= n) {
+                    rmin = t.range[2 * (cid - n)];
+                    rmax = t.range[2 * (cid - n) + 1];
+                }
+                const bool leaf = cid < n || (rmax - rmin < leaf_size);
+                if (!leaf) {
+                    const Box<S, D>& b = t.data[cid];
+                    bool in = true;
+                    for (int a = 0; a < D; ++a) {
+                        in &= q.c[a] - q.r < b.hi[a];
+                        in &= q.c[a] + q.r > b.lo[a];
+                    }
+                    if (in) stack[ss++] = cid;
+                } else {
+                    for (uint32_t o = rmin; o <= rmax; ++o) {
+                        const Sphere<S, D>& s = t.obj_m[o];
+                        const S rs = q.r + s.r;
+                        S d2 = 0;
+                        for (int a = 0; a < D; ++a) {
+                            const S dd = q.c[a] - s.c[a];
+                            d2 += dd * dd;
+                        }
+                        if (d2 <= rs * rs && o != tid) ++total;
+                    }
+                }
+            }
+        }
+    }
+    return total;
+}
+
+// --------------------------------------------------------------------------
 // Robust predicates (include/geometry/predicates.hpp:342-583).
 //
 // Filters are restated operation for operation (including the fma form of

@@ -156,6 +156,48 @@ void get3d(AlphaShape3D<S>* a, uint32_t* elem, uint32_t* row, uint32_t* neig, ui
 
 extern "C" {
 
+// ---- agglomerative LBVH: outputs as LBVHa2D_build_* ----
+int orc_lbvha2d_build_f64(uint32_t n, const double* xyr, int bits, int leaf_mode, uint32_t* perm, uint32_t* children,
+                          uint32_t* range, uint32_t* parent, uint32_t* root, double* boxes, uint32_t* split) {
+    if (n < 2) return -1;
+    auto obj = pack<double, 2>(n, xyr);
+    LbvhA<double, 2> t;
+    if (leaf_mode == 0)
+        build_lbvha(t, obj, bits, [](const Sphere<double, 2>& s) { return centre_box(s); });
+    else
+        build_lbvha(t, obj, bits, [](const Sphere<double, 2>& s) { return sphere_box(s); });
+    std::memcpy(perm, t.perm.data(), 4ull * n);
+    std::memcpy(children, t.children.data(), 8ull * (n - 1));
+    std::memcpy(range, t.range.data(), 8ull * (n - 1));
+    std::memcpy(parent, t.parent.data(), 4ull * (2 * n - 1));
+    *root = t.root;
+    for (uint32_t i = 0; i < 2 * n - 1; ++i)
+        for (int a = 0; a < 2; ++a) {
+            boxes[4 * i + a] = t.data[i].lo[a];
+            boxes[4 * i + 2 + a] = t.data[i].hi[a];
+        }
+    std::memcpy(split, t.split.data(), 4ull * (n - 1));
+    return 0;
+}
+uint64_t orc_lbvha2d_collisions_f64(uint32_t n, const double* xyr, int bits, uint32_t leaf_size) {
+    auto obj = pack<double, 2>(n, xyr);
+    LbvhA<double, 2> t;
+    build_lbvha(t, obj, bits, [](const Sphere<double, 2>& s) { return sphere_box(s); });
+    return lbvha_collisions(t, leaf_size);
+}
+uint64_t orc_lbvha2d_collisions_f32(uint32_t n, const float* xyr, int bits, uint32_t leaf_size) {
+    auto obj = pack<float, 2>(n, xyr);
+    LbvhA<float, 2> t;
+    build_lbvha(t, obj, bits, [](const Sphere<float, 2>& s) { return sphere_box(s); });
+    return lbvha_collisions(t, leaf_size);
+}
+uint64_t orc_lbvha3d_collisions_f64(uint32_t n, const double* xyzr, int bits, uint32_t leaf_size) {
+    auto obj = pack<double, 3>(n, xyzr);
+    LbvhA<double, 3> t;
+    build_lbvha(t, obj, bits, [](const Sphere<double, 3>& s) { return sphere_box(s); });
+    return lbvha_collisions(t, leaf_size);
+}
+
 // ---- 3D predicates ----
 float orc_orient3d_f32(const float* a, const float* b, const float* c, const float* d) { return orient3d(a, b, c, d); }
 double orc_orient3d_f64(const double* a, const double* b, const double* c, const double* d) {

@@ -58,7 +58,7 @@ def synchronize() -> None:
 
 
 def tune(key: str, value: float) -> None:
-    """Set a solver knob: 'replace_factor' or 'batch'."""
+    """Set a solver knob: 'replace_factor', 'batch', 'cg_blocks' or 'graph'."""
     check(lib.Stream_tune(key.encode(), float(value)))
 
 

@@ -215,6 +215,34 @@ template <class S, int D> void LbvhGpu<S, D>::codes_only(const Sph4<S>* d_obj, u
 }
 
 template <class S, int D>
+void LbvhGpu<S, D>::sort_only(const Sph4<S>* d_obj, uint32_t n_, int bits_, hipStream_t s) {
+    n = n_;
+    bits = bits_;
+    if (bits < 1 || bits * D > 64) throw std::invalid_argument("Morton bits per axis out of range");
+    obj_m.resize((size_t)n + 1);
+    gbox.resize(1);
+    partials.resize(kBoxBlocks);
+    const uint32_t nn = n ? n : 1;
+    k0.resize(nn);
+    k1.resize(nn);
+    v0.resize(nn);
+    v1.resize(nn);
+    sortws.resize(radix_workspace_words(nn));
+    if (!n) return;
+    const int passes = (bits * D + 7) / 8;
+    RadixWork w = radix_work_from(sortws.p, n);
+    radix_reset(w, passes, s);
+    const unsigned nb = grid_for(n, 256, kBoxBlocks);
+    hipLaunchKernelGGL((box_partials_kernel<S, D>), dim3(nb), dim3(256), 0, s, d_obj, n, partials.p);
+    hipLaunchKernelGGL((morton_hist_kernel<S, D>), dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, d_obj, n,
+                       partials.p, nb, gbox.p, bits, k0.p, v0.p, w.hist, passes);
+    const bool in1 = radix_sort_pairs(k0.p, v0.p, k1.p, v1.p, n, passes * 8, w, ctx().d_flags, s);
+    codes_sorted = in1 ? k1.p : k0.p;
+    perm = in1 ? v1.p : v0.p;
+    hipLaunchKernelGGL((gather_kernel<S>), dim3(grid_for(n, 256)), dim3(256), 0, s, d_obj, perm, n, obj_m.p);
+}
+
+template <class S, int D>
 void LbvhGpu<S, D>::build(const Sph4<S>* d_obj, uint32_t n_, int bits_, uint32_t leaf_mode_, uint32_t extra,
                           hipStream_t s) {
     n = n_;

@@ -52,6 +52,9 @@ template <class S, int D> struct LbvhGpu {
                hipStream_t s);
     // Only steps 1-2: unsorted Morton codes into k0 (identity map in v0).
     void codes_only(const Sph4<S>* d_obj, uint32_t n, int bits, hipStream_t s);
+    // Steps 1-4 (global box, codes, stable sort, gather into obj_m), shared
+    // by the agglomerative build.
+    void sort_only(const Sph4<S>* d_obj, uint32_t n, int bits, hipStream_t s);
 };
 
 }  // namespace stream_amd

@@ -635,6 +635,31 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
     auto merge = [&] {
         if constexpr (sizeof(V) == 4) hipLaunchKernelGGL(merge_delta_kernel, grid, blk, 0, s, n, x, vv.delta);
     };
+    // one batch of o.batch iterations starting at iteration it0 (kernels
+    // only: a captured batch contains no memset or copy nodes)
+    auto launch_batch = [&](uint32_t it0) {
+        for (uint32_t k = 0; k < o.batch; ++k) {
+            const uint32_t it = it0 + k;
+            V2<V>* cur = vv.P[it & 1];
+            V2<V>* nxt = vv.P[(it + 1) & 1];
+            if constexpr (PREC == kPrecLLt) {
+                V2<V>* rqc = vv.RQ[it & 1];
+                V2<V>* rqn = vv.RQ[(it + 1) & 1];
+                hipLaunchKernelGGL((cg_dir_kernel<V, DIFF>), grid, blk, 0, s, MA, cur, nxt,
+                                   reinterpret_cast<V*>(rqc) + 1, 2u, w.sc.p, w.partials.p, w.counter.p);
+                hipLaunchKernelGGL((llt_update_kernel<V, XV>), grid, blk, 0, s, MU, xacc, nxt, rqc, rqn, vv.w,
+                                   w.sc.p, w.partials.p, w.counter.p, 0);
+                hipLaunchKernelGGL((llt_lo_kernel<V>), grid, blk, 0, s, ML, vv.w, nxt, w.sc.p, 0);
+            } else {
+                hipLaunchKernelGGL((cg_dir_kernel<V, DIFF>), grid, blk, 0, s, MA, cur, nxt, vv.q, 1u, w.sc.p,
+                                   w.partials.p, w.counter.p);
+                hipLaunchKernelGGL((cg_update_kernel<V, XV, PREC>), grid, blk, 0, s, n, xacc, vv.r, nxt, vv.q, idiag,
+                                   w.sc.p, w.partials.p, w.counter.p);
+            }
+        }
+    };
+    const bool use_graph = tuning().graph != 0;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
     residual(0, 0);
     CgScalars h = read_sc(w, s);
     if (o.min_one && h.done) {
@@ -657,24 +682,21 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
             continue;
         }
         const uint32_t it0 = h.iter;
-        for (uint32_t k = 0; k < o.batch; ++k) {
-            const uint32_t it = it0 + k;
-            V2<V>* cur = vv.P[it & 1];
-            V2<V>* nxt = vv.P[(it + 1) & 1];
-            if constexpr (PREC == kPrecLLt) {
-                V2<V>* rqc = vv.RQ[it & 1];
-                V2<V>* rqn = vv.RQ[(it + 1) & 1];
-                hipLaunchKernelGGL((cg_dir_kernel<V, DIFF>), grid, blk, 0, s, MA, cur, nxt,
-                                   reinterpret_cast<V*>(rqc) + 1, 2u, w.sc.p, w.partials.p, w.counter.p);
-                hipLaunchKernelGGL((llt_update_kernel<V, XV>), grid, blk, 0, s, MU, xacc, nxt, rqc, rqn, vv.w,
-                                   w.sc.p, w.partials.p, w.counter.p, 0);
-                hipLaunchKernelGGL((llt_lo_kernel<V>), grid, blk, 0, s, ML, vv.w, nxt, w.sc.p, 0);
-            } else {
-                hipLaunchKernelGGL((cg_dir_kernel<V, DIFF>), grid, blk, 0, s, MA, cur, nxt, vv.q, 1u, w.sc.p,
-                                   w.partials.p, w.counter.p);
-                hipLaunchKernelGGL((cg_update_kernel<V, XV, PREC>), grid, blk, 0, s, n, xacc, vv.r, nxt, vv.q, idiag,
-                                   w.sc.p, w.partials.p, w.counter.p);
+        if (use_graph) {
+            // the batch depends on it0 only through its parity (buffer
+            // ping-pong), so one graph per parity serves the whole solve
+            const uint32_t par = it0 & 1u;
+            if (!exec[par]) {
+                hipGraph_t gr = nullptr;
+                SA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                launch_batch(par);
+                SA_HIP(hipStreamEndCapture(s, &gr));
+                SA_HIP(hipGraphInstantiate(&exec[par], gr, nullptr, nullptr, 0));
+                SA_HIP(hipGraphDestroy(gr));
             }
+            SA_HIP(hipGraphLaunch(exec[par], s));
+        } else {
+            launch_batch(it0);
         }
         h = read_sc(w, s);
         if (!h.done && o.replace_factor > 0 && h.rr < o.replace_factor * o.replace_factor * rr_last_rep) {
@@ -696,6 +718,8 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
     SA_HIP(hipEventElapsedTime(&ms, e0, e1));
     SA_HIP(hipEventDestroy(e0));
     SA_HIP(hipEventDestroy(e1));
+    for (auto& ge : exec)
+        if (ge) SA_HIP(hipGraphExecDestroy(ge));
     h = read_sc(w, s);
     st.iters = h.iter;
     st.replacements = reps;

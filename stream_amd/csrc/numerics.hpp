@@ -130,6 +130,7 @@ struct Tuning {
     double replace_factor = 3e-1;  // mixed: residual replacement when ||r|| fell by this factor
     uint32_t batch = 32;           // CG iterations between host checks
     uint32_t cg_blocks = 1024;     // grid cap of the CG kernels (64..4096)
+    uint32_t graph = 0;            // 1: replay each batch of iterations as a captured HIP graph
 };
 Tuning& tuning();
 

@@ -741,6 +741,9 @@ int32_t Stream_tune(const char* key, double value) {
         } else if (k == "cg_blocks") {
             if (!(value >= 64 && value <= 4096)) throw std::invalid_argument("cg_blocks must be in [64, 4096]");
             tuning().cg_blocks = (uint32_t)value;
+        } else if (k == "graph") {
+            if (!(value == 0.0 || value == 1.0)) throw std::invalid_argument("graph must be 0 or 1");
+            tuning().graph = (uint32_t)value;
         } else {
             throw std::invalid_argument("unknown tuning key: " + k);
         }

@@ -79,3 +79,47 @@ def collisions(centres, radii, bits=None, dtype=np.float64):
     tot = ctypes.c_uint64(0)
     check(_collide[np.dtype(dtype).type](n, xyr, bits, ctypes.byref(tot)))
     return int(tot.value)
+
+
+_abuild = {
+    (2, np.float32): bind(libgeo, "LBVHa2D_build_f32", i32,
+                          [u32, opt(np.float32, False), i32, i32] + [opt(np.uint32)] * 5 + [opt(np.float32),
+                                                                                            opt(np.uint32)]),
+    (2, np.float64): bind(libgeo, "LBVHa2D_build_f64", i32,
+                          [u32, opt(np.float64, False), i32, i32] + [opt(np.uint32)] * 5 + [opt(np.float64),
+                                                                                            opt(np.uint32)]),
+    (3, np.float64): bind(libgeo, "LBVHa3D_build_f64", i32,
+                          [u32, opt(np.float64, False), i32, i32] + [opt(np.uint32)] * 5 + [opt(np.float64),
+                                                                                            opt(np.uint32)]),
+}
+_acollide = {
+    (2, np.float32): bind(libgeo, "LBVHa2D_collisions_f32", i32, [u32, opt(np.float32, False), i32, u32, vp]),
+    (2, np.float64): bind(libgeo, "LBVHa2D_collisions_f64", i32, [u32, opt(np.float64, False), i32, u32, vp]),
+    (3, np.float64): bind(libgeo, "LBVHa3D_collisions_f64", i32, [u32, opt(np.float64, False), i32, u32, vp]),
+}
+
+
+def build_agglomerative(centres, radii, bits=None, leaf_mode=LEAF_SPHERE, dtype=np.float64):
+    """Agglomerative LBVH (the reference's LBVHa); returns a dict of host arrays."""
+    xyr = _spheres(centres, radii, dtype)
+    n, dim = xyr.shape[0], xyr.shape[1] - 1
+    bits = 64 // dim if bits is None else bits
+    r = dict(perm=np.empty(n, np.uint32), children=np.empty((n - 1, 2), np.uint32),
+             range=np.empty((n - 1, 2), np.uint32), parent=np.empty(2 * n - 1, np.uint32),
+             root=np.zeros(1, np.uint32), boxes=np.empty((2 * n - 1, 2 * dim), dtype),
+             split=np.empty(n - 1, np.uint32))
+    check(_abuild[(dim, np.dtype(dtype).type)](n, xyr, bits, leaf_mode, r["perm"], r["children"], r["range"],
+                                               r["parent"], r["root"], r["boxes"], r["split"]))
+    return r
+
+
+def collisions_agglomerative(centres, radii, leaf_size=1, bits=None, dtype=np.float64):
+    """Overlaps (self excluded) through the agglomerative LBVH with leaf buckets."""
+    import ctypes
+
+    xyr = _spheres(centres, radii, dtype)
+    n, dim = xyr.shape[0], xyr.shape[1] - 1
+    bits = 64 // dim if bits is None else bits
+    tot = ctypes.c_uint64(0)
+    check(_acollide[(dim, np.dtype(dtype).type)](n, xyr, bits, leaf_size, ctypes.byref(tot)))
+    return int(tot.value)

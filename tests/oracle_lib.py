@@ -98,6 +98,11 @@ def _declare(L):
         getattr(L, f"orc_ashape3d_run_{tag}").argtypes = [u32, vp, i32]
         getattr(L, f"orc_ashape3d_get_{tag}").argtypes = [vp] * 9
     L.orc_ashape3d_sizes.argtypes = [vp, i32, vp]
+    L.orc_lbvha2d_build_f64.restype = i32
+    L.orc_lbvha2d_build_f64.argtypes = [u32, vp, i32, i32] + [vp] * 7
+    for name in ("orc_lbvha2d_collisions_f64", "orc_lbvha2d_collisions_f32", "orc_lbvha3d_collisions_f64"):
+        getattr(L, name).restype = u64
+        getattr(L, name).argtypes = [u32, vp, i32, u32]
     L.orc_assemble_p1_tet.restype = ctypes.c_int64
     L.orc_assemble_p1_tet.argtypes = [u32, vp, u32, vp, vp, vp, u64, vp, vp, vp, vp]
     L.orc_ashape3d_free.argtypes = [vp, i32]
@@ -154,6 +159,29 @@ def ashape2d(coords, alpha, bits=32, dtype=np.float64):
         _p(r["perm"]), _p(r["ordered"]), _p(r["nineig"]))
     L.orc_ashape2d_free(h, int(is64))
     return r
+
+
+def lbvha2d(coords, radii, bits=32, leaf_mode=1):
+    L = lib()
+    xyr = spheres(coords, radii, np.float64)
+    n = xyr.shape[0]
+    r = dict(perm=np.empty(n, np.uint32), children=np.empty((n - 1, 2), np.uint32),
+             range=np.empty((n - 1, 2), np.uint32), parent=np.empty(2 * n - 1, np.uint32),
+             root=np.zeros(1, np.uint32), boxes=np.empty((2 * n - 1, 4)), split=np.empty(n - 1, np.uint32))
+    rc = L.orc_lbvha2d_build_f64(n, _p(xyr), bits, leaf_mode, _p(r["perm"]), _p(r["children"]), _p(r["range"]),
+                                 _p(r["parent"]), _p(r["root"]), _p(r["boxes"]), _p(r["split"]))
+    assert rc == 0
+    return r
+
+
+def lbvha_collisions(coords, radii, leaf_size, bits=32, dtype=np.float64):
+    L = lib()
+    xyr = spheres(coords, radii, dtype)
+    d = xyr.shape[1] - 1
+    if d == 3:
+        return int(L.orc_lbvha3d_collisions_f64(xyr.shape[0], _p(xyr), bits, leaf_size))
+    fn = L.orc_lbvha2d_collisions_f64 if dtype == np.float64 else L.orc_lbvha2d_collisions_f32
+    return int(fn(xyr.shape[0], _p(xyr), bits, leaf_size))
 
 
 def ashape3d(coords, alpha, bits=21, dtype=np.float64):

@@ -197,3 +197,91 @@ def test_ashape2d_repeat_is_deterministic():
     a.compute()
     e2 = a.get_elem()
     assert np.array_equal(e1, e2)
+
+
+# ---------------------------------------------------------------------------
+# Agglomerative LBVH
+# ---------------------------------------------------------------------------
+def test_lbvha_gpu_equals_oracle():
+    from stream_amd import geometry
+
+    rng = np.random.default_rng(21)
+    n = 200_000
+    c = rng.random((n, 2))
+    g = geometry.build_agglomerative(c, 0.002, bits=32, leaf_mode=geometry.LEAF_SPHERE)
+    o = O.lbvha2d(c, 0.002, bits=32, leaf_mode=1)
+    for k in ("perm", "children", "range", "parent", "root", "split"):
+        assert np.array_equal(g[k], o[k]), k
+    assert np.array_equal(g["boxes"], o["boxes"])
+
+
+def test_lbvha_gpu_collision_kats():
+    from stream_amd import geometry
+
+    n = 100
+    c = np.stack([np.arange(n, dtype=np.float32), np.zeros(n, np.float32)], 1)
+    for leaf in (1, 2, 4, 8, 16):
+        assert geometry.collisions_agglomerative(c, np.float32(0.55), leaf, bits=32, dtype=np.float32) == 2 * n - 2
+    rng = np.random.default_rng(4)
+    c3 = rng.random((50_000, 3))
+    got = geometry.collisions_agglomerative(c3, 0.01, 4, bits=21)
+    assert got == O.lbvha_collisions(c3, 0.01, 4, bits=21)
+
+
+def _heap_ref(items):
+    """The reference's sift order (traversal_stack.hpp:50-90) in Python."""
+    h = []
+    for it in items:
+        if len(h) >= 32:
+            continue
+        h.append(it)
+        i = len(h) - 1
+        p = (i - 1) >> 1
+        while i > 0 and h[p][1] > h[i][1]:
+            h[i], h[p] = h[p], h[i]
+            i = p
+            p = (i - 1) >> 1
+    out = []
+    while h:
+        out.append(h[0])
+        h[0] = h[-1]
+        h.pop()
+        i = 0
+        while True:
+            l, r, s = 2 * i + 1, 2 * i + 2, i
+            if l < len(h) and h[l][1] < h[s][1]:
+                s = l
+            if r < len(h) and h[r][1] < h[s][1]:
+                s = r
+            if s == i:
+                break
+            h[i], h[s] = h[s], h[i]
+            i = s
+    return out
+
+
+def test_traversal_min_heap_device():
+    import ctypes
+
+    import stream_amd
+
+    fn = stream_amd.lib.Stream_heap_order
+    fn.restype = ctypes.c_int32
+    # tests/geometry/traversal_stack.cpp: costs 10..0 pushed, popped ascending
+    cases = [[(k, float(k)) for k in range(10, -1, -1)]]
+    rng = np.random.default_rng(2)
+    cases.append([(k, float(v)) for k, v in enumerate(rng.integers(0, 5, 40))])  # ties + overflow
+    for items in cases:
+        n = len(items)
+        idx = np.array([a for a, _ in items], np.uint32)
+        cost = np.array([b for _, b in items], np.float32)
+        oi = np.zeros(32, np.uint32)
+        oc = np.zeros(32, np.float32)
+        pushed = ctypes.c_int32(0)
+        stream_amd.check(fn(n, idx.ctypes.data_as(ctypes.c_void_p), cost.ctypes.data_as(ctypes.c_void_p),
+                            oi.ctypes.data_as(ctypes.c_void_p), oc.ctypes.data_as(ctypes.c_void_p),
+                            ctypes.byref(pushed)))
+        ref = _heap_ref(items)
+        assert pushed.value == len(ref)
+        assert [(int(a), float(b)) for a, b in zip(oi[:len(ref)], oc[:len(ref)])] == ref
+    assert [a for a, _ in _heap_ref(cases[0])] == list(range(11))

@@ -18,6 +18,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+import oracle_lib as ol
 from stream_amd import clouds
 
 P = ctypes.POINTER
@@ -418,3 +419,71 @@ def test_cg_against_direct_solve_square():
     err = np.linalg.norm(x - xd) / np.linalg.norm(xd)
     assert err < 1e-6, err
     assert 100 < it < 2000
+
+
+# ---------------------------------------------------------------------------
+# Agglomerative LBVH (include/geometry/lbvh_agglo.hpp)
+# ---------------------------------------------------------------------------
+def test_lbvha_line_of_spheres_kat():
+    """tests/geometry/lbvh_agglo_simple_collision.cpp:48-160: n spheres of
+    radius 0.55 on a line; root box [-0.55, n-1+0.55] x [-0.55, 0.55] and
+    2(n-2)+2 collisions for every leaf bucket size 1..16."""
+    n = 100
+    c = np.stack([np.arange(n, dtype=np.float64), np.zeros(n)], 1)
+    r = ol.lbvha2d(c, 0.55, leaf_mode=1)
+    root = int(r["root"][0])
+    assert np.allclose(r["boxes"][root], [-0.55, -0.55, n - 1 + 0.55, 0.55])
+    for leaf in (1, 2, 4, 8, 16):
+        assert ol.lbvha_collisions(c.astype(np.float32), np.float32(0.55), leaf, dtype=np.float32) == 2 * (n - 2) + 2
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_lbvha_equals_karras_radix_tree(seed):
+    """With unique codes the binary radix tree is unique: every Karras
+    internal node (range [a, b], split g) appears as agglomerative node g + n
+    with the same range, and the boxes agree."""
+    rng = np.random.default_rng(seed)
+    n = 3000
+    c = rng.random((n, 2))
+    a = ol.lbvha2d(c, 0.003, leaf_mode=1)
+    k = ol.lbvh2d(c, np.full(n, 0.003), bits=32, leaf_mode=1)
+    assert np.array_equal(a["perm"], k["perm"])
+    codes = k["codes_sorted"]
+    assert len(np.unique(codes)) == n
+    # Karras ranges by recursion from the root
+    sep = k["sep"].astype(np.int64)
+    cl = k["child_is_leaf"]
+    ranges = {}
+    stack = [(0, 0, n - 1)]
+    while stack:
+        node, lo, hi = stack.pop()
+        ranges[node] = (lo, hi)
+        g = int(sep[node])
+        if not (cl[node] & 1):
+            stack.append((g, lo, g))
+        if not (cl[node] & 2):
+            stack.append((g + 1, g + 1, hi))
+    got = {}
+    for kk in range(n - 1):
+        got[int(a["split"][kk])] = tuple(int(v) for v in a["range"][kk])
+    for node, (lo, hi) in ranges.items():
+        g = int(sep[node])
+        assert got[g] == (lo, hi)
+        assert np.allclose(a["boxes"][g + n], k["internal_box"][node])
+
+
+def test_lbvha_collisions_against_brute_force():
+    rng = np.random.default_rng(3)
+    n = 2000
+    c = rng.random((n, 2))
+    rad = 0.01 + 0.01 * rng.random(n)
+    d2 = ((c[:, None, :] - c[None, :, :]) ** 2).sum(-1)
+    rs = rad[:, None] + rad[None, :]
+    brute = int(np.sum(d2 <= rs * rs) - n)
+    for leaf in (1, 4, 16):
+        assert ol.lbvha_collisions(c, rad, leaf) == brute
+    c3 = rng.random((n, 3))
+    d2 = ((c3[:, None, :] - c3[None, :, :]) ** 2).sum(-1)
+    rs3 = 0.04
+    brute3 = int(np.sum(d2 <= (2 * rs3) ** 2) - n)
+    assert ol.lbvha_collisions(c3, rs3, 4, bits=21) == brute3
