@@ -62,6 +62,23 @@ int32_t LBVHa2D_collisions_f32(uint32_t n, const float* xyr, int32_t bits, uint3
 int32_t LBVHa2D_collisions_f64(uint32_t n, const double* xyr, int32_t bits, uint32_t leaf_size, uint64_t* total);
 int32_t LBVHa3D_collisions_f64(uint32_t n, const double* xyzr, int32_t bits, uint32_t leaf_size, uint64_t* total);
 
+/* TRBVH in float (include/geometry/trbvh.hpp:64-681, TRBVH::build,
+ * :637-679): the agglomerative topology with surface-area costs, then
+ * `rounds` treelet restructuring passes (min subtree 5, 10, 20, ...; the
+ * reference runs 3).  Outputs as LBVHa2D_build_* (range as built; the
+ * restructuring leaves it stale, as the reference does) plus node costs
+ * sah [2n-1] and subtree object counts [2n-1]; ms (NULL or [2]) gets the
+ * GPU time of the build and of the rounds.  Any output may be NULL. */
+int32_t TRBVH2D_build_f32(uint32_t n, const float* xyr, int32_t rounds, uint32_t* perm, uint32_t* children,
+                          uint32_t* range, uint32_t* parent, uint32_t* root, float* boxes, float* sah,
+                          uint32_t* subtree, double* ms);
+int32_t TRBVH3D_build_f32(uint32_t n, const float* xyzr, int32_t rounds, uint32_t* perm, uint32_t* children,
+                          uint32_t* range, uint32_t* parent, uint32_t* root, float* boxes, float* sah,
+                          uint32_t* subtree, double* ms);
+/* overlaps (self excluded), tests/geometry/trbvh_random_collision.cpp:87-139 */
+int32_t TRBVH2D_collisions_f32(uint32_t n, const float* xyr, int32_t rounds, uint64_t* total);
+int32_t TRBVH3D_collisions_f32(uint32_t n, const float* xyzr, int32_t rounds, uint64_t* total);
+
 /* device TraversalMinHeap (include/geometry/traversal_stack.hpp:31-90): push
  * n items (32 slots), pop all into out_idx/out_cost; *pushed = accepted. */
 int32_t Stream_heap_order(uint32_t n, const uint32_t* idx, const float* cost, uint32_t* out_idx, float* out_cost,

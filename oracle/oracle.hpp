@@ -364,6 +364,257 @@ template <class S, int D> uint64_t lbvha_collisions(const LbvhA<S, D>& t, uint32
 }
 
 // --------------------------------------------------------------------------
+// TRBVH (include/geometry/trbvh.hpp:64-681): the agglomerative topology with
+// surface-area costs and subtree sizes (:294-388), then treelet
+// restructuring rounds (:390-633) with minimum subtree sizes 5, 10, 20
+// (:670-674).  fp_tt = float, the only build of the reference.
+//
+// Every parallel loop of the reference is run here leaf after leaf.  Two
+// facts make that a faithful restatement:
+//  * the cost sum at a node depends on which child's thread arrives second
+//    (that thread's running cost counts its own leaf's area, the sibling's
+//    stored cost does not, :340-343 vs :321-323).  In leaf order the second
+//    arrival always comes from the right child; a parallel run of the
+//    reference may pick either, this restatement (and the GPU build) fixes
+//    the right child;
+//  * a treelet rooted at a node is formed only after both of the node's
+//    children are final, and it touches that node's subtree only, so the
+//    rounds do not depend on the interleaving.
+// One guard departs from the reference: the leaf chosen for expansion is the
+// costliest treelet leaf among internal nodes.  The reference picks the
+// costliest of all treelet leaves (:432-439) and would index an object leaf
+// as an internal node when every internal cost is zero (boxes of zero
+// area); whenever some internal leaf carries the maximum the two agree.
+// --------------------------------------------------------------------------
+constexpr float kTrbvhCi = 1.2f;  // trbvh.hpp:72
+constexpr float kTrbvhCt = 1.0f;  // trbvh.hpp:73
+constexpr int kTreeletLeaves = 5; // trbvh.hpp:74
+
+template <int D> struct Trbvh {
+    LbvhA<float, D> t;              // topology, ranges (as built), parents, root, boxes
+    std::vector<float> sah;         // [2n-1]
+    std::vector<uint32_t> subtree;  // [2n-1]
+};
+
+// Area used by the reference in every dimension: ext[0] * ext[1].
+template <int D> float trbvh_area(const Box<float, D>& b) {
+    const float e0 = b.hi[0] - b.lo[0];
+    const float e1 = b.hi[1] - b.lo[1];
+    return e0 * e1;
+}
+
+// Costs and subtree sizes of the freshly built hierarchy (trbvh.hpp:294-388):
+// leaf cost 0 and size 1 (:283-284, :323); a thread's running cost starts
+// at Ct * area(own object) (:321) and gains Ci * area(node) + cost(sibling)
+// at every node it completes (:340-343).
+template <int D> void trbvh_costs(Trbvh<D>& T) {
+    const uint32_t n = T.t.n;
+    T.sah.assign(2ull * n - 1, 0.0f);
+    T.subtree.assign(2ull * n - 1, 1u);
+    std::vector<uint32_t> touched(n - 1, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t idx = i, sub = 1;
+        const Box<float, D>& bl = T.t.data[i];
+        float run = (kTrbvhCt * (bl.hi[0] - bl.lo[0])) * (bl.hi[1] - bl.lo[1]);
+        while (idx != T.t.root) {
+            const uint32_t p = T.t.parent[idx];
+            const uint32_t k = p - n;
+            if (touched[k]++ == 0) break;
+            const uint32_t l = T.t.children[2 * k], r = T.t.children[2 * k + 1];
+            const uint32_t sib = l == idx ? r : l;
+            sub += T.subtree[sib];
+            const Box<float, D>& b = T.t.data[p];
+            const float node = (kTrbvhCi * (b.hi[0] - b.lo[0])) * (b.hi[1] - b.lo[1]);
+            run = run + (node + T.sah[sib]);
+            T.sah[p] = run;
+            T.subtree[p] = sub;
+            idx = p;
+        }
+    }
+}
+
+// Optimal partition of a treelet of 5 leaves (trbvh.hpp:460-533): surface
+// areas of the 31 leaf subsets, then for every subset s of two or more
+// leaves c(s) = min(Ci A(s) + min over splits (c(p) + c(s^p)), Ct A(s) N(s))
+// with N(s) the number of objects under s.  copt/popt/size get 32 entries
+// (index = leaf subset bit mask).
+template <int D>
+void treelet_dp(const Box<float, D>* lbox, const float* cost, const uint32_t* lsize, float* copt, uint32_t* popt,
+                uint32_t* size) {
+    constexpr uint32_t kAll = (1u << kTreeletLeaves) - 1;
+    float area[kAll + 1] = {};
+    for (uint32_t s = 0; s <= kAll; ++s) {
+        copt[s] = 0.0f;
+        popt[s] = 0;
+        size[s] = 0;
+    }
+    for (uint32_t s = 1; s <= kAll; ++s) {
+        Box<float, D> b = Box<float, D>::empty();
+        for (int l = 0; l < kTreeletLeaves; ++l)
+            if (s & (1u << l)) {
+                b.combine(lbox[l]);
+                size[s] += lsize[l];
+            }
+        area[s] = trbvh_area<D>(b);
+    }
+    for (int l = 0; l < kTreeletLeaves; ++l) copt[1u << l] = cost[l];
+    // Karras & Aila 2013 (HPG), partition search: the lowest member of s
+    // stays on one side, p runs over the non-empty subsets of the others in
+    // increasing order; ties keep the first.
+    // Subsets are visited by increasing size; within one size the order is
+    // immaterial (each depends on smaller subsets only).
+    for (int k = 2; k <= kTreeletLeaves; ++k) {
+        for (uint32_t s = 3; s <= kAll; ++s) {
+            if (__builtin_popcount(s) != k) continue;
+            float cs = std::numeric_limits<float>::max();
+            uint32_t ps = 0;
+            const uint32_t delta = (s - 1) & s;
+            for (uint32_t p = (0u - delta) & s; p; p = (p - delta) & s) {
+                const float c = copt[p] + copt[s ^ p];
+                if (c < cs) {
+                    cs = c;
+                    ps = p;
+                }
+            }
+            const float split = kTrbvhCi * area[s] + cs;
+            const float flat = kTrbvhCt * area[s] * (float)size[s];
+            copt[s] = std::fmin(split, flat);
+            popt[s] = ps;
+        }
+    }
+}
+
+// One treelet at internal node idx (trbvh.hpp:408-605).
+template <int D> void trbvh_treelet(Trbvh<D>& T, uint32_t idx) {
+    const uint32_t n = T.t.n;
+    auto child = [&](uint32_t node, int side) { return T.t.children[2 * (node - n) + side]; };
+    uint32_t inner[kTreeletLeaves - 1] = {};
+    uint32_t leaf[kTreeletLeaves] = {};
+    float cost[kTreeletLeaves] = {};
+    inner[0] = idx;
+    leaf[0] = child(idx, 0);
+    leaf[1] = child(idx, 1);
+    cost[0] = T.sah[leaf[0]];
+    cost[1] = T.sah[leaf[1]];
+    for (int m = 1; m < kTreeletLeaves - 1; ++m) {
+        int best = -1;
+        for (int l = 0; l <= m; ++l)
+            if (leaf[l] >= n && (best < 0 || cost[l] > cost[best])) best = l;
+        const uint32_t b = leaf[best];
+        inner[m] = b;
+        leaf[best] = leaf[m];
+        cost[best] = cost[m];
+        leaf[m] = child(b, 0);
+        cost[m] = T.sah[leaf[m]];
+        leaf[m + 1] = child(b, 1);
+        cost[m + 1] = T.sah[leaf[m + 1]];
+    }
+    constexpr uint32_t kAll = (1u << kTreeletLeaves) - 1;
+    Box<float, D> lbox[kTreeletLeaves];
+    uint32_t lsize[kTreeletLeaves];
+    for (int l = 0; l < kTreeletLeaves; ++l) {
+        lbox[l] = T.t.data[leaf[l]];
+        lsize[l] = T.subtree[leaf[l]];
+    }
+    float copt[kAll + 1];
+    uint32_t popt[kAll + 1], size[kAll + 1];
+    treelet_dp<D>(lbox, cost, lsize, copt, popt, size);
+    // Rebuild from the partitions (:540-602): the partition p_opt goes to
+    // the right child, its complement to the left; internal nodes are
+    // reused in the order they come off the stack.
+    T.sah[idx] = copt[kAll];
+    struct Item {
+        uint32_t part, right, parent;
+    } st[kTreeletLeaves];
+    int ss = 0, used = 0;
+    st[ss++] = {popt[kAll], 1u, idx};
+    st[ss++] = {kAll & ~popt[kAll], 0u, idx};
+    while (ss) {
+        const Item it = st[--ss];
+        const bool is_leaf = __builtin_popcount(it.part) == 1;
+        const uint32_t node = is_leaf ? leaf[__builtin_ctz(it.part)] : inner[++used];
+        T.t.children[2 * (it.parent - n) + it.right] = node;
+        T.t.parent[node] = it.parent;
+        if (!is_leaf) {
+            Box<float, D> b = Box<float, D>::empty();
+            for (int l = 0; l < kTreeletLeaves; ++l)
+                if (it.part & (1u << l)) b.combine(T.t.data[leaf[l]]);
+            T.t.data[node] = b;
+            T.subtree[node] = size[it.part];
+            T.sah[node] = copt[it.part];
+            const uint32_t pl = popt[it.part];
+            st[ss++] = {pl, 1u, node};
+            st[ss++] = {it.part & ~pl, 0u, node};
+        }
+    }
+}
+
+// One restructuring round (trbvh.hpp:390-633), leaf after leaf.
+template <int D> void trbvh_round(Trbvh<D>& T, uint32_t min_size) {
+    const uint32_t n = T.t.n;
+    std::vector<uint32_t> touched(n - 1, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t idx = i, parent = T.t.parent[i], sub = T.subtree[i];
+        while (true) {
+            if (sub >= min_size) trbvh_treelet<D>(T, idx);
+            if (parent == 2 * n) break;
+            if (touched[parent - n]++ == 0) break;
+            idx = parent;
+            sub = T.subtree[idx];
+            parent = T.t.parent[idx];
+        }
+    }
+}
+
+// Full build (trbvh.hpp:637-679); rounds = 3 in the reference.
+template <int D> void build_trbvh(Trbvh<D>& T, const std::vector<Sphere<float, D>>& obj, int rounds) {
+    build_lbvha(T.t, obj, 64 / D, [](const Sphere<float, D>& s) { return sphere_box(s); });
+    trbvh_costs<D>(T);
+    uint32_t min_size = kTreeletLeaves;
+    for (int r = 0; r < rounds; ++r, min_size *= 2) trbvh_round<D>(T, min_size);
+}
+
+// Collision count over a TRBVH (tests/geometry/trbvh_random_collision.cpp:
+// 87-139): leaves are objects, self excluded.
+template <int D> uint64_t trbvh_collisions(const Trbvh<D>& T) {
+    const uint32_t n = T.t.n;
+    uint64_t total = 0;
+#pragma omp parallel for reduction(+ : total) schedule(dynamic, 256)
+    for (uint32_t tid = 0; tid < n; ++tid) {
+        std::vector<uint32_t> stack;
+        stack.push_back(T.t.root);
+        const Sphere<float, D>& q = T.t.obj_m[tid];
+        while (!stack.empty()) {
+            const uint32_t cur = stack.back();
+            stack.pop_back();
+            if (cur < n) continue;
+            for (int ch = 0; ch < 2; ++ch) {
+                const uint32_t cid = T.t.children[2 * (cur - n) + ch];
+                if (cid >= n) {
+                    const Box<float, D>& b = T.t.data[cid];
+                    bool in = true;
+                    for (int a = 0; a < D; ++a) {
+                        in &= q.c[a] - q.r < b.hi[a];
+                        in &= q.c[a] + q.r > b.lo[a];
+                    }
+                    if (in) stack.push_back(cid);
+                } else {
+                    const Sphere<float, D>& s = T.t.obj_m[cid];
+                    const float rs = q.r + s.r;
+                    float d2 = 0;
+                    for (int a = 0; a < D; ++a) {
+                        const float dd = q.c[a] - s.c[a];
+                        d2 += dd * dd;
+                    }
+                    if (d2 <= rs * rs && cid != tid) ++total;
+                }
+            }
+        }
+    }
+    return total;
+}
+
+// --------------------------------------------------------------------------
 // Robust predicates (include/geometry/predicates.hpp:342-583).
 //
 // Filters are restated operation for operation (including the fma form of

@@ -152,6 +152,27 @@ void get3d(AlphaShape3D<S>* a, uint32_t* elem, uint32_t* row, uint32_t* neig, ui
     if (nineig)
         for (uint32_t i = 0; i < n; ++i) nineig[i] = a->nineig[i];
 }
+template <int D>
+int trbvh_build(uint32_t n, const float* xyr, int rounds, uint32_t* perm, uint32_t* children, uint32_t* range,
+                uint32_t* parent, uint32_t* root, float* boxes, float* sah, uint32_t* subtree) {
+    if (n < 2 || rounds < 0) return -1;
+    Trbvh<D> T;
+    build_trbvh<D>(T, pack<float, D>(n, xyr), rounds);
+    if (perm) std::memcpy(perm, T.t.perm.data(), 4ull * n);
+    if (children) std::memcpy(children, T.t.children.data(), 8ull * (n - 1));
+    if (range) std::memcpy(range, T.t.range.data(), 8ull * (n - 1));
+    if (parent) std::memcpy(parent, T.t.parent.data(), 4ull * (2 * n - 1));
+    if (root) *root = T.t.root;
+    if (boxes)
+        for (uint32_t i = 0; i < 2 * n - 1; ++i)
+            for (int a = 0; a < D; ++a) {
+                boxes[2 * D * i + a] = T.t.data[i].lo[a];
+                boxes[2 * D * i + D + a] = T.t.data[i].hi[a];
+            }
+    if (sah) std::memcpy(sah, T.sah.data(), 4ull * (2 * n - 1));
+    if (subtree) std::memcpy(subtree, T.subtree.data(), 4ull * (2 * n - 1));
+    return 0;
+}
 }  // namespace
 
 extern "C" {
@@ -196,6 +217,41 @@ uint64_t orc_lbvha3d_collisions_f64(uint32_t n, const double* xyzr, int bits, ui
     LbvhA<double, 3> t;
     build_lbvha(t, obj, bits, [](const Sphere<double, 3>& s) { return sphere_box(s); });
     return lbvha_collisions(t, leaf_size);
+}
+
+// ---- TRBVH (float, the reference's only build) ----
+int orc_trbvh2d_build_f32(uint32_t n, const float* xyr, int rounds, uint32_t* perm, uint32_t* children,
+                          uint32_t* range, uint32_t* parent, uint32_t* root, float* boxes, float* sah,
+                          uint32_t* subtree) {
+    return trbvh_build<2>(n, xyr, rounds, perm, children, range, parent, root, boxes, sah, subtree);
+}
+int orc_trbvh3d_build_f32(uint32_t n, const float* xyzr, int rounds, uint32_t* perm, uint32_t* children,
+                          uint32_t* range, uint32_t* parent, uint32_t* root, float* boxes, float* sah,
+                          uint32_t* subtree) {
+    return trbvh_build<3>(n, xyzr, rounds, perm, children, range, parent, root, boxes, sah, subtree);
+}
+uint64_t orc_trbvh2d_collisions_f32(uint32_t n, const float* xyr, int rounds) {
+    Trbvh<2> T;
+    build_trbvh<2>(T, pack<float, 2>(n, xyr), rounds);
+    return trbvh_collisions<2>(T);
+}
+uint64_t orc_trbvh3d_collisions_f32(uint32_t n, const float* xyzr, int rounds) {
+    Trbvh<3> T;
+    build_trbvh<3>(T, pack<float, 3>(n, xyzr), rounds);
+    return trbvh_collisions<3>(T);
+}
+// The treelet optimiser's dynamic programme on 5 leaves given as box rows
+// (lo..., hi...), leaf costs and subtree sizes (trbvh.hpp:460-533):
+// copt/popt/size get 32 entries indexed by leaf-subset bit mask.
+void orc_treelet_dp2d(const float* boxes, const float* cost, const uint32_t* lsize, float* copt, uint32_t* popt,
+                      uint32_t* size) {
+    Box<float, 2> b[kTreeletLeaves];
+    for (int l = 0; l < kTreeletLeaves; ++l)
+        for (int a = 0; a < 2; ++a) {
+            b[l].lo[a] = boxes[4 * l + a];
+            b[l].hi[a] = boxes[4 * l + 2 + a];
+        }
+    treelet_dp<2>(b, cost, lsize, copt, popt, size);
 }
 
 // ---- 3D predicates ----

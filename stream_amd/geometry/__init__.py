@@ -123,3 +123,44 @@ def collisions_agglomerative(centres, radii, leaf_size=1, bits=None, dtype=np.fl
     tot = ctypes.c_uint64(0)
     check(_acollide[(dim, np.dtype(dtype).type)](n, xyr, bits, leaf_size, ctypes.byref(tot)))
     return int(tot.value)
+
+
+_tbuild = {
+    2: bind(libgeo, "TRBVH2D_build_f32", i32,
+            [u32, opt(np.float32, False), i32] + [opt(np.uint32)] * 5 + [opt(np.float32)] * 2
+            + [opt(np.uint32), opt(np.float64)]),
+    3: bind(libgeo, "TRBVH3D_build_f32", i32,
+            [u32, opt(np.float32, False), i32] + [opt(np.uint32)] * 5 + [opt(np.float32)] * 2
+            + [opt(np.uint32), opt(np.float64)]),
+}
+_tcollide = {
+    2: bind(libgeo, "TRBVH2D_collisions_f32", i32, [u32, opt(np.float32, False), i32, vp]),
+    3: bind(libgeo, "TRBVH3D_collisions_f32", i32, [u32, opt(np.float32, False), i32, vp]),
+}
+
+
+def build_trbvh(centres, radii, rounds=3):
+    """TRBVH (the reference's treelet-restructured BVH, float); returns a
+    dict of host arrays: perm, children, range, parent, root, boxes, sah,
+    subtree and ms = [build, rounds] GPU milliseconds."""
+    xyr = _spheres(centres, radii, np.float32)
+    n, dim = xyr.shape[0], xyr.shape[1] - 1
+    r = dict(perm=np.empty(n, np.uint32), children=np.empty((n - 1, 2), np.uint32),
+             range=np.empty((n - 1, 2), np.uint32), parent=np.empty(2 * n - 1, np.uint32),
+             root=np.zeros(1, np.uint32), boxes=np.empty((2 * n - 1, 2 * dim), np.float32),
+             sah=np.empty(2 * n - 1, np.float32), subtree=np.empty(2 * n - 1, np.uint32),
+             ms=np.zeros(2, np.float64))
+    check(_tbuild[dim](n, xyr, rounds, r["perm"], r["children"], r["range"], r["parent"], r["root"], r["boxes"],
+                       r["sah"], r["subtree"], r["ms"]))
+    return r
+
+
+def collisions_trbvh(centres, radii, rounds=3):
+    """Overlaps (self excluded) through the TRBVH."""
+    import ctypes
+
+    xyr = _spheres(centres, radii, np.float32)
+    n, dim = xyr.shape[0], xyr.shape[1] - 1
+    tot = ctypes.c_uint64(0)
+    check(_tcollide[dim](n, xyr, rounds, ctypes.byref(tot)))
+    return int(tot.value)

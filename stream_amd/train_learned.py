@@ -68,6 +68,8 @@ def train(step, pts, al, net, epochs=4, per_snapshot=2, train_steps=150, lr=2e-3
                 loss = net.train_step(rate, read_loss=last)
                 k += 1
             losses.append(loss)
+            if s % 50 == 0:
+                log(f"[train] epoch {ep} step {s}: loss {loss:.5f} {time.perf_counter() - t0:.1f}s")
             step.advect(modes, dt, 1)
         history.append(float(np.mean(losses)))
         log(f"[train] epoch {ep}: mean loss {history[-1]:.5f} (first {losses[0]:.5f}, last {losses[-1]:.5f}) "

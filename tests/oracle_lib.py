@@ -103,6 +103,13 @@ def _declare(L):
     for name in ("orc_lbvha2d_collisions_f64", "orc_lbvha2d_collisions_f32", "orc_lbvha3d_collisions_f64"):
         getattr(L, name).restype = u64
         getattr(L, name).argtypes = [u32, vp, i32, u32]
+    for d in (2, 3):
+        getattr(L, f"orc_trbvh{d}d_build_f32").restype = i32
+        getattr(L, f"orc_trbvh{d}d_build_f32").argtypes = [u32, vp, i32] + [vp] * 8
+        getattr(L, f"orc_trbvh{d}d_collisions_f32").restype = u64
+        getattr(L, f"orc_trbvh{d}d_collisions_f32").argtypes = [u32, vp, i32]
+    L.orc_treelet_dp2d.restype = None
+    L.orc_treelet_dp2d.argtypes = [vp] * 6
     L.orc_assemble_p1_tet.restype = ctypes.c_int64
     L.orc_assemble_p1_tet.argtypes = [u32, vp, u32, vp, vp, vp, u64, vp, vp, vp, vp]
     L.orc_ashape3d_free.argtypes = [vp, i32]
@@ -184,6 +191,43 @@ def lbvha_collisions(coords, radii, leaf_size, bits=32, dtype=np.float64):
     return int(fn(xyr.shape[0], _p(xyr), bits, leaf_size))
 
 
+def trbvh(coords, radii, rounds=3):
+    """Oracle TRBVH (float): dict of perm, children, range, parent, root,
+    boxes, sah, subtree."""
+    L = lib()
+    xyr = spheres(coords, radii, np.float32)
+    n, d = xyr.shape[0], xyr.shape[1] - 1
+    r = dict(perm=np.empty(n, np.uint32), children=np.empty((n - 1, 2), np.uint32),
+             range=np.empty((n - 1, 2), np.uint32), parent=np.empty(2 * n - 1, np.uint32),
+             root=np.zeros(1, np.uint32), boxes=np.empty((2 * n - 1, 2 * d), np.float32),
+             sah=np.empty(2 * n - 1, np.float32), subtree=np.empty(2 * n - 1, np.uint32))
+    fn = getattr(L, f"orc_trbvh{d}d_build_f32")
+    rc = fn(n, _p(xyr), rounds, _p(r["perm"]), _p(r["children"]), _p(r["range"]), _p(r["parent"]), _p(r["root"]),
+            _p(r["boxes"]), _p(r["sah"]), _p(r["subtree"]))
+    assert rc == 0
+    return r
+
+
+def trbvh_collisions(coords, radii, rounds=3):
+    L = lib()
+    xyr = spheres(coords, radii, np.float32)
+    d = xyr.shape[1] - 1
+    return int(getattr(L, f"orc_trbvh{d}d_collisions_f32")(xyr.shape[0], _p(xyr), rounds))
+
+
+def treelet_dp(boxes, cost, size):
+    """The treelet optimiser's subset programme on 5 leaves (2D boxes as rows
+    lo0, lo1, hi0, hi1): returns (copt[32], popt[32], size[32])."""
+    b = np.ascontiguousarray(boxes, np.float32)
+    c = np.ascontiguousarray(cost, np.float32)
+    z = np.ascontiguousarray(size, np.uint32)
+    copt = np.zeros(32, np.float32)
+    popt = np.zeros(32, np.uint32)
+    sz = np.zeros(32, np.uint32)
+    lib().orc_treelet_dp2d(_p(b), _p(c), _p(z), _p(copt), _p(popt), _p(sz))
+    return copt, popt, sz
+
+
 def ashape3d(coords, alpha, bits=21, dtype=np.float64):
     """Run the 3D oracle alpha-shape; returns a dict of outputs."""
     L = lib()
@@ -262,3 +306,16 @@ def glibc_uniform(n, dim, seed=42):
     rnd = libc.rand
     r = np.fromiter((rnd() for _ in range(n * dim)), dtype=np.int64, count=n * dim)
     return (r.astype(np.float32) / np.float32(2147483647)).reshape(n, dim)
+
+
+def trbvh_test_spheres(n):
+    """Spheres as tests/geometry/trbvh_random_collision.cpp:26-34 draws them
+    (glibc rand() from its default seed): per sphere x, y = rand()/RAND_MAX,
+    r = 0.25 / sqrt(n) * (1 + rand()/RAND_MAX), all in float."""
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(ctypes.c_uint(1))
+    r = np.fromiter((libc.rand() for _ in range(3 * n)), dtype=np.int64, count=3 * n).reshape(n, 3)
+    u = r.astype(np.float32) / np.float32(2147483647)
+    min_r = np.float32(1.0) / np.sqrt(np.float32(n))
+    rad = (np.float32(0.25) * min_r) * (np.float32(1.0) + u[:, 2])
+    return np.ascontiguousarray(u[:, :2]), rad.astype(np.float32)

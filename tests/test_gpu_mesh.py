@@ -228,6 +228,75 @@ def test_lbvha_gpu_collision_kats():
     assert got == O.lbvha_collisions(c3, 0.01, 4, bits=21)
 
 
+# ---------------------------------------------------------------------------
+# TRBVH
+# ---------------------------------------------------------------------------
+TRBVH_KEYS = ("perm", "children", "range", "parent", "root", "boxes", "sah", "subtree")
+
+
+@pytest.mark.parametrize("case", ["ref100", "ref5000", "rand2d_200k", "rand3d_50k"])
+@pytest.mark.parametrize("rounds", [0, 1, 3])
+def test_trbvh_gpu_equals_oracle(case, rounds):
+    """Topology, boxes, costs and subtree counts bit-equal to the oracle."""
+    from stream_amd import geometry
+
+    _say(f"trbvh {case} rounds={rounds}")
+    if case.startswith("ref"):
+        c, r = O.trbvh_test_spheres(int(case[3:]))
+    else:
+        n = 200_000 if case == "rand2d_200k" else 50_000
+        d = 2 if case.startswith("rand2d") else 3
+        rng = np.random.default_rng(n + d)
+        c = rng.random((n, d)).astype(np.float32)
+        r = (np.float32(0.25) / np.sqrt(np.float32(n)) * (1 + rng.random(n))).astype(np.float32)
+    g = geometry.build_trbvh(c, r, rounds)
+    o = O.trbvh(c, r, rounds)
+    for k in TRBVH_KEYS:
+        assert np.array_equal(g[k], o[k]), k
+
+
+@pytest.mark.parametrize("n", [100, 1000, 100_000])
+def test_trbvh_gpu_collisions(n):
+    from stream_amd import geometry
+
+    _say(f"trbvh collisions {n}")
+    c, r = O.trbvh_test_spheres(n)
+    assert geometry.collisions_trbvh(c, r, 3) == O.trbvh_collisions(c, r, 3)
+    c3 = np.random.default_rng(n).random((n, 3)).astype(np.float32)
+    r3 = np.full(n, 0.6 / n ** (1 / 3), np.float32)
+    assert geometry.collisions_trbvh(c3, r3, 3) == O.trbvh_collisions(c3, r3, 3)
+
+
+def test_trbvh_gpu_1m_timing_and_structure():
+    """Reference test cloud at 1M spheres: the tree is complete and
+    consistent; prints the GPU build / restructuring times."""
+    from stream_amd import geometry
+
+    n = 1_000_000
+    _say("trbvh 1M")
+    rng = np.random.default_rng(11)
+    c = rng.random((n, 2)).astype(np.float32)
+    r = (np.float32(0.25) / np.sqrt(np.float32(n)) * (1 + rng.random(n))).astype(np.float32)
+    g = geometry.build_trbvh(c, r, 3)
+    geometry.build_trbvh(c, r, 3)
+    g = geometry.build_trbvh(c, r, 3)
+    print(f"trbvh 1M: build {g['ms'][0]:.3f} ms, 3 rounds {g['ms'][1]:.3f} ms", flush=True)
+    root = int(g["root"][0])
+    assert g["parent"][root] == 2 * n and g["subtree"][root] == n
+    ch = g["children"]
+    inner = np.arange(n, 2 * n - 1)
+    assert np.array_equal(g["parent"][ch[:, 0]], inner) and np.array_equal(g["parent"][ch[:, 1]], inner)
+    assert np.array_equal(g["subtree"][inner], g["subtree"][ch[:, 0]] + g["subtree"][ch[:, 1]])
+    assert np.array_equal(g["boxes"][inner, :2], np.minimum(g["boxes"][ch[:, 0], :2], g["boxes"][ch[:, 1], :2]))
+    assert np.array_equal(g["boxes"][inner, 2:], np.maximum(g["boxes"][ch[:, 0], 2:], g["boxes"][ch[:, 1], 2:]))
+    seen = np.bincount(ch.ravel(), minlength=2 * n - 1)
+    seen[root] += 1
+    assert np.all(seen == 1)
+    o = O.trbvh(c, r, 3)
+    for k in TRBVH_KEYS:
+        assert np.array_equal(g[k], o[k]), k
+
+
 def _heap_ref(items):
     """The reference's sift order (traversal_stack.hpp:50-90) in Python."""
     h = []
