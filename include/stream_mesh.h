@@ -97,6 +97,49 @@ uint32_t AlphaShape3D_get_nexact(const AlphaShape3D* ashape);
 uint32_t AlphaShape3D_get_nclamped(const AlphaShape3D* ashape);
 int32_t AlphaShape3D_last_status(const AlphaShape3D* ashape);
 
+/* Full 2D Delaunay (include/mesh/mesh.hpp:23-62, src/mesh/mesh.cpp:800-862),
+ * float nodes.  Triangles and node ids are in Morton order, as in the
+ * reference; remove_super_nodes must precede get_nelem for a non-empty
+ * result. */
+typedef struct Mesh2D Mesh2D;
+
+/* include/mesh/mesh.hpp:39 */
+Mesh2D* Mesh2D_create(void);
+/* include/mesh/mesh.hpp:42 */
+void Mesh2D_destroy(Mesh2D* mesh);
+/* include/mesh/mesh.hpp:45  nodes row-major [2N] (N >= 2) */
+void Mesh2D_set_nodes(Mesh2D* mesh, uint32_t nnodes, const float* coords);
+/* include/mesh/mesh.hpp:48  agglomerative BVH, super-triangle, 3-triangle stars */
+void Mesh2D_init(Mesh2D* mesh);
+/* include/mesh/mesh.hpp:51  Morton neighbours (+-5) */
+void Mesh2D_insert_morton_neighbors(Mesh2D* mesh);
+/* include/mesh/mesh.hpp:55  neighbours whose BVH kd cell touches the node's */
+void Mesh2D_insert_BVH_neighbors(Mesh2D* mesh);
+/* include/mesh/mesh.hpp:58  up to 32 rounds of circumcircle search + insertion */
+void Mesh2D_insert_iterative(Mesh2D* mesh);
+/* include/mesh/mesh.hpp:60 */
+void Mesh2D_remove_super_nodes(Mesh2D* mesh);
+/* include/mesh/mesh.hpp:62  side effect: compresses the stars (mesh.cpp:849-852) */
+uint32_t Mesh2D_get_nelem(Mesh2D* mesh);
+/* include/mesh/mesh.hpp:63  [nelem][3] */
+void Mesh2D_get_elem(Mesh2D* mesh, uint32_t* elems);
+/* src/mesh/mesh.cpp:860 (defined and bound there, not declared in mesh.hpp)  [N][2] */
+void Mesh2D_get_ordered_nodes(Mesh2D* mesh, float* nodes);
+/* status of the last void call on the handle */
+int32_t Mesh2D_last_status(const Mesh2D* mesh);
+/* counters [3]: nodes frozen on a capacity overflow (64 triangles / local
+ * indices / heap entries per node), most rounds of insert_iterative, exact
+ * incircle evaluations; ms [5]: GPU time of init, Morton, BVH, iterative,
+ * remove phases.  Either may be NULL. */
+int32_t Mesh2D_get_stats(Mesh2D* mesh, uint32_t* counters, float* ms);
+/* stars: elem [N][64] (a | b << 8), nelem [N], neig [N][64] (local index
+ * j >= 3 at j - 3), nlocal [N] (bit 31 = frozen), Morton permutation [N] */
+int32_t Mesh2D_get_stars(Mesh2D* mesh, uint16_t* elem, uint32_t* nelem, uint32_t* neig, uint32_t* nlocal,
+                         uint32_t* perm);
+/* kd cells of the 2N-1 BVH nodes, rows (lo0, lo1, hi0, hi1), after
+ * insert_BVH_neighbors */
+int32_t Mesh2D_get_cells(Mesh2D* mesh, float* cells);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,6 +5,7 @@
 
 #include "oracle.hpp"
 #include "oracle3d.hpp"
+#include "mesh2d.hpp"
 
 using namespace orc;
 
@@ -252,6 +253,62 @@ void orc_treelet_dp2d(const float* boxes, const float* cost, const uint32_t* lsi
             b[l].hi[a] = boxes[4 * l + 2 + a];
         }
     treelet_dp<2>(b, cost, lsize, copt, popt, size);
+}
+
+// ---- Mesh2D (full 2D Delaunay, float) ----
+void* orc_mesh2d_create(uint32_t n, const float* xy) {
+    auto* m = new Mesh2D();
+    m->init(xy, n);
+    return m;
+}
+void orc_mesh2d_free(void* h) { delete static_cast<Mesh2D*>(h); }
+void orc_mesh2d_insert_morton(void* h) { static_cast<Mesh2D*>(h)->insert_morton_neighbors(); }
+void orc_mesh2d_insert_bvh(void* h) { static_cast<Mesh2D*>(h)->insert_bvh_neighbors(); }
+uint32_t orc_mesh2d_insert_iterative(void* h) {
+    auto* m = static_cast<Mesh2D*>(h);
+    m->insert_iterative();
+    return m->last_rounds;
+}
+void orc_mesh2d_remove_super(void* h) { static_cast<Mesh2D*>(h)->remove_super_nodes(); }
+uint32_t orc_mesh2d_nelem(void* h) {
+    auto* m = static_cast<Mesh2D*>(h);
+    m->compress();
+    return m->n_elems;
+}
+void orc_mesh2d_elem(void* h, uint32_t* out) {
+    auto* m = static_cast<Mesh2D*>(h);
+    std::memcpy(out, m->elem.data(), 12ull * m->n_elems);
+}
+uint64_t orc_mesh2d_overflow(void* h) { return static_cast<Mesh2D*>(h)->overflow; }
+// Stars: elem [n][64] as (a | b << 8), nelem [n], neig [n][64] (local index
+// j >= 3 at j - 3), nlocal [n], ordered nodes [n][2] (any may be null).
+void orc_mesh2d_stars(void* h, uint16_t* elem, uint32_t* nelem, uint32_t* neig, uint32_t* nlocal, float* xy) {
+    auto* m = static_cast<Mesh2D*>(h);
+    for (uint32_t i = 0; i < m->n; ++i) {
+        const auto& st = m->star[i];
+        if (elem)
+            for (uint32_t k = 0; k < Mesh2D::kMaxElem; ++k)
+                elem[64ull * i + k] = k < st.nelem ? (uint16_t)(st.e[k].a | (st.e[k].b << 8)) : 0;
+        if (nelem) nelem[i] = st.nelem;
+        if (neig)
+            for (uint32_t k = 0; k < Mesh2D::kMaxLocal; ++k) neig[64ull * i + k] = k + 3 < st.nlocal ? st.neig[k] : 0;
+        if (nlocal) nlocal[i] = st.nlocal;
+        if (xy) {
+            xy[2 * i] = m->xy[2 * i];
+            xy[2 * i + 1] = m->xy[2 * i + 1];
+        }
+    }
+}
+// kd cells of the BVH nodes [2n-1] rows (lo0, lo1, hi0, hi1), computed on demand.
+void orc_mesh2d_cells(void* h, float* out) {
+    auto* m = static_cast<Mesh2D*>(h);
+    m->compute_cells();
+    for (size_t i = 0; i < m->cell.size(); ++i) {
+        out[4 * i] = m->cell[i].lo[0];
+        out[4 * i + 1] = m->cell[i].lo[1];
+        out[4 * i + 2] = m->cell[i].hi[0];
+        out[4 * i + 3] = m->cell[i].hi[1];
+    }
 }
 
 // ---- 3D predicates ----

@@ -16,7 +16,7 @@
 // surface-area costs and subtree sizes, then treelet restructuring rounds.
 #include <vector>
 
-#include "lbvh.hpp"
+#include "lbvha.hpp"
 #include "traversal.hpp"
 
 namespace stream_amd {
@@ -183,33 +183,6 @@ __global__ __launch_bounds__(256) void agglo_collide_kernel(const Sph4<S>* __res
     c = wave_sum(c);
     if (lane_id() == 0 && c) atomicAdd(total, (unsigned long long)c);
 }
-
-template <class S, int D> struct Agglo {
-    LbvhGpu<S, D> base;
-    DBuf<BoxD<S, D>> data;  // [2n-1]
-    DBuf<uint64_t> delta;   // [n-1]
-    DBuf<uint32_t> children, range, parent_of, split, touched, root;
-
-    // sah / subtree (both or neither, [2n-1]) receive the TRBVH costs.
-    void build(const Sph4<S>* d_obj, uint32_t n, int bits, uint32_t leaf_mode, hipStream_t s, float* sah = nullptr,
-               uint32_t* subtree = nullptr) {
-        if (n < 2) throw std::invalid_argument("LBVHa needs at least 2 objects");
-        base.sort_only(d_obj, n, bits, s);
-        data.resize(2ull * n);
-        delta.resize(n);
-        children.resize(2ull * n);
-        range.resize(2ull * n);
-        parent_of.resize(2ull * n);
-        split.resize(n);
-        touched.resize(n);
-        root.resize(1);
-        hipLaunchKernelGGL((agglo_init_kernel<S, D>), dim3(grid_for(n, 256)), dim3(256), 0, s, base.obj_m.p,
-                           base.codes_sorted, n, leaf_mode, data.p, delta.p, touched.p, sah, subtree);
-        hipLaunchKernelGGL((agglo_build_kernel<S, D>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, delta.p,
-                           children.p, range.p, parent_of.p, split.p, touched.p, data.p, root.p, sah, subtree,
-                           ctx().d_flags);
-    }
-};
 
 // ---------------------------------------------------------------------------
 // TRBVH treelet restructuring (trbvh.hpp:390-633).  One thread per leaf
@@ -564,6 +537,29 @@ __global__ void heap_order_kernel(uint32_t n, const uint32_t* idx, const float* 
 }
 
 }  // namespace
+
+template <class S, int D>
+void Agglo<S, D>::build(const Sph4<S>* d_obj, uint32_t n, int bits, uint32_t leaf_mode, hipStream_t s, float* sah,
+                        uint32_t* subtree) {
+    if (n < 2) throw std::invalid_argument("LBVHa needs at least 2 objects");
+    base.sort_only(d_obj, n, bits, s);
+    data.resize(2ull * n);
+    delta.resize(n);
+    children.resize(2ull * n);
+    range.resize(2ull * n);
+    parent_of.resize(2ull * n);
+    split.resize(n);
+    touched.resize(n);
+    root.resize(1);
+    hipLaunchKernelGGL((agglo_init_kernel<S, D>), dim3(grid_for(n, 256)), dim3(256), 0, s, base.obj_m.p,
+                       base.codes_sorted, n, leaf_mode, data.p, delta.p, touched.p, sah, subtree);
+    hipLaunchKernelGGL((agglo_build_kernel<S, D>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, delta.p, children.p,
+                       range.p, parent_of.p, split.p, touched.p, data.p, root.p, sah, subtree, ctx().d_flags);
+}
+template struct Agglo<float, 2>;
+template struct Agglo<double, 2>;
+template struct Agglo<float, 3>;
+template struct Agglo<double, 3>;
 
 }  // namespace stream_amd
 

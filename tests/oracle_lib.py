@@ -108,6 +108,23 @@ def _declare(L):
         getattr(L, f"orc_trbvh{d}d_build_f32").argtypes = [u32, vp, i32] + [vp] * 8
         getattr(L, f"orc_trbvh{d}d_collisions_f32").restype = u64
         getattr(L, f"orc_trbvh{d}d_collisions_f32").argtypes = [u32, vp, i32]
+    L.orc_mesh2d_create.restype = vp
+    L.orc_mesh2d_create.argtypes = [u32, vp]
+    for name in ("orc_mesh2d_free", "orc_mesh2d_insert_morton", "orc_mesh2d_insert_bvh", "orc_mesh2d_remove_super"):
+        getattr(L, name).restype = None
+        getattr(L, name).argtypes = [vp]
+    L.orc_mesh2d_insert_iterative.restype = u32
+    L.orc_mesh2d_insert_iterative.argtypes = [vp]
+    L.orc_mesh2d_nelem.restype = u32
+    L.orc_mesh2d_nelem.argtypes = [vp]
+    L.orc_mesh2d_elem.restype = None
+    L.orc_mesh2d_elem.argtypes = [vp, vp]
+    L.orc_mesh2d_overflow.restype = u64
+    L.orc_mesh2d_overflow.argtypes = [vp]
+    L.orc_mesh2d_stars.restype = None
+    L.orc_mesh2d_stars.argtypes = [vp] * 6
+    L.orc_mesh2d_cells.restype = None
+    L.orc_mesh2d_cells.argtypes = [vp, vp]
     L.orc_treelet_dp2d.restype = None
     L.orc_treelet_dp2d.argtypes = [vp] * 6
     L.orc_assemble_p1_tet.restype = ctypes.c_int64
@@ -319,3 +336,61 @@ def trbvh_test_spheres(n):
     min_r = np.float32(1.0) / np.sqrt(np.float32(n))
     rad = (np.float32(0.25) * min_r) * (np.float32(1.0) + u[:, 2])
     return np.ascontiguousarray(u[:, :2]), rad.astype(np.float32)
+
+
+class Mesh2D:
+    """Oracle Mesh2D: same phases as the reference's Mesh2D."""
+
+    def __init__(self, xy):
+        self.L = lib()
+        self.xy = np.ascontiguousarray(xy, np.float32)
+        self.n = self.xy.shape[0]
+        self.h = self.L.orc_mesh2d_create(self.n, _p(self.xy))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_mesh2d_free(self.h)
+            self.h = None
+
+    def insert_morton_neighbors(self):
+        self.L.orc_mesh2d_insert_morton(self.h)
+
+    def insert_BVH_neighbors(self):
+        self.L.orc_mesh2d_insert_bvh(self.h)
+
+    def insert_iterative(self):
+        return int(self.L.orc_mesh2d_insert_iterative(self.h))
+
+    def remove_super_nodes(self):
+        self.L.orc_mesh2d_remove_super(self.h)
+
+    def get_elem(self):
+        k = self.L.orc_mesh2d_nelem(self.h)
+        e = np.empty((k, 3), np.uint32)
+        self.L.orc_mesh2d_elem(self.h, _p(e))
+        return e
+
+    def overflow(self):
+        return int(self.L.orc_mesh2d_overflow(self.h))
+
+    def stars(self):
+        n = self.n
+        r = dict(elem=np.zeros((n, 64), np.uint16), nelem=np.zeros(n, np.uint32),
+                 neig=np.zeros((n, 64), np.uint32), nlocal=np.zeros(n, np.uint32), xy=np.zeros((n, 2), np.float32))
+        self.L.orc_mesh2d_stars(self.h, _p(r["elem"]), _p(r["nelem"]), _p(r["neig"]), _p(r["nlocal"]), _p(r["xy"]))
+        return r
+
+    def cells(self):
+        c = np.zeros((2 * self.n - 1, 4), np.float32)
+        self.L.orc_mesh2d_cells(self.h, _p(c))
+        return c
+
+
+def super_nodes(xym):
+    """The three super-triangle vertices of Mesh::init (mesh.cpp:109-126)."""
+    f = np.float32
+    lo, hi = xym.min(axis=0), xym.max(axis=0)
+    a0 = f(0.5) * (hi[0] + lo[0])
+    a1 = f(0.5) * (hi[1] + lo[1])
+    mr = f(max(hi[0] - lo[0], hi[1] - lo[1])) + f(10)
+    return np.array([[a0 - mr, a1 - mr], [a0 + mr, a1 - mr], [a0, a1 + mr]], np.float32)
