@@ -97,6 +97,94 @@ def sell_dir_bytes(n, nnz, value_bytes):
     return n * (2 + vb + 2 * vb + vb + vb) + (nnz - n) * (4 + vb) + nslices * 6
 
 
+def tank3d_leg(steps=2):
+    """The 3D step on the sloshing-tank cloud (1,996,920 FP64 points,
+    alpha = 1.3h): LBVH + alpha-shape tetrahedra + P1 + CG to rtol 1e-8,
+    mixed and FP64 Jacobi-PCG; steps/s and stage times of the last step."""
+    import numpy as np
+
+    import stream_amd
+    from stream_amd import clouds
+    from stream_amd.pfem import PfemStep3D
+
+    pts, al = clouds.sloshing_tank()
+    f = np.sin(3.0 * pts[:, 0]) * np.cos(2.0 * pts[:, 1]) + pts[:, 2]
+    st = PfemStep3D(pts, al, f)
+    out = {"points": int(len(pts))}
+    for solver in ("mixed", "fp64"):
+        st.run(solver, rtol=1e-8)
+        stream_amd.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = st.run(solver, rtol=1e-8)
+        stream_amd.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+        out[solver] = {"steps_per_s": round(1000.0 / ms, 4), "ms_per_step": round(ms, 2),
+                       "mesh_ms": round(r["mesh_ms"], 2), "assembly_ms": round(r["assemble_ms"], 2),
+                       "solve_ms": round(r["solve_ms"], 2), "iters": r["iters"], "true_rel": r["true_rel"],
+                       "tets": r["n_elems"], "nnz": r["nnz"]}
+        print(f"[bench] tank3d {solver}: {ms:.1f} ms/step {out[solver]}", file=sys.stderr, flush=True)
+    del st
+    return out
+
+
+def components(step, pts, al, net):
+    """GPU times of the path's components on the 1M dam-break cloud (the
+    step's own stage timers, the TRBVH and Mesh2D phase timers, and the CG
+    kernels timed alone), for the per-component record."""
+    import numpy as np
+
+    import stream_amd
+    from stream_amd import geometry
+    from stream_amd.mesh import Mesh2D
+
+    out = {"workload": "2D dam-break cloud, 1,000,000 FP64 points (FP32 copies for TRBVH and Mesh2D)"}
+    for solver in ("fp64", "mixed", "fsai") + (("learned",) if net is not None else ()):
+        step.run(solver, rtol=1e-8)
+        r = step.run(solver, rtol=1e-8)
+        out[f"step_{solver}"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+    out["dir_kernel_fp32_ms"] = round(step.time_dir_kernel(fp32=True, reps=200), 5)
+    out["dir_kernel_fp64_ms"] = round(step.time_dir_kernel(fp32=False, reps=200), 5)
+    xy = np.ascontiguousarray(pts, np.float32)
+    rad = np.full(len(xy), np.float32(0.5) * np.float32(al[0]), np.float32)
+    geometry.build_trbvh(xy, rad, 3)
+    t = geometry.build_trbvh(xy, rad, 3)
+    out["trbvh_ms"] = {"build": round(float(t["ms"][0]), 4), "3_rounds": round(float(t["ms"][1]), 4)}
+    m = Mesh2D()
+    m.set_nodes(xy)
+    m.insert_BVH_neighbors()
+    m.insert_iterative()
+    m.remove_super_nodes()
+    ne = len(m.get_elem())
+    st = m.get_stats()
+    out["mesh2d"] = {"triangles": ne, **st}
+    stream_amd.synchronize()
+    return out
+
+
+COPY_BYTES = 256 << 20
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cg_dir_kernel_dam1M_traffic.txt")
+
+
+def measured_traffic(n, nnz, fp32):
+    """HBM bytes per direction-kernel launch from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
+    reading of a copy kernel of known size), when it was taken on this
+    workload; else None."""
+    try:
+        kv = {}
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if "=" in line and not line.startswith("#"):
+                    k, v = line.split("=", 1)
+                    kv[k.strip()] = v.strip()
+        if int(kv["n"]) != n or int(kv["nnz"]) != nnz or kv["precision"] != ("fp32" if fp32 else "fp64"):
+            return None
+        return int(float(kv["traffic_bytes_per_launch"]))
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +194,15 @@ def main():
     ap.add_argument("--weights", default=WEIGHTS, help="weight file prefix (.bin + .json) of the learned solver")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 companion measurement")
+    ap.add_argument("--tank3d", action="store_true",
+                    help="also time the 3D step: 1,996,920-point sloshing tank, alpha-shape tetrahedra, "
+                         "P1 assembly, mixed FP32/FP64 Jacobi-PCG and FP64 Jacobi-PCG to rtol 1e-8")
+    ap.add_argument("--components", action="store_true",
+                    help="time the path's components at 1M points (LBVH step, TRBVH, Mesh2D phases, P1, CG "
+                         "kernels) and print them as one JSON line instead of the headline")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="counter runs: one step, then 20 direction-kernel launches and 5 copies of 256 MiB "
+                         "(the byte-counter calibration); prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -146,6 +243,21 @@ def main():
         stream_amd.synchronize()
         return (time.perf_counter() - t0) * 1000.0 / k, infos
 
+    if args.components:
+        print(json.dumps(components(step, pts, al, net)), flush=True)
+        return
+
+    if args.profile_kernels:
+        import ctypes
+
+        r = step.run(args.solver, rtol=1e-8)
+        kms = step.time_dir_kernel(fp32=args.solver != "fp64", reps=20)
+        cms = ctypes.c_double(0)
+        stream_amd.check(stream_amd.lib.Stream_copy_calibration(ctypes.c_uint64(COPY_BYTES), 5, ctypes.byref(cms)))
+        print(f"[profile] n={pts.shape[0]} nnz={r['nnz']} dir kernel {kms * 1000:.2f} us, "
+              f"copy {COPY_BYTES} B {cms.value * 1000:.1f} us", flush=True)
+        return
+
     ms, infos = timed(args.solver, args.steps, args.warmup)
     last = infos[-1]
     if any(i["fallback"] for i in infos):
@@ -167,8 +279,9 @@ def main():
     nnz = last["nnz"]
     alg_bytes = sell_dir_bytes(n, nnz, 4 if fp32 else 8)
     achieved = alg_bytes / (kms * 1e-3) / 1e9
+    traffic = measured_traffic(n, nnz, fp32)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": f"cg_dir_kernel<{'float, true' if fp32 else 'double, false'}>", "kernel_ms": round(kms, 5),
                 "alg_bytes_per_launch": int(alg_bytes)}
 
@@ -181,6 +294,10 @@ def main():
         mfma = {"bound": "mfma", "kernel": "gemm_rows_kernel<64, 64, 128, 0> (layer-0 [P|Q] = H W_pq, bf16)",
                 "achieved": round(tf, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(tf / BF16_PEAK_TFLOPS, 5), "kernel_ms": round(gms, 5), "flops_per_launch": gflops}
+
+    tank = None
+    if args.tank3d:
+        tank = tank3d_leg()
 
     cpu = None
     if not args.no_cpu:
@@ -218,6 +335,7 @@ def main():
         "speedup_vs_fp64_jacobi": None if fp64 is None else round(fp64["ms_per_step"] / ms, 3),
         "roofline": roofline,
         "mfma": mfma,
+        "tank3d": tank,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -190,6 +190,7 @@ struct Mesh2D {
     void compute_cells() {
         cell.assign(2ull * n - 1, Box<float, 2>::empty());
         if (n < 2) return;
+        cell[bvh.root] = bvh.global;  // never read; the reference leaves it unset
 #pragma omp parallel for schedule(static)
         for (uint32_t tid = 0; tid < n; ++tid) {
             Box<float, 2> bb = bvh.global;

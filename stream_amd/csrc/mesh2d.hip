@@ -248,6 +248,7 @@ __global__ __launch_bounds__(256) void m2_cells_kernel(uint32_t n, const uint32_
     if (tid >= n) return;
     BoxD<float, 2> bb = *gbox;
     uint32_t cur = *root;
+    if (tid == 0) cell[cur] = bb;  // never read; the reference leaves it unset
     for (uint32_t level = 0; level < 2 * n; ++level) {
         const uint32_t k = cur - n;
         const uint32_t sp = split[k];

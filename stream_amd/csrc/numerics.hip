@@ -211,7 +211,8 @@ template void build_sell<double>(uint32_t, const uint32_t*, const uint32_t*, con
 
 namespace {
 
-// Matrix arguments of the SELL kernels.
+// Matrix arguments of the SELL kernels.  map selects how slices are dealt
+// to wavefronts (slice_span).
 template <class V> struct SellArgs {
     uint32_t n, nslices;
     const uint32_t* off;
@@ -220,16 +221,42 @@ template <class V> struct SellArgs {
     const uint32_t* col;
     const V* val;
     const V* diag;
+    uint32_t map;
 };
 template <class V> SellArgs<V> sell_args(const Sell<V>& S) {
-    return {S.n, S.nslices, S.off.p, S.width.p, S.len.p, S.col.p, S.val.p, S.diag.p};
+    return {S.n, S.nslices, S.off.p, S.width.p, S.len.p, S.col.p, S.val.p, S.diag.p, tuning().xcd_map};
 }
 
-// y_i = diag_i * x_i + sum_k A_ik x_k over one SELL slice, the vector given
-// through a loader; four entries per step, loads issued before use.
-#define SELL_FOR_SLICES(s)                                                             \
-    for (uint32_t s = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; s < M.nslices; \
-         s += (gridDim.x * blockDim.x) >> 6)
+// Slices of one wavefront: [s, e) with stride step.
+//  map 0: grid stride (wave w takes w, w + W, ...);
+//  map 1: contiguous chunks per wave with the blocks renumbered so that the
+//         blocks one XCD receives (hardware deals workgroups round-robin to
+//         the 8 XCDs: block b runs on XCD b % 8) own one contiguous eighth
+//         of the rows.  Rows are in Morton order, so the neighbour entries a
+//         wave gathers were mostly fetched into the same XCD's L2 by the
+//         neighbouring waves.
+constexpr uint32_t kXcds = 8;
+struct SliceSpan {
+    uint32_t s, e, step;
+};
+__device__ __forceinline__ SliceSpan slice_span(uint32_t nslices, uint32_t map) {
+    const uint32_t wpb = blockDim.x >> 6;
+    const uint32_t G = gridDim.x;
+    const uint32_t nw = G * wpb;
+    if (map == 0) return {blockIdx.x * wpb + (threadIdx.x >> 6), nslices, nw};
+    const uint32_t b = blockIdx.x;
+    const uint32_t lb = (G % kXcds == 0) ? (b % kXcds) * (G / kXcds) + b / kXcds : b;
+    const uint32_t gw = lb * wpb + (threadIdx.x >> 6);
+    const uint32_t chunk = (nslices + nw - 1) / nw;
+    const uint32_t s0 = min(gw * chunk, nslices);
+    return {s0, min(s0 + chunk, nslices), 1u};
+}
+
+// Loop over the slices of this wavefront (the kernel's matrix is M).
+#define SELL_FOR_SLICES(s)                                                                        \
+    for (SliceSpan _span = slice_span(M.nslices, M.map), _once = {0, 1, 1}; _once.s < _once.e; \
+         ++_once.s)                                                                               \
+        for (uint32_t s = _span.s; s < _span.e; s += _span.step)
 
 // ---------------------------------------------------------------------------
 // init / replacement: r = b - A64 x64 (FP64), stored in precision V;

@@ -131,6 +131,7 @@ struct Tuning {
     uint32_t batch = 32;           // CG iterations between host checks
     uint32_t cg_blocks = 1024;     // grid cap of the CG kernels (64..4096)
     uint32_t graph = 0;            // 1: replay each batch of iterations as a captured HIP graph
+    uint32_t xcd_map = 1;          // SELL slices: 0 grid stride, 1 XCD-contiguous chunks per wave
 };
 Tuning& tuning();
 

@@ -744,6 +744,9 @@ int32_t Stream_tune(const char* key, double value) {
         } else if (k == "graph") {
             if (!(value == 0.0 || value == 1.0)) throw std::invalid_argument("graph must be 0 or 1");
             tuning().graph = (uint32_t)value;
+        } else if (k == "xcd_map") {
+            if (!(value == 0.0 || value == 1.0)) throw std::invalid_argument("xcd_map must be 0 or 1");
+            tuning().xcd_map = (uint32_t)value;
         } else {
             throw std::invalid_argument("unknown tuning key: " + k);
         }

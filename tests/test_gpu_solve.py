@@ -177,3 +177,35 @@ def test_dam_break_1m_fp64_and_mixed():
               f"asm={info['assemble_ms']:.2f}ms solve={info['solve_ms']:.1f}ms")
         assert rel <= 1e-8
     assert (A != A.T).nnz == 0
+
+
+@pytest.mark.parametrize("solver", ["fp64", "mixed", "fsai"])
+def test_graph_replay_and_xcd_map(solver):
+    """CG batches replayed as captured HIP graphs give the eager run's bits
+    (every batch is a replay, with host checks between replays); both slice
+    mappings converge to the tolerance."""
+    import stream_amd
+    from stream_amd.pfem import PfemStep2D
+
+    _say(f"graph replay / xcd map {solver}")
+    pts, al = clouds.dam_break_scaled(60_000)
+    f = clouds.divergence_source(pts, clouds.fourier_modes(11))
+    st = PfemStep2D(pts, al, f)
+    try:
+        for xm in (0, 1):
+            stream_amd.tune("xcd_map", xm)
+            stream_amd.tune("graph", 0)
+            r0 = st.run(solver, rtol=1e-8)
+            x0 = st.solution(morton=True)
+            stream_amd.tune("graph", 1)
+            for _ in range(2):
+                r1 = st.run(solver, rtol=1e-8)
+                x1 = st.solution(morton=True)
+                assert r1["iters"] == r0["iters"]
+                assert np.array_equal(x1, x0)
+            assert r0["true_rel"] <= 1e-8
+            print(f"{solver} xcd_map={xm}: iters={r0['iters']} solve={r0['solve_ms']:.1f}ms "
+                  f"graph solve={r1['solve_ms']:.1f}ms")
+    finally:
+        stream_amd.tune("graph", 0)
+        stream_amd.tune("xcd_map", 1)
