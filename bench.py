@@ -32,6 +32,7 @@ METRIC = "mesh+solve steps/s @1M 2D pts, rtol 1e-8: learned mixed PCG vs FP64 Ja
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 WEIGHTS = os.path.join(ROOT, "stream_amd", "weights", "mpnn_dam2d")
+WEIGHTS3D = os.path.join(ROOT, "stream_amd", "weights", "mpnn_tank3d")
 SOLVER_LABEL = {
     "learned": "learned mixed PCG (MPNN L L^T, FP32 SpMV/apply, FP64 dots + residual replacement)",
     "mixed": "mixed FP32/FP64 Jacobi-PCG",
@@ -162,6 +163,64 @@ def components(step, pts, al, net):
     return out
 
 
+def sequence3d(steps, weights):
+    """The 3D advect-remesh-solve sequence on the 8,012,034-point tank
+    (342 x 171 x 137 jittered grid, alpha = 1.3h): every step meshes,
+    assembles and solves with the learned mixed PCG to a verified FP64
+    rtol of 1e-8, then advects the points (dt = 1e-3, seed-44 field).
+    Reports steps/s over the timed steps, per-stage means and the peak
+    device memory (the library's allocator high-water mark and the
+    device's used memory)."""
+    import numpy as np
+
+    import stream_amd
+    from stream_amd import clouds
+    from stream_amd.learned import PrecLearned
+    from stream_amd.pfem import PfemStep3D
+
+    if not os.path.exists(weights + ".bin"):
+        print(f"3D learned solver: weight file {weights}.bin not found "
+              "(python -m stream_amd.train_learned --dim 3 --out <prefix>)", file=sys.stderr)
+        sys.exit(1)
+    pts, al = clouds.sloshing_tank(*clouds.TANK8M)
+    f = np.sin(3.0 * pts[:, 0]) * np.cos(2.0 * pts[:, 1]) + pts[:, 2]
+    st = PfemStep3D(pts, al, f)
+    net = PrecLearned(PrecLearned.load_params(weights))
+    st.set_learned(net)
+    modes = clouds.potential_modes()
+    stream_amd.device_bytes(reset_peak=True)
+    used_peak = 0
+    rows = []
+    stream_amd.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        r = st.run("learned", rtol=1e-8)
+        mem = stream_amd.device_bytes()
+        used_peak = max(used_peak, mem["total"] - mem["free"])
+        rows.append(r)
+        print(f"[seq3d] step {k}: mesh {r['mesh_ms']:.1f} asm {r['assemble_ms']:.1f} prep {r['prep_ms']:.1f} "
+              f"solve {r['solve_ms']:.1f} ms, {r['iters']} it, rel {r['true_rel']:.2e}, tets {r['n_elems']}",
+              file=sys.stderr, flush=True)
+        st.advect(modes, 1e-3, 1)
+    stream_amd.synchronize()
+    wall = time.perf_counter() - t0
+    mem = stream_amd.device_bytes()
+    mean = lambda key: round(float(np.mean([r[key] for r in rows])), 3)
+    return {
+        "metric": "3D advect-remesh-solve steps/s, 8,012,034-point tank, learned mixed PCG rtol 1e-8",
+        "value": round(steps / wall, 4), "unit": "steps/s", "steps": steps, "ms_per_step": round(1000 * wall / steps, 2),
+        "points": int(len(pts)),
+        "stages_ms_mean": {"mesh": mean("mesh_ms"), "assembly": mean("assemble_ms"), "precond_setup": mean("prep_ms"),
+                           "solve": mean("solve_ms")},
+        "iters_mean": mean("iters"), "iters_max": int(max(r["iters"] for r in rows)),
+        "true_rel_max": float(max(r["true_rel"] for r in rows)), "fallbacks": int(sum(r["fallback"] for r in rows)),
+        "tets_step0": int(rows[0]["n_elems"]), "nnz_step0": int(rows[0]["nnz"]),
+        "peak_library_bytes": int(mem["peak"]), "peak_device_used_bytes": int(used_peak),
+        "device_total_bytes": int(mem["total"]),
+        "weights": os.path.relpath(weights, ROOT),
+    }
+
+
 COPY_BYTES = 256 << 20
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cg_dir_kernel_dam1M_traffic.txt")
 
@@ -197,6 +256,12 @@ def main():
     ap.add_argument("--tank3d", action="store_true",
                     help="also time the 3D step: 1,996,920-point sloshing tank, alpha-shape tetrahedra, "
                          "P1 assembly, mixed FP32/FP64 Jacobi-PCG and FP64 Jacobi-PCG to rtol 1e-8")
+    ap.add_argument("--sequence3d", action="store_true",
+                    help="the 3D advect-remesh-solve sequence instead of the headline: 8,012,034-point tank, "
+                         "--seq-steps steps of LBVH + alpha-shape tetrahedra + P1 + learned mixed PCG (rtol 1e-8), "
+                         "advected in the seeded 3D divergence-free field; prints one JSON line")
+    ap.add_argument("--seq-steps", type=int, default=50)
+    ap.add_argument("--weights3d", default=WEIGHTS3D, help="weight prefix of the 3D learned solver")
     ap.add_argument("--components", action="store_true",
                     help="time the path's components at 1M points (LBVH step, TRBVH, Mesh2D phases, P1, CG "
                          "kernels) and print them as one JSON line instead of the headline")
@@ -218,6 +283,9 @@ def main():
     if stream_amd.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.sequence3d:
+        print(json.dumps(sequence3d(args.seq_steps, args.weights3d)), flush=True)
+        return
     pts, al = clouds.dam_break()
     f = clouds.divergence_source(pts, clouds.fourier_modes(11))
     step = PfemStep2D(pts, al, f)

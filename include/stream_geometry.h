@@ -21,6 +21,10 @@ extern "C" {
 const char* Stream_last_error(void);
 int32_t Stream_device_count(void);
 int32_t Stream_synchronize(void);
+/* no reference counterpart: device bytes held by the library now / at the
+ * high-water mark (reset_peak != 0 restarts it), device free / total memory */
+int32_t Stream_device_bytes(uint64_t* current, uint64_t* peak, int32_t reset_peak, uint64_t* dev_free,
+                            uint64_t* dev_total);
 
 /* xyr: rows (x, y, r).  Outputs may be NULL.  ibox rows (xmin, ymin, xmax,
  * ymax) per internal node; gbox the global sphere box.  leaf_mode 0 boxes

@@ -36,8 +36,15 @@ int32_t PrecLearned_set_adam(PrecLearned* net, float beta1, float beta2, float e
  * [n][2] in row order, Dirichlet flags [n].  Copied to the device. */
 int32_t PrecLearned_set_system(PrecLearned* net, uint32_t n, const uint32_t* row, const uint32_t* col,
                                const double* val, const double* xy, const uint8_t* bnd);
+/* the same with coordinates [n][dim], dim 2 or 3 */
+int32_t PrecLearned_set_system_dim(PrecLearned* net, uint32_t dim, uint32_t n, const uint32_t* row,
+                                   const uint32_t* col, const double* val, const double* xyz, const uint8_t* bnd);
 /* The step's current (assembled) system, device-resident. */
 int32_t PrecLearned_bind_step(PrecLearned* net, PfemStep2D* step);
+/* The same for the 3D step (node features use x, y, z). */
+int32_t PrecLearned_bind_step3d(PrecLearned* net, PfemStep3D* step);
+/* Network of the 3D step's solver 3 (not owned; NULL clears). */
+int32_t PfemStep3D_set_learned(PfemStep3D* step, PrecLearned* net);
 /* Forward + loss; ghat [nnz] (nullable) receives Ghat per CSR entry (upper
  * entries and diagonal). */
 int32_t PrecLearned_forward(PrecLearned* net, double* loss, float* ghat);

@@ -96,6 +96,11 @@ void PfemStep3D_destroy(PfemStep3D* step);
 int32_t PfemStep3D_set_points(PfemStep3D* step, uint32_t n, const double* xyz, const double* alpha, const double* f);
 int32_t PfemStep3D_set_morton_bits(PfemStep3D* step, int32_t bits);
 int32_t PfemStep3D_run(PfemStep3D* step, int32_t solver, double rtol, uint32_t maxit, double* stats);
+/* no reference counterpart: forward-Euler advection of the points in the
+ * divergence-free field u(x) = sum cos(k . x + phi) b of n_modes rows
+ * (kx, ky, kz, phi, bx, by, bz), and the current points (input order) */
+int32_t PfemStep3D_advect(PfemStep3D* step, uint32_t n_modes, const double* modes, double dt, uint32_t steps);
+int32_t PfemStep3D_get_points(PfemStep3D* step, double* xyz);
 int32_t PfemStep3D_assemble(PfemStep3D* step, double* stats);
 int32_t PfemStep3D_get_solution(PfemStep3D* step, double* p);
 uint32_t PfemStep3D_nnz(PfemStep3D* step);

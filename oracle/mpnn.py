@@ -21,7 +21,8 @@ entries A_ki with i > k); the preconditioner is M^-1 = L L^T with
 L = D^-1/2 Ghat^T, so L^T A L = Ghat Ahat Ghat^T.
 
 Network (hidden 64, three message-passing layers):
-  F_i   = [(x_i - xmin)/ext, (y_i - ymin)/ext, bnd_i, A_ii / 8]
+  F_i   = [(x_i - xmin)/ext, (y_i - ymin)/ext, (z_i - zmin)/ext, bnd_i, A_ii / 8]
+          (ext = largest box extent; z term 0 for 2D systems)
   e_ij  = [Ahat_ij, |x_i - x_j| / lbar, +1 if j > i else -1]  (lbar: mean
           length over the off-diagonal pattern)
   H0    = relu(F W_enc + b_enc)
@@ -49,7 +50,7 @@ import scipy.sparse as sp
 
 HIDDEN = 64
 LAYERS = 3
-NODE_FEATS = 4
+NODE_FEATS = 5
 EDGE_FEATS = 3
 DIAG_SHIFT = float(np.log(np.e - 1.0))  # softplus(DIAG_SHIFT) = 1
 
@@ -115,9 +116,12 @@ class Graph:
         hi = np.maximum(self.ei, self.ej)
         ahat = self.aval * (sd[lo] * sd[hi])
         xy = np.asarray(xy, np.float64)
-        dx = xy[lo, 0] - xy[hi, 0]
-        dy = xy[lo, 1] - xy[hi, 1]
-        length = np.sqrt(dx * dx + dy * dy)
+        dim = xy.shape[1]
+        d2 = np.zeros(lo.shape[0])
+        for a in range(dim):
+            d = xy[lo, a] - xy[hi, a]
+            d2 = d2 + d * d
+        length = np.sqrt(d2)
         lbar = length.mean() if length.size else 1.0
         sgn = np.where(self.ej > self.ei, 1.0, -1.0)
         self.efeat = np.stack([ahat.astype(np.float32), (length / lbar).astype(np.float32), sgn], 1).astype(np.float64)
@@ -126,9 +130,9 @@ class Graph:
         ext = ext if ext > 0 else 1.0
         bnd = np.asarray(bnd, np.float64)
         self.bnd = bnd
-        self.nfeat = np.stack([((xy[:, 0] - lo_xy[0]) / ext).astype(np.float32),
-                               ((xy[:, 1] - lo_xy[1]) / ext).astype(np.float32), bnd,
-                               (diag / 8.0).astype(np.float32)], 1).astype(np.float64)
+        pos = [((xy[:, a] - lo_xy[a]) / ext).astype(np.float32) if a < dim else np.zeros(n, np.float32)
+               for a in range(3)]
+        self.nfeat = np.stack(pos + [bnd, (diag / 8.0).astype(np.float32)], 1).astype(np.float64)
         self.up = self.ej > self.ei          # upper entries among off-diagonals
         self.mask = (self.aval != 0.0) & self.up
         # Ahat as a sparse matrix (full pattern, diagonal included)

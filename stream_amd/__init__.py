@@ -36,6 +36,9 @@ lib.Stream_synchronize.restype = ctypes.c_int32
 lib.Stream_synchronize.argtypes = []
 lib.Stream_tune.restype = ctypes.c_int32
 lib.Stream_tune.argtypes = [ctypes.c_char_p, ctypes.c_double]
+lib.Stream_device_bytes.restype = ctypes.c_int32
+lib.Stream_device_bytes.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                    ctypes.c_void_p]
 
 
 class StreamError(RuntimeError):
@@ -58,8 +61,20 @@ def synchronize() -> None:
 
 
 def tune(key: str, value: float) -> None:
-    """Set a solver knob: 'replace_factor', 'batch', 'cg_blocks' or 'graph'."""
+    """Set a solver knob: 'replace_factor', 'batch', 'cg_blocks', 'graph' or
+    'xcd_map'."""
     check(lib.Stream_tune(key.encode(), float(value)))
+
+
+def device_bytes(reset_peak: bool = False) -> dict:
+    """Device bytes held by the library's buffers (now and high-water mark)
+    and the device's free / total memory."""
+    import ctypes
+
+    v = [ctypes.c_uint64(0) for _ in range(4)]
+    check(lib.Stream_device_bytes(ctypes.byref(v[0]), ctypes.byref(v[1]), int(reset_peak), ctypes.byref(v[2]),
+                                  ctypes.byref(v[3])))
+    return dict(current=v[0].value, peak=v[1].value, free=v[2].value, total=v[3].value)
 
 
 from . import geometry, mesh, numerics, pfem  # noqa: E402,F401

@@ -169,3 +169,42 @@ def advect(xy: np.ndarray, modes: np.ndarray, dt: float, steps: int = 1) -> np.n
         p[:, 0] += dt * ux
         p[:, 1] += dt * uy
     return p
+
+
+TANK8M = (342, 171, 137)  # 8,012,034 points: the tank at 4x the point count
+ADVECT3_SEED = 44
+
+
+def potential_modes(seed: int = ADVECT3_SEED, n_modes: int = ADVECT_MODES, speed: float = 0.25):
+    """Divergence-free 3D field u(x) = sum_m cos(k_m . x + phi_m) b_m, the
+    curl of the vector potential sum_m a_m sin(k_m . x + phi_m) (b_m =
+    k_m x a_m, so div u = 0 exactly): wave numbers pi*(1 + floor(4u)) per
+    axis, phases 2*pi*u, directions a_m uniform on the sphere, scaled so
+    |b_m| = speed / n_modes (|u| <= speed).  Rows (kx, ky, kz, phi, bx, by, bz)."""
+    u = uniform(seed, 6 * n_modes).reshape(n_modes, 6)
+    k = np.pi * (1.0 + np.floor(4.0 * u[:, :3]))
+    phi = 2.0 * np.pi * u[:, 3]
+    ct = 2.0 * u[:, 4] - 1.0
+    st = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
+    az = 2.0 * np.pi * u[:, 5]
+    a = np.stack([st * np.cos(az), st * np.sin(az), ct], axis=1)
+    b = np.cross(k, a)
+    nb = np.linalg.norm(b, axis=1)
+    nb[nb == 0] = 1.0
+    b *= (speed / n_modes) / nb[:, None]
+    return np.concatenate([k, phi[:, None], b], axis=1)
+
+
+def advect3(xyz: np.ndarray, modes: np.ndarray, dt: float, steps: int = 1) -> np.ndarray:
+    """Forward Euler in the potential_modes field (host restatement of the
+    device advection, for tests)."""
+    p = np.array(xyz, np.float64, copy=True)
+    for _ in range(steps):
+        u = np.zeros_like(p)
+        for kx, ky, kz, phi, bx, by, bz in modes:
+            c = np.cos(kx * p[:, 0] + ky * p[:, 1] + kz * p[:, 2] + phi)
+            u[:, 0] += c * bx
+            u[:, 1] += c * by
+            u[:, 2] += c * bz
+        p += dt * u
+    return p

@@ -20,7 +20,7 @@ constexpr uint32_t kRsd = 3 * kHid;                  // row stride of [R|S|Dg]
 constexpr uint32_t kPq = 2 * kHid;                   // row stride of [P|Q]
 constexpr uint32_t kHeadPart = 448;                  // de_w 192, de_b 64, o_w 64, dd_b 64, d_w 64
 constexpr uint32_t kMsgPart = 256;                   // e_w 192, m_b 64
-constexpr uint32_t kEncPart = 320;                   // enc_w 256, enc_b 64
+constexpr uint32_t kEncPart = (kNodeFeats + 1) * kHid;  // enc_w 5 x 64, enc_b 64
 constexpr uint32_t kEdgeGrid = 2048;                 // blocks of the partial-writing edge kernels
 constexpr uint32_t kGemmGrid = 2048;
 constexpr uint32_t kDwGrid = 512;
@@ -60,21 +60,36 @@ __device__ __forceinline__ float softplusf(float x) { return x > 20.f ? x : log1
 // ---------------------------------------------------------------------------
 
 // Per row: diagonal position, 1/sqrt(A_ii), degree; per block: box of the
-// points and the sum of off-diagonal edge lengths.
+// points and the sum of off-diagonal edge lengths (dim 2 or 3).
+constexpr int kPrepVals = 8;  // xmin, xmax, ymin, ymax, zmin, zmax, length sum, count
+
+__device__ __forceinline__ double edge_length(const double* __restrict__ xy, uint32_t xs, uint32_t dim, uint32_t lo,
+                                              uint32_t hi) {
+    const double dx = xy[(size_t)lo * xs] - xy[(size_t)hi * xs];
+    const double dy = xy[(size_t)lo * xs + 1] - xy[(size_t)hi * xs + 1];
+    double q = dx * dx + dy * dy;
+    if (dim == 3) {
+        const double dz = xy[(size_t)lo * xs + 2] - xy[(size_t)hi * xs + 2];
+        q += dz * dz;
+    }
+    return sqrt(q);
+}
+
 __global__ __launch_bounds__(256) void prep_rows_kernel(uint32_t n, const uint32_t* __restrict__ row,
                                                         const uint32_t* __restrict__ col,
                                                         const double* __restrict__ val, const double* __restrict__ xy,
-                                                        uint32_t xs, double* __restrict__ sd,
+                                                        uint32_t xs, uint32_t dim, double* __restrict__ sd,
                                                         uint32_t* __restrict__ dpos, float* __restrict__ deg,
                                                         double* __restrict__ part) {
-    __shared__ double red[6][256];
-    double bx0 = INFINITY, bx1 = -INFINITY, by0 = INFINITY, by1 = -INFINITY, ls = 0.0, cnt = 0.0;
+    __shared__ double red[kPrepVals][256];
+    double bx[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+    double ls = 0.0, cnt = 0.0;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const double xi = xy[(size_t)i * xs], yi = xy[(size_t)i * xs + 1];
-        bx0 = fmin(bx0, xi);
-        bx1 = fmax(bx1, xi);
-        by0 = fmin(by0, yi);
-        by1 = fmax(by1, yi);
+        for (uint32_t a = 0; a < dim; ++a) {
+            const double c = xy[(size_t)i * xs + a];
+            bx[2 * a] = fmin(bx[2 * a], c);
+            bx[2 * a + 1] = fmax(bx[2 * a + 1], c);
+        }
         uint32_t dp = ~0u, d = 0;
         double aii = 0.0;
         for (uint32_t e = row[i]; e < row[i + 1]; ++e) {
@@ -85,75 +100,69 @@ __global__ __launch_bounds__(256) void prep_rows_kernel(uint32_t n, const uint32
                 continue;
             }
             ++d;
-            const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
-            const double dx = xy[(size_t)lo * xs] - xy[(size_t)hi * xs];
-            const double dy = xy[(size_t)lo * xs + 1] - xy[(size_t)hi * xs + 1];
-            ls += sqrt(dx * dx + dy * dy);
+            ls += edge_length(xy, xs, dim, i < j ? i : j, i < j ? j : i);
         }
         cnt += d;
         dpos[i] = dp;
         deg[i] = (float)d;
         sd[i] = aii > 0.0 ? 1.0 / sqrt(aii) : 1.0;
     }
-    red[0][threadIdx.x] = bx0;
-    red[1][threadIdx.x] = bx1;
-    red[2][threadIdx.x] = by0;
-    red[3][threadIdx.x] = by1;
-    red[4][threadIdx.x] = ls;
-    red[5][threadIdx.x] = cnt;
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = bx[k];
+    red[6][threadIdx.x] = ls;
+    red[7][threadIdx.x] = cnt;
     __syncthreads();
     for (uint32_t w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) {
             const uint32_t o = threadIdx.x + w;
-            red[0][threadIdx.x] = fmin(red[0][threadIdx.x], red[0][o]);
-            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][o]);
-            red[2][threadIdx.x] = fmin(red[2][threadIdx.x], red[2][o]);
-            red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][o]);
-            red[4][threadIdx.x] += red[4][o];
-            red[5][threadIdx.x] += red[5][o];
+            for (int k = 0; k < 6; k += 2) {
+                red[k][threadIdx.x] = fmin(red[k][threadIdx.x], red[k][o]);
+                red[k + 1][threadIdx.x] = fmax(red[k + 1][threadIdx.x], red[k + 1][o]);
+            }
+            red[6][threadIdx.x] += red[6][o];
+            red[7][threadIdx.x] += red[7][o];
         }
         __syncthreads();
     }
-    if (threadIdx.x < 6) part[(size_t)blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < kPrepVals) part[(size_t)blockIdx.x * kPrepVals + threadIdx.x] = red[threadIdx.x][0];
 }
 
 __global__ __launch_bounds__(64) void prep_fold_kernel(uint32_t nblk, const double* __restrict__ part,
                                                        double* __restrict__ scal) {
     if (threadIdx.x != 0) return;
-    double b[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, 0.0, 0.0};
+    double b[kPrepVals] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY, 0.0, 0.0};
     for (uint32_t k = 0; k < nblk; ++k) {
-        const double* p = part + (size_t)k * 6;
-        b[0] = fmin(b[0], p[0]);
-        b[1] = fmax(b[1], p[1]);
-        b[2] = fmin(b[2], p[2]);
-        b[3] = fmax(b[3], p[3]);
-        b[4] += p[4];
-        b[5] += p[5];
+        const double* p = part + (size_t)k * kPrepVals;
+        for (int c = 0; c < 6; c += 2) {
+            b[c] = fmin(b[c], p[c]);
+            b[c + 1] = fmax(b[c + 1], p[c + 1]);
+        }
+        b[6] += p[6];
+        b[7] += p[7];
     }
-    for (int k = 0; k < 6; ++k) scal[k] = b[k];
+    for (int k = 0; k < kPrepVals; ++k) scal[k] = b[k];
 }
 
 // Node features, edge features (bf16-rounded) and the transpose map.
 __global__ __launch_bounds__(256) void prep_feat_kernel(uint32_t n, const uint32_t* __restrict__ row,
                                                         const uint32_t* __restrict__ col,
                                                         const double* __restrict__ val, const double* __restrict__ xy,
-                                                        uint32_t xs, const uint8_t* __restrict__ bnd,
+                                                        uint32_t xs, uint32_t dim, const uint8_t* __restrict__ bnd,
                                                         const double* __restrict__ sd,
                                                         const uint32_t* __restrict__ dpos,
                                                         const double* __restrict__ scal, float* __restrict__ nfeat,
                                                         float2* __restrict__ efeat, uint32_t* __restrict__ tr,
                                                         uint32_t* __restrict__ flags) {
-    const double xmin = scal[0], ymin = scal[2];
     double ext = fmax(scal[1] - scal[0], scal[3] - scal[2]);
+    if (dim == 3) ext = fmax(ext, scal[5] - scal[4]);
     if (!(ext > 0.0)) ext = 1.0;
-    const double lbar = scal[5] > 0.0 ? scal[4] / scal[5] : 1.0;
+    const double lbar = scal[7] > 0.0 ? scal[6] / scal[7] : 1.0;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const double xi = xy[(size_t)i * xs], yi = xy[(size_t)i * xs + 1];
         const double aii = dpos[i] != ~0u ? val[dpos[i]] : 0.0;
-        nfeat[(size_t)i * 4 + 0] = (float)((xi - xmin) / ext);
-        nfeat[(size_t)i * 4 + 1] = (float)((yi - ymin) / ext);
-        nfeat[(size_t)i * 4 + 2] = bnd[i] ? 1.f : 0.f;
-        nfeat[(size_t)i * 4 + 3] = (float)(aii / 8.0);
+        float* nf = nfeat + (size_t)i * kNodeFeats;
+        for (uint32_t a = 0; a < 3; ++a)
+            nf[a] = a < dim ? (float)((xy[(size_t)i * xs + a] - scal[2 * a]) / ext) : 0.f;
+        nf[3] = bnd[i] ? 1.f : 0.f;
+        nf[4] = (float)(aii / 8.0);
         for (uint32_t e = row[i]; e < row[i + 1]; ++e) {
             const uint32_t j = col[e];
             if (j == i) {
@@ -163,9 +172,7 @@ __global__ __launch_bounds__(256) void prep_feat_kernel(uint32_t n, const uint32
             }
             const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
             const double ah = val[e] * (sd[lo] * sd[hi]);
-            const double dx = xy[(size_t)lo * xs] - xy[(size_t)hi * xs];
-            const double dy = xy[(size_t)lo * xs + 1] - xy[(size_t)hi * xs + 1];
-            const double len = sqrt(dx * dx + dy * dy) / lbar;
+            const double len = edge_length(xy, xs, dim, lo, hi) / lbar;
             efeat[e] = make_float2(bfr((float)ah), bfr((float)len));
             // (j, i) in row j: binary search (columns sorted)
             uint32_t a = row[j], b = row[j + 1];
@@ -344,7 +351,7 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __res
 }
 
 // ---------------------------------------------------------------------------
-// Encoder (VALU: four inputs)
+// Encoder (VALU: kNodeFeats inputs)
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void enc_fwd_kernel(uint32_t n, const float* __restrict__ nfeat,
@@ -352,13 +359,13 @@ __global__ __launch_bounds__(256) void enc_fwd_kernel(uint32_t n, const float* _
                                                       float* __restrict__ h0) {
     const uint32_t f = threadIdx.x & 63;
     const float b = w[kParams.enc_b + f];
-    float wc[4];
+    float wc[kNodeFeats];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) wc[c] = bf2f(wb[kParams.enc_w + c * kHid + f]);
+    for (int c = 0; c < (int)kNodeFeats; ++c) wc[c] = bf2f(wb[kParams.enc_w + c * kHid + f]);
     for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += gridDim.x * 4) {
         float z = 0.f;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) z += bfr(nfeat[(size_t)i * 4 + c]) * wc[c];
+        for (int c = 0; c < (int)kNodeFeats; ++c) z += bfr(nfeat[(size_t)i * kNodeFeats + c]) * wc[c];
         h0[(size_t)i * kHid + f] = fmaxf(z + b, 0.f);
     }
 }
@@ -366,22 +373,25 @@ __global__ __launch_bounds__(256) void enc_fwd_kernel(uint32_t n, const float* _
 __global__ __launch_bounds__(256) void enc_bwd_kernel(uint32_t n, const float* __restrict__ nfeat,
                                                       const float* __restrict__ h0, const float* __restrict__ dh,
                                                       float* __restrict__ partial) {
-    __shared__ float red[5][256];
+    constexpr int K = kNodeFeats + 1;
+    __shared__ float red[K][256];
     const uint32_t f = threadIdx.x & 63;
-    float a[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    float a[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) a[c] = 0.f;
     for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += gridDim.x * 4) {
         const float d = h0[(size_t)i * kHid + f] > 0.f ? dh[(size_t)i * kHid + f] : 0.f;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) a[c] += bfr(nfeat[(size_t)i * 4 + c]) * d;
-        a[4] += d;
+        for (int c = 0; c < (int)kNodeFeats; ++c) a[c] += bfr(nfeat[(size_t)i * kNodeFeats + c]) * d;
+        a[kNodeFeats] += d;
     }
 #pragma unroll
-    for (int c = 0; c < 5; ++c) red[c][threadIdx.x] = a[c];
+    for (int c = 0; c < K; ++c) red[c][threadIdx.x] = a[c];
     __syncthreads();
     if (threadIdx.x < 64) {
         float* out = partial + (size_t)blockIdx.x * kEncPart;
 #pragma unroll
-        for (int c = 0; c < 5; ++c)
+        for (int c = 0; c < K; ++c)
             out[c * kHid + f] = red[c][f] + red[c][f + 64] + red[c][f + 128] + red[c][f + 192];
     }
 }
@@ -803,18 +813,19 @@ void mpnn_prepare(MpnnGraph& g, hipStream_t s) {
     g.sd.resize(n ? n : 1);
     g.dpos.resize(n ? n : 1);
     g.deg.resize(n ? n : 1);
-    g.nfeat.resize(4ull * (n ? n : 1));
+    if (g.dim != 2 && g.dim != 3) throw std::invalid_argument("graph dimension must be 2 or 3");
+    g.nfeat.resize((size_t)kNodeFeats * (n ? n : 1));
     g.efeat.resize(g.nnz ? g.nnz : 1);
     g.tr.resize(g.nnz ? g.nnz : 1);
     g.scal.resize(8);
     if (!n) return;
     const unsigned nb = blocks_for(n, 256, 1024);
-    g.part.resize(6ull * nb);
-    hipLaunchKernelGGL(prep_rows_kernel, dim3(nb), dim3(256), 0, s, n, g.row, g.col, g.val, g.xy, g.xy_stride,
+    g.part.resize((size_t)kPrepVals * nb);
+    hipLaunchKernelGGL(prep_rows_kernel, dim3(nb), dim3(256), 0, s, n, g.row, g.col, g.val, g.xy, g.xy_stride, g.dim,
                        g.sd.p, g.dpos.p, g.deg.p, g.part.p);
     hipLaunchKernelGGL(prep_fold_kernel, dim3(1), dim3(64), 0, s, nb, g.part.p, g.scal.p);
     hipLaunchKernelGGL(prep_feat_kernel, dim3(blocks_for(n, 256, 4096)), dim3(256), 0, s, n, g.row, g.col, g.val,
-                       g.xy, g.xy_stride, g.bnd, g.sd.p, g.dpos.p, g.scal.p, g.nfeat.p, g.efeat.p, g.tr.p,
+                       g.xy, g.xy_stride, g.dim, g.bnd, g.sd.p, g.dpos.p, g.scal.p, g.nfeat.p, g.efeat.p, g.tr.p,
                        ctx().d_flags);
 }
 
@@ -1024,23 +1035,24 @@ int32_t PrecLearned_set_adam(PrecLearned* h, float b1, float b2, float eps) {
 }
 
 // Host system: CSR of A (FP64, sorted columns, symmetric pattern), point
-// coordinates [n][2] in the matrix's row order and the Dirichlet flags.
-int32_t PrecLearned_set_system(PrecLearned* h, uint32_t n, const uint32_t* row, const uint32_t* col,
-                               const double* val, const double* xy, const uint8_t* bnd) {
+// coordinates [n][dim] in the matrix's row order and the Dirichlet flags.
+int32_t PrecLearned_set_system_dim(PrecLearned* h, uint32_t dim, uint32_t n, const uint32_t* row,
+                                   const uint32_t* col, const double* val, const double* xyz, const uint8_t* bnd) {
     return guarded([&] {
-        if (!h || !row || (n && (!xy || !bnd))) throw std::invalid_argument("PrecLearned_set_system: null argument");
+        if (!h || !row || (n && (!xyz || !bnd))) throw std::invalid_argument("PrecLearned_set_system: null argument");
+        if (dim != 2 && dim != 3) throw std::invalid_argument("PrecLearned_set_system: dim must be 2 or 3");
         const uint32_t nnz = row[n];
         if (nnz && (!col || !val)) throw std::invalid_argument("PrecLearned_set_system: null argument");
         MpnnGraph& g = h->graph;
         g.own_row.resize(n + 1);
         g.own_col.resize(nnz ? nnz : 1);
         g.own_val.resize(nnz ? nnz : 1);
-        g.own_xy.resize(2ull * (n ? n : 1));
+        g.own_xy.resize((size_t)dim * (n ? n : 1));
         g.own_bnd.resize(n ? n : 1);
         h2d(g.own_row.p, row, n + 1);
         h2d(g.own_col.p, col, nnz);
         h2d(g.own_val.p, val, nnz);
-        h2d(g.own_xy.p, xy, 2ull * n);
+        h2d(g.own_xy.p, xyz, (size_t)dim * n);
         h2d(g.own_bnd.p, bnd, n);
         g.n = n;
         g.nnz = nnz;
@@ -1048,11 +1060,16 @@ int32_t PrecLearned_set_system(PrecLearned* h, uint32_t n, const uint32_t* row, 
         g.col = g.own_col.p;
         g.val = g.own_val.p;
         g.xy = g.own_xy.p;
-        g.xy_stride = 2;
+        g.xy_stride = dim;
+        g.dim = dim;
         g.bnd = g.own_bnd.p;
         mpnn_prepare(g, ctx().stream);
         mpnn_check_flags();
     });
+}
+int32_t PrecLearned_set_system(PrecLearned* h, uint32_t n, const uint32_t* row, const uint32_t* col,
+                               const double* val, const double* xy, const uint8_t* bnd) {
+    return PrecLearned_set_system_dim(h, 2, n, row, col, val, xy, bnd);
 }
 
 // Forward pass and loss on the bound system.  loss: Kaporin loss (nullable);

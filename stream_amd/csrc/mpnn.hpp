@@ -7,7 +7,8 @@
 //
 // Network (hidden 64, 3 message-passing layers, weights in BF16 on MFMA,
 // FP32 accumulation, FP32 master weights + Adam):
-//   F_i  = [(x_i-xmin)/ext, (y_i-ymin)/ext, bnd_i, A_ii/8]
+//   F_i  = [(x_i-xmin)/ext, (y_i-ymin)/ext, (z_i-zmin)/ext, bnd_i, A_ii/8]
+//          (ext = largest box extent; the z feature is 0 for 2D systems)
 //   e_ij = [Ahat_ij, |x_i-x_j|/lbar, sign(j-i)]
 //   H0 = relu(F W_enc + b_enc)
 //   layer: [P|Q] = H W_pq;  M_i = mean_j relu(P_i + Q_j + e_ij W_e + b_m)
@@ -32,13 +33,13 @@ namespace stream_amd {
 
 constexpr uint32_t kHid = 64;
 constexpr uint32_t kLayers = 3;
-constexpr uint32_t kNodeFeats = 4;
+constexpr uint32_t kNodeFeats = 5;
 constexpr uint32_t kEdgeFeats = 3;
 constexpr uint32_t kMaxUpper = 31;  // upper entries per row handled by the loss kernel
 constexpr int kFlagLearned = 1;      // d_flags word: non-finite learned factor
 
 // Flat parameter layout (FP32 master copy), in this order:
-// enc_w[4][64] enc_b[64] | per layer: pq_w[64][128] e_w[3][64] m_b[64]
+// enc_w[5][64] enc_b[64] | per layer: pq_w[64][128] e_w[3][64] m_b[64]
 // u_w[128][64] u_b[64] | dec_w[64][192] de_w[3][64] de_b[64] o_w[64]
 // dd_b[64] d_w[64].  Matrices are [in][out] row-major.
 struct ParamOffsets {
@@ -79,17 +80,18 @@ struct MpnnGraph {
     const uint32_t* row = nullptr;
     const uint32_t* col = nullptr;
     const double* val = nullptr;
-    const double* xy = nullptr;  // point i at xy[i * xy_stride], xy[i * xy_stride + 1]
+    const double* xy = nullptr;  // point i at xy[i * xy_stride + 0 .. dim-1]
     uint32_t xy_stride = 2;
+    uint32_t dim = 2;            // 2 or 3
     const uint8_t* bnd = nullptr;
     DBuf<double> sd;          // [n] 1/sqrt(A_ii) (1 when A_ii <= 0)
     DBuf<uint32_t> dpos;      // [n] CSR position of the diagonal (~0u if absent)
     DBuf<float> deg;          // [n] off-diagonal count
-    DBuf<float> nfeat;        // [n][4]
+    DBuf<float> nfeat;        // [n][kNodeFeats]
     DBuf<float2> efeat;       // [nnz] (Ahat_ij, len_ij/lbar), rounded to bf16
     DBuf<uint32_t> tr;        // [nnz] position of (j, i) for entry (i, j)
     DBuf<double> part;        // reduction partials
-    DBuf<double> scal;        // [8] xmin, xmax, ymin, ymax, len sum, count
+    DBuf<double> scal;        // [8] xmin, xmax, ymin, ymax, zmin, zmax, len sum, count
     // owned copies for host-provided systems
     DBuf<uint32_t> own_row, own_col;
     DBuf<double> own_val, own_xy;

@@ -249,6 +249,7 @@ struct Step2D {
         g.val = A.val.p;
         g.xy = reinterpret_cast<const double*>(mesh.lbvh.obj_m.p);  // Sph4: c[0], c[1] first
         g.xy_stride = sizeof(Sph4<double>) / sizeof(double);
+        g.dim = 2;
         g.bnd = mesh.node_is_bnd.p;
         mpnn_prepare(g, s);
     }

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "ashape3d.hpp"
+#include "mpnn.hpp"
 #include "numerics.hpp"
 #include "precond.hpp"
 
@@ -225,6 +226,36 @@ __global__ __launch_bounds__(256) void permute3_bwd_kernel(const T* __restrict__
 
 }  // namespace
 
+namespace {
+// Divergence-free 3D velocity u(x) = sum_m cos(k_m . x + phi_m) b_m with
+// b_m = k_m x a_m, the curl of the vector potential sum_m a_m sin(k_m . x +
+// phi_m).  Rows (kx, ky, kz, phi, bx, by, bz).
+struct Modes3 {
+    uint32_t n = 0;
+    double m[32][7];
+};
+
+// One forward-Euler step x += dt u(x) on the input-order points.
+__global__ __launch_bounds__(256) void advect3_kernel(uint32_t n, Modes3 md, double dt,
+                                                      Sph4<double>* __restrict__ pts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Sph4<double> p = pts[i];
+    double u[3] = {0.0, 0.0, 0.0};
+    for (uint32_t k = 0; k < md.n; ++k) {
+        const double* r = md.m[k];
+        const double c = cos(r[0] * p.c[0] + r[1] * p.c[1] + r[2] * p.c[2] + r[3]);
+        u[0] += c * r[4];
+        u[1] += c * r[5];
+        u[2] += c * r[6];
+    }
+    p.c[0] += dt * u[0];
+    p.c[1] += dt * u[1];
+    p.c[2] += dt * u[2];
+    pts[i] = p;
+}
+}  // namespace
+
 struct Step3D {
     Ash3D<double> mesh;
     DBuf<Sph4<double>> pts;  // input order
@@ -241,6 +272,15 @@ struct Step3D {
     SolveStats last;
     double t_mesh = 0, t_asm = 0, t_prep = 0, t_solve = 0;
     uint32_t n = 0;
+    PrecLearned* learned = nullptr;  // network of solver 3 (not owned)
+    bool learned_fallback = false;   // the last learned factor was rejected
+
+    void advect(const Modes3& md, double dt, uint32_t steps) {
+        hipStream_t s = ctx().stream;
+        for (uint32_t k = 0; k < steps; ++k)
+            hipLaunchKernelGGL(advect3_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, md, dt, pts.p);
+        SA_HIP(hipStreamSynchronize(s));
+    }
 
     void set_points(uint32_t n_, const double* h_xyz, const double* h_alpha, const double* h_f) {
         n = n_;
@@ -306,12 +346,42 @@ struct Step3D {
         hipLaunchKernelGGL(p1_tet_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, a);
     }
 
+    // Point the network's graph at this step's system (after assembly).
+    void bind_graph(MpnnGraph& g, hipStream_t s) {
+        g.n = n;
+        g.nnz = A.nnz;
+        g.row = A.row.p;
+        g.col = A.col.p;
+        g.val = A.val.p;
+        g.xy = reinterpret_cast<const double*>(mesh.lbvh.obj_m.p);  // Sph4: c[0..2] first
+        g.xy_stride = sizeof(Sph4<double>) / sizeof(double);
+        g.dim = 3;
+        g.bnd = mesh.node_is_bnd.p;
+        mpnn_prepare(g, s);
+    }
+
+    // SELL copy and factor for the solver; the learned factor is rejected
+    // (Jacobi used instead) when a value is non-finite or a diagonal is not
+    // positive.
     void prepare(int solver, hipStream_t s) {
+        learned_fallback = false;
         if (solver == 0) {
             build_sell<double>(n, A.row.p, A.col.p, A.val.p, nullptr, kSellStandard, kSellOffDiag, S64, s);
         } else {
             build_sell<float>(n, A.row.p, A.col.p, A.val.p, nullptr, kSellRowSum, kSellOffDiag, S32, s);
             if (solver == 2) fsai_factor<float>(A, L32, s);
+            if (solver == 3) {
+                if (!learned) throw std::invalid_argument("solver 3 needs a network (PfemStep3D_set_learned)");
+                factor_structure<float>(A, L32, s);
+                bind_graph(learned->graph, s);
+                learned->net.forward(learned->graph, s);
+                learned->net.pack(learned->graph, L32, s);
+                const uint32_t bad = read_scalar(ctx().d_flags + kFlagLearned);
+                if (bad) {
+                    fill<uint32_t>(ctx().d_flags + kFlagLearned, 1, 0u, s);
+                    learned_fallback = true;
+                }
+            }
         }
     }
 
@@ -341,6 +411,7 @@ struct Step3D {
     }
 
     void solve(int solver, double rtol, uint32_t maxit) {
+        if (solver == 3 && learned_fallback) solver = 1;
         hipStream_t s = ctx().stream;
         fill<double>(x.p, n, 0.0, s);
         CgOptions o;
@@ -402,10 +473,11 @@ int32_t PfemStep3D_set_morton_bits(PfemStep3D* h, int32_t bits) {
 
 // One 3D step: LBVH + alpha-shape tetrahedra + P1 assembly + CG.  solver:
 // 0 = FP64 Jacobi-PCG, 1 = mixed FP32/FP64 Jacobi-PCG, 2 = mixed PCG with the
-// FSAI L L^T factor.  stats [10] as PfemStep2D_run (entry 9 is 0).
+// FSAI L L^T factor, 3 = mixed PCG with the learned L L^T factor (network
+// set by PfemStep3D_set_learned).  stats [10] as PfemStep2D_run.
 int32_t PfemStep3D_run(PfemStep3D* h, int32_t solver, double rtol, uint32_t maxit, double* stats) {
     return g3([&] {
-        if (solver < 0 || solver > 2) throw std::invalid_argument("3D step solvers are 0, 1 and 2");
+        if (solver < 0 || solver > 3) throw std::invalid_argument("3D step solvers are 0, 1, 2 and 3");
         h->s.mesh_and_assemble(h->bits, solver);
         h->s.solve(solver, rtol, maxit);
         if (stats) {
@@ -418,7 +490,7 @@ int32_t PfemStep3D_run(PfemStep3D* h, int32_t solver, double rtol, uint32_t maxi
             stats[6] = h->s.A.nnz;
             stats[7] = h->s.mesh.n_elems();
             stats[8] = h->s.t_prep;
-            stats[9] = 0.0;
+            stats[9] = h->s.learned_fallback ? 1.0 : 0.0;
         }
         const uint32_t fl = read_scalar(ctx().d_flags + kFlagError);
         if (fl) {
@@ -426,6 +498,43 @@ int32_t PfemStep3D_run(PfemStep3D* h, int32_t solver, double rtol, uint32_t maxi
             throw std::runtime_error("device error flag raised during the 3D step");
         }
         if (h->s.last.true_rel > rtol) throw std::runtime_error("CG did not reach the requested residual");
+    });
+}
+
+// Network used by solver 3 (not owned; nullptr clears it).
+int32_t PfemStep3D_set_learned(PfemStep3D* h, PrecLearned* net) {
+    return g3([&] { h->s.learned = net; });
+}
+// Point the network at the step's current (assembled) system.
+int32_t PrecLearned_bind_step3d(PrecLearned* net, PfemStep3D* h) {
+    return g3([&] {
+        if (!net || !h) throw std::invalid_argument("PrecLearned_bind_step3d: null argument");
+        h->s.bind_graph(net->graph, ctx().stream);
+        mpnn_check_flags();
+    });
+}
+
+// Advect the points `steps` forward-Euler steps of dt in the
+// divergence-free field of n_modes rows (kx, ky, kz, phi, bx, by, bz):
+// u(x) = sum cos(k . x + phi) b (b = k x a, the curl of a sin(k . x +
+// phi)).  The source f is left as given.
+int32_t PfemStep3D_advect(PfemStep3D* h, uint32_t n_modes, const double* modes, double dt, uint32_t steps) {
+    return g3([&] {
+        if (n_modes > 32) throw std::invalid_argument("at most 32 modes");
+        Modes3 md;
+        md.n = n_modes;
+        for (uint32_t k = 0; k < n_modes; ++k)
+            for (int c = 0; c < 7; ++c) md.m[k][c] = modes[7 * k + c];
+        h->s.advect(md, dt, steps);
+    });
+}
+// Current points (input order, row-major xyz [3N]).
+int32_t PfemStep3D_get_points(PfemStep3D* h, double* xyz) {
+    return g3([&] {
+        std::vector<Sph4<double>> hp(h->s.n);
+        d2h(hp.data(), h->s.pts.p, h->s.n);
+        for (uint32_t i = 0; i < h->s.n; ++i)
+            for (int c = 0; c < 3; ++c) xyz[3ull * i + c] = hp[i].c[c];
     });
 }
 

@@ -1,4 +1,5 @@
 // Process context and error reporting for the C-ABI.
+#include <atomic>
 #include <mutex>
 #include <string>
 
@@ -9,7 +10,16 @@ namespace stream_amd {
 namespace {
 std::mutex g_err_mu;
 std::string g_err;
+std::atomic<int64_t> g_bytes{0};
+std::atomic<int64_t> g_peak{0};
 }  // namespace
+
+void note_device_bytes(int64_t delta) {
+    const int64_t now = g_bytes.fetch_add(delta) + delta;
+    int64_t pk = g_peak.load();
+    while (now > pk && !g_peak.compare_exchange_weak(pk, now)) {
+    }
+}
 
 void set_last_error(const std::string& msg) {
     std::lock_guard<std::mutex> lk(g_err_mu);
@@ -48,6 +58,25 @@ int32_t Stream_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+// Device bytes held by the library's buffers now and at the high-water mark
+// (reset_peak != 0 restarts the mark at the current value), and the device
+// free / total memory as the runtime reports it.  Any pointer may be null.
+int32_t Stream_device_bytes(uint64_t* current, uint64_t* peak, int32_t reset_peak, uint64_t* dev_free,
+                            uint64_t* dev_total) {
+    return stream_amd::guarded([&] {
+        if (reset_peak) stream_amd::g_peak.store(stream_amd::g_bytes.load());
+        if (current) *current = (uint64_t)stream_amd::g_bytes.load();
+        if (peak) *peak = (uint64_t)stream_amd::g_peak.load();
+        if (dev_free || dev_total) {
+            size_t f = 0, t = 0;
+            (void)stream_amd::ctx();
+            SA_HIP(hipMemGetInfo(&f, &t));
+            if (dev_free) *dev_free = f;
+            if (dev_total) *dev_total = t;
+        }
+    });
 }
 
 // Block until all queued device work is done.

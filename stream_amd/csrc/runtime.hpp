@@ -57,6 +57,10 @@ Context& ctx();
 
 // Grow-only device buffer.  resize() never shrinks; contents are not
 // preserved across a growing resize.
+// Bytes held by DBuf buffers of this process, now and at the high-water
+// mark since the last reset (Stream_device_bytes).
+void note_device_bytes(int64_t delta);
+
 template <class T> struct DBuf {
     T* p = nullptr;
     size_t n = 0;
@@ -66,17 +70,25 @@ template <class T> struct DBuf {
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            note_device_bytes(-(int64_t)(cap * sizeof(T)));
+        }
         p = nullptr;
         n = cap = 0;
     }
     void resize(size_t m) {
         if (m > cap) {
-            if (p) SA_HIP(hipFree(p));
+            if (p) {
+                SA_HIP(hipFree(p));
+                note_device_bytes(-(int64_t)(cap * sizeof(T)));
+            }
             p = nullptr;
+            cap = 0;
             size_t c = m < 64 ? 64 : m;
             SA_HIP(hipMalloc((void**)&p, c * sizeof(T)));
             cap = c;
+            note_device_bytes((int64_t)(c * sizeof(T)));
         }
         n = m;
     }

@@ -4,7 +4,9 @@ GPU by the library's own forward and backward kernels.
 No counterpart in the reference, whose only preconditioner is PrecJacobi
 (include/numerics/preconditioner.hpp:60-72, python/numerics/jacobi.py);
 `PrecLearned` follows that class's create/destroy pattern and is consumed
-by `PfemStep2D.run(solver="learned")`.
+by `PfemStep2D.run(solver="learned")` and `PfemStep3D.run(solver="learned")`.
+Node features are (x, y, z, boundary flag, A_ii / 8), z = 0 for 2D
+systems, so one architecture serves both.
 """
 from __future__ import annotations
 
@@ -28,7 +30,11 @@ PrecLearned_set_adam = bind(lib, "PrecLearned_set_adam", i32, [_p, ctypes.c_floa
 PrecLearned_set_system = bind(lib, "PrecLearned_set_system", i32,
                               [_p, u32, arr(np.uint32, 1), arr(np.uint32, 1), arr(np.float64, 1),
                                arr(np.float64, 2), arr(np.uint8, 1)])
+PrecLearned_set_system_dim = bind(lib, "PrecLearned_set_system_dim", i32,
+                                  [_p, u32, u32, arr(np.uint32, 1), arr(np.uint32, 1), arr(np.float64, 1),
+                                   arr(np.float64, 2), arr(np.uint8, 1)])
 PrecLearned_bind_step = bind(lib, "PrecLearned_bind_step", i32, [_p, _p])
+PrecLearned_bind_step3d = bind(lib, "PrecLearned_bind_step3d", i32, [_p, _p])
 PrecLearned_forward = bind(lib, "PrecLearned_forward", i32, [_p, _dp, opt(np.float32)])
 PrecLearned_gradient = bind(lib, "PrecLearned_gradient", i32, [_p, _dp, opt(np.float32)])
 PrecLearned_train_step = bind(lib, "PrecLearned_train_step", i32, [_p, ctypes.c_float, _dp])
@@ -36,13 +42,18 @@ PrecLearned_time_gemm = bind(lib, "PrecLearned_time_gemm", i32, [_p, u32, _dp, _
 
 HIDDEN = 64
 LAYERS = 3
+NODE_FEATS = 5
+# Parameter count of files written before the z feature existed (4 node
+# features); load_params widens them with a zero z row, which leaves every
+# 2D output unchanged.
+_COUNT_4FEAT = 63168
 WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
 
 
 def param_layout():
     """(name, shape) of the parameter tensors in flat order ([in][out] matrices)."""
     H = HIDDEN
-    out = [("enc_w", (4, H)), ("enc_b", (H,))]
+    out = [("enc_w", (NODE_FEATS, H)), ("enc_b", (H,))]
     for l in range(LAYERS):
         out += [(f"pq_w{l}", (H, 2 * H)), (f"e_w{l}", (3, H)), (f"m_b{l}", (H,)), (f"u_w{l}", (2 * H, H)),
                 (f"u_b{l}", (H,))]
@@ -97,15 +108,23 @@ class PrecLearned:
         check(PrecLearned_set_adam(self.h, beta1, beta2, eps))
 
     def set_system(self, row, col, val, xy, bnd):
+        """Host system; xy holds 2 or 3 coordinates per row."""
         row = np.ascontiguousarray(row, np.uint32)
         n = row.shape[0] - 1
-        check(PrecLearned_set_system(self.h, n, row, np.ascontiguousarray(col, np.uint32),
-                                     np.ascontiguousarray(val, np.float64), np.ascontiguousarray(xy, np.float64),
-                                     np.ascontiguousarray(bnd, np.uint8)))
+        xy = np.ascontiguousarray(xy, np.float64)
+        check(PrecLearned_set_system_dim(self.h, xy.shape[1], n, row, np.ascontiguousarray(col, np.uint32),
+                                         np.ascontiguousarray(val, np.float64), xy,
+                                         np.ascontiguousarray(bnd, np.uint8)))
         self.nnz = int(row[-1])
 
     def bind_step(self, step):
-        check(PrecLearned_bind_step(self.h, step.h))
+        """Point the network at a PfemStep2D / PfemStep3D's assembled system."""
+        from .pfem import PfemStep3D
+
+        if isinstance(step, PfemStep3D):
+            check(PrecLearned_bind_step3d(self.h, step.h))
+        else:
+            check(PrecLearned_bind_step(self.h, step.h))
         self.nnz = step.nnz()
 
     def forward(self, with_ghat=False):
@@ -149,4 +168,6 @@ class PrecLearned:
         w = np.fromfile(path + ".bin", dtype="<f4")
         if w.shape[0] != info["count"]:
             raise ValueError("weight file does not match its description")
+        if w.shape[0] == _COUNT_4FEAT:
+            w = np.concatenate([w[:2 * HIDDEN], np.zeros(HIDDEN, np.float32), w[2 * HIDDEN:]])
         return w.astype(np.float32)

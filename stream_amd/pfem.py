@@ -145,13 +145,16 @@ PfemStep3D_nnz = bind(lib, "PfemStep3D_nnz", u32, [_p])
 PfemStep3D_get_nelem = bind(lib, "PfemStep3D_get_nelem", u32, [_p])
 PfemStep3D_get_system = bind(lib, "PfemStep3D_get_system", i32,
                              [_p, opt(np.uint32), opt(np.uint32), opt(np.float64), opt(np.float64)])
+PfemStep3D_advect = bind(lib, "PfemStep3D_advect", i32, [_p, u32, arr(np.float64, 2), f64, u32])
+PfemStep3D_set_learned = bind(lib, "PfemStep3D_set_learned", i32, [_p, _p])
+PfemStep3D_get_points = bind(lib, "PfemStep3D_get_points", i32, [_p, arr(np.float64, 2, writeable=True)])
 PfemStep3D_get_mesh = bind(lib, "PfemStep3D_get_mesh", i32,
                            [_p, opt(np.uint32), opt(np.uint8), opt(np.float64), opt(np.uint32)])
 
 
 class PfemStep3D:
     """The 3D step: LBVH + 3D alpha-shape + P1 on tetrahedra + CG (solvers
-    "fp64", "mixed", "fsai")."""
+    "fp64", "mixed", "fsai", "learned")."""
 
     def __init__(self, xyz, alpha, f, morton_bits: int = 21):
         self.h = PfemStep3D_create()
@@ -173,12 +176,38 @@ class PfemStep3D:
         kind = SOLVER_NAMES[solver] if isinstance(solver, str) else int(solver)
         check(PfemStep3D_run(self.h, kind, rtol, maxit, st))
         return dict(mesh_ms=st[0], assemble_ms=st[1], solve_ms=st[2], iters=int(st[3]), true_rel=st[4],
-                    replacements=int(st[5]), nnz=int(st[6]), n_elems=int(st[7]), prep_ms=st[8])
+                    replacements=int(st[5]), nnz=int(st[6]), n_elems=int(st[7]), prep_ms=st[8],
+                    fallback=bool(st[9]))
+
+    def set_learned(self, net):
+        """Network used by solver "learned" (kept alive by this object)."""
+        self._learned = net
+        check(PfemStep3D_set_learned(self.h, net.h if net is not None else None))
+
+    def nnz(self):
+        return int(PfemStep3D_nnz(self.h))
 
     def assemble(self):
         st = np.zeros(2)
         check(PfemStep3D_assemble(self.h, st))
         return float(st[0]), float(st[1])
+
+    def set_points(self, xyz, alpha, f):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        self.n = xyz.shape[0]
+        check(PfemStep3D_set_points(self.h, self.n, xyz, np.ascontiguousarray(alpha, np.float64),
+                                    np.ascontiguousarray(f, np.float64)))
+
+    def advect(self, modes, dt: float, steps: int = 1):
+        """Advect the points on the device in the divergence-free field of
+        clouds.potential_modes rows (kx, ky, kz, phi, bx, by, bz)."""
+        m = np.ascontiguousarray(modes, np.float64).reshape(-1, 7)
+        check(PfemStep3D_advect(self.h, m.shape[0], m, float(dt), int(steps)))
+
+    def points(self):
+        xyz = np.empty((self.n, 3))
+        check(PfemStep3D_get_points(self.h, xyz))
+        return xyz
 
     def solution(self):
         p = np.empty(self.n)

@@ -13,6 +13,12 @@ mixed Jacobi-PCG and the FP64 Jacobi-PCG, each verified to a true FP64
 relative residual of 1e-8.
 
     python -m stream_amd.train_learned --out stream_amd/weights/mpnn_dam2d
+
+With --dim 3 the sequence is the sloshing-tank cloud (jittered cubic grid,
+alpha = 1.3h; --points picks the size) advected in the divergence-free 3D
+field of clouds.potential_modes (seed 44), source z + sin(3x) cos(2y):
+
+    python -m stream_amd.train_learned --dim 3 --points 2000000 --out stream_amd/weights/mpnn_tank3d
 """
 from __future__ import annotations
 
@@ -26,14 +32,21 @@ import numpy as np
 
 from . import clouds, synchronize
 from .learned import PrecLearned
-from .pfem import PfemStep2D
+from .pfem import PfemStep2D, PfemStep3D
 
 
 def _log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_step(n_points: int):
+def _source3(pts):
+    return np.sin(3.0 * pts[:, 0]) * np.cos(2.0 * pts[:, 1]) + pts[:, 2]
+
+
+def make_step(n_points: int, dim: int = 2):
+    if dim == 3:
+        pts, al = clouds.sloshing_tank() if n_points == 1_996_920 else clouds.sloshing_tank_scaled(n_points)
+        return PfemStep3D(pts, al, _source3(pts)), pts, al
     if n_points == 1_000_000:
         pts, al = clouds.dam_break()
     else:
@@ -44,13 +57,20 @@ def make_step(n_points: int):
 
 
 def reset(step, pts, al):
+    if isinstance(step, PfemStep3D):
+        step.set_points(pts, al, _source3(pts))
+        return
     step.set_points(pts, al, np.zeros(len(pts)))
     step.set_source_modes(clouds.fourier_modes(11))
 
 
+def field(step):
+    return clouds.potential_modes() if isinstance(step, PfemStep3D) else clouds.stream_modes()
+
+
 def train(step, pts, al, net, epochs=4, per_snapshot=2, train_steps=150, lr=2e-3, lr_min=2e-4, dt=1e-3,
           log=_log):
-    modes = clouds.stream_modes()
+    modes = field(step)
     total = epochs * train_steps * per_snapshot
     k = 0
     history = []
@@ -79,7 +99,7 @@ def train(step, pts, al, net, epochs=4, per_snapshot=2, train_steps=150, lr=2e-3
 
 def evaluate(step, pts, al, net, start=150, stop=200, dt=1e-3, solvers=("learned", "mixed", "fp64", "fsai"),
              log=_log):
-    modes = clouds.stream_modes()
+    modes = field(step)
     reset(step, pts, al)
     step.advect(modes, dt, start)
     step.set_learned(net)
@@ -112,6 +132,7 @@ def evaluate(step, pts, al, net, start=150, stop=200, dt=1e-3, solvers=("learned
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--points", type=int, default=1_000_000)
+    ap.add_argument("--dim", type=int, default=2, choices=[2, 3])
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--per-snapshot", type=int, default=2)
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -120,14 +141,17 @@ def main():
     ap.add_argument("--out", default=None, help="weight file prefix (.bin + .json)")
     ap.add_argument("--summary", default=None, help="JSON summary path")
     args = ap.parse_args()
-    step, pts, al = make_step(args.points)
+    step, pts, al = make_step(args.points, args.dim)
     net = PrecLearned()
     hist, secs = train(step, pts, al, net, epochs=args.epochs, per_snapshot=args.per_snapshot,
                        train_steps=args.train_steps, lr=args.lr, lr_min=args.lr / 10)
     summary, rows = evaluate(step, pts, al, net, start=args.train_steps, stop=args.eval_stop)
     meta = dict(points=args.points, epochs=args.epochs, per_snapshot=args.per_snapshot, lr=args.lr,
                 train_steps=args.train_steps, train_seconds=secs, loss_per_epoch=hist, eval=summary,
-                sequence="dam-break cloud advected by the 16-mode stream function (seed 33), dt 1e-3")
+                dim=args.dim,
+                sequence=("sloshing-tank cloud advected by the 16-mode divergence-free potential field (seed 44), "
+                          "dt 1e-3" if args.dim == 3 else
+                          "dam-break cloud advected by the 16-mode stream function (seed 33), dt 1e-3"))
     if args.out:
         net.save(args.out, meta)
     out = dict(meta, per_step=[{k: (v if k == "step" else {kk: vv for kk, vv in v.items()

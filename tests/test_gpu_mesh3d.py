@@ -117,3 +117,22 @@ def test_step3d_assembly_bit_exact_and_solve():
         print(f"3D tank {n}: {solver} iters={r['iters']} true_rel={r['true_rel']:.2e} err_vs_direct={err:.2e} "
               f"mesh={r['mesh_ms']:.1f}ms asm={r['assemble_ms']:.2f}ms solve={r['solve_ms']:.1f}ms", flush=True)
         assert r["true_rel"] <= 1e-8 and err < 1e-6
+
+
+def test_step3d_advect_remesh_solve():
+    """Device advection = host restatement; the advected cloud re-meshes and
+    solves to the tolerance."""
+    from stream_amd.pfem import PfemStep3D
+
+    pts, al = clouds.sloshing_tank_scaled(40_000)
+    f = _source3(pts)
+    st = PfemStep3D(pts, al, f)
+    m = clouds.potential_modes()
+    r0 = st.run("mixed", rtol=1e-8)
+    st.advect(m, 1e-3, 5)
+    want = clouds.advect3(pts, m, 1e-3, 5)
+    got = st.points()
+    assert np.max(np.abs(got - want)) < 1e-14
+    r1 = st.run("mixed", rtol=1e-8)
+    print(f"3D advect: tets {r0['n_elems']} -> {r1['n_elems']}, iters {r0['iters']} -> {r1['iters']}", flush=True)
+    assert r1["true_rel"] <= 1e-8

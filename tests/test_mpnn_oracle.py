@@ -41,8 +41,8 @@ def _random_params(seed=3):
 
 
 def test_param_count():
-    # 4x64+64 + 3 x (64x128 + 3x64 + 64 + 128x64 + 64) + 64x192 + 3x64 + 4x64
-    assert mpnn.n_params() == 320 + 3 * 16704 + 12288 + 192 + 256
+    # 5x64+64 + 3 x (64x128 + 3x64 + 64 + 128x64 + 64) + 64x192 + 3x64 + 4x64
+    assert mpnn.n_params() == 384 + 3 * 16704 + 12288 + 192 + 256
 
 
 def test_bf16_rounding_kat():
@@ -101,6 +101,46 @@ def test_gradient_finite_differences(graph, emu):
         assert err < 5e-4, (name, fd, an)
         offs += k
     print("worst relative FD error", worst)
+
+
+def _system3d():
+    pts, al = clouds.sloshing_tank(7, 4, 4)
+    r = ol.ashape3d(pts, al, bits=21)
+    xyz = r["ordered"]
+    row, col, val, b = ol.assemble_p1_tet(xyz, r["elem"], r["node_is_bnd"], xyz[:, 2] + 1.0)
+    return mpnn.Graph(row, col, val, xyz, r["node_is_bnd"])
+
+
+def test_gradient_finite_differences_3d():
+    """3D systems: the z feature is live (non-zero gradient) and every
+    gradient block matches finite differences."""
+    g3 = _system3d()
+    assert g3.nfeat[:, 2].max() > 0.5
+    p = _random_params()
+    _, grad = mpnn.backward(mpnn.unflatten(p), g3, emu=False)  # exact arithmetic: FD sees every weight
+    assert np.abs(grad["enc_w"][2]).max() > 0
+    rng = np.random.default_rng(8)
+    offs = 0
+    for name, shape in mpnn.param_layout():
+        k = int(np.prod(shape))
+        d = np.zeros_like(p)
+        d[offs:offs + k] = rng.standard_normal(k)
+        eps = 1e-7
+        fd = (mpnn.loss(mpnn.unflatten(p + eps * d), g3, False) - mpnn.loss(mpnn.unflatten(p - eps * d), g3, False)) / (
+            2 * eps)
+        an = float(grad[name].ravel() @ d[offs:offs + k])
+        assert abs(fd - an) / max(1e-8, abs(fd), abs(an)) < 5e-4, (name, fd, an)
+        offs += k
+
+
+def test_2d_systems_ignore_the_z_weights(graph):
+    p = _random_params()
+    P = mpnn.unflatten(p)
+    a = mpnn.forward(P, graph, emu=True)
+    P2 = {k: v.copy() for k, v in P.items()}
+    P2["enc_w"][2] += 1.0
+    b = mpnn.forward(P2, graph, emu=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
 def _fsai_dense(A):
