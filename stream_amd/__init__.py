@@ -61,8 +61,8 @@ def synchronize() -> None:
 
 
 def tune(key: str, value: float) -> None:
-    """Set a solver knob: 'replace_factor', 'batch', 'cg_blocks', 'graph' or
-    'xcd_map'."""
+    """Set a solver knob: 'replace_factor', 'batch', 'cg_blocks', 'graph',
+    'xcd_map' or 'fold_group'."""
     check(lib.Stream_tune(key.encode(), float(value)))
 
 

@@ -12,6 +12,9 @@ constexpr int kCgThreads = 256;
 // grid-stride rows; the last block folds at most 16 partials per thread,
 // all loads in flight at once.
 constexpr uint32_t kMaxCgBlocks = 4096;
+constexpr uint32_t kMaxFoldGroups = kMaxCgBlocks / 8;  // smallest fold group: 8 blocks
+constexpr uint32_t kCounterStride = 32;                // one 128-byte line per ticket counter
+constexpr uint32_t kCounterWords = (1 + kMaxFoldGroups) * kCounterStride;
 
 Tuning& tuning() {
     static Tuning t;
@@ -25,11 +28,23 @@ void CgWork::resize(uint32_t n) {
     if (blocks == 0) blocks = 1;
     sc.resize(1);
     out2.resize(2);
-    partials.resize(4ull * kMaxCgBlocks);
+    partials.resize(4ull * kMaxCgBlocks + 4ull * kMaxFoldGroups);
     if (!counter.p) {
-        counter.resize(1);
-        fill<uint32_t>(counter.p, 1, 0u, ctx().stream);
+        counter.resize(kCounterWords);
+        fill<uint32_t>(counter.p, kCounterWords, 0u, ctx().stream);
     }
+}
+
+// Block-group size of the two-level fold (0: one level), read by every
+// fold; set through Stream_tune("fold_group", ...).
+__constant__ uint32_t c_fold_group = 32;
+
+void set_fold_group(uint32_t g) {
+    if (!(g == 0 || g == 8 || g == 16 || g == 32 || g == 64))
+        throw std::invalid_argument("fold_group must be 0, 8, 16, 32 or 64");
+    (void)ctx();
+    SA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_fold_group), &g, sizeof(g)));
+    tuning().fold_group = g;
 }
 
 namespace {
@@ -49,41 +64,104 @@ template <int NT> __device__ __forceinline__ double block_sum(double v, double* 
 }
 
 // Fold K per-thread values over the grid.  Every block stores its K block
-// sums; the last block to arrive sums all partials in block order and
-// returns true in all its threads with the totals in tot[].  The ticket
-// counter is reset by the last block.
+// sums.  One level (c_fold_group = 0): the last block to take a ticket sums
+// all partials in block order.  Two levels (c_fold_group = g): blocks take
+// tickets on their group's counter (one 128-byte line per group), the last
+// block of each group sums its group's partials in block order and takes a
+// ticket on the global counter, and the last group sums the group totals in
+// group order; no counter sees more than max(g, G/g) atomics, where the flat
+// fold serialises G atomics on one address.  The last block returns true in
+// all its threads with the totals in tot[]; counters are reset by the blocks
+// that finish them.  Every order is fixed, so the totals are reproducible.
 template <int K>
 __device__ bool grid_fold(const double (&v)[K], double* partials, uint32_t* counter, double (&tot)[K]) {
     __shared__ double s_tmp[kCgThreads / kWave];
     __shared__ double s_tot[K];
     __shared__ bool s_last;
     const uint32_t G = gridDim.x;  // <= kMaxCgBlocks
+    const uint32_t grp = c_fold_group;
     double bs[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) bs[k] = block_sum<kCgThreads>(v[k], s_tmp);
+    constexpr int PER = kMaxCgBlocks / kCgThreads;
+    if (grp == 0) {
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) coherent_store(&partials[(size_t)k * G + blockIdx.x], bs[k]);
+            vmem_release_order();
+            const uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_last = (t == G - 1);
+        }
+        __syncthreads();
+        if (!s_last) return false;
+        vmem_acquire_order();
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            // all loads issued before any add: one memory round trip
+            double pv[PER];
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const uint32_t b = threadIdx.x + u * kCgThreads;
+                pv[u] = b < G ? coherent_load(&partials[(size_t)k * G + b]) : 0.0;
+            }
+            double a = 0.0;
+#pragma unroll
+            for (int u = 0; u < PER; ++u) a += pv[u];
+            const double t = block_sum<kCgThreads>(a, s_tmp);
+            if (threadIdx.x == 0) s_tot[k] = t;
+        }
+        if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k) tot[k] = s_tot[k];
+        return true;
+    }
+    // two levels
+    const uint32_t ng = (G + grp - 1) / grp;
+    const uint32_t g = blockIdx.x / grp;
+    const uint32_t gsize = min(grp, G - g * grp);
+    uint32_t* gcount = counter + (size_t)(1 + g) * kCounterStride;
+    double* gpart = partials + (size_t)K * kMaxCgBlocks;  // [K][ng] group totals
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) coherent_store(&partials[(size_t)k * G + blockIdx.x], bs[k]);
         vmem_release_order();
-        const uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (t == G - 1);
+        const uint32_t t = __hip_atomic_fetch_add(gcount, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (t == gsize - 1);
     }
     __syncthreads();
     if (!s_last) return false;
     vmem_acquire_order();
-    constexpr int PER = kMaxCgBlocks / kCgThreads;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        // all loads issued before any add: one memory round trip
-        double pv[PER];
+        const uint32_t b = g * grp + threadIdx.x;
+        const double a = threadIdx.x < gsize ? coherent_load(&partials[(size_t)k * G + b]) : 0.0;
+        const double t = block_sum<kCgThreads>(a, s_tmp);
+        if (threadIdx.x == 0) s_tot[k] = t;
+    }
+    if (threadIdx.x == 0) {
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const uint32_t b = threadIdx.x + u * kCgThreads;
-            pv[u] = b < G ? coherent_load(&partials[(size_t)k * G + b]) : 0.0;
+        for (int k = 0; k < K; ++k) coherent_store(&gpart[(size_t)k * ng + g], s_tot[k]);
+        __hip_atomic_store(gcount, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        vmem_release_order();
+        const uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (t == ng - 1);
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    vmem_acquire_order();
+    constexpr int PER2 = kMaxFoldGroups / kCgThreads;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double pv[PER2];
+#pragma unroll
+        for (int u = 0; u < PER2; ++u) {
+            const uint32_t q = threadIdx.x + u * kCgThreads;
+            pv[u] = q < ng ? coherent_load(&gpart[(size_t)k * ng + q]) : 0.0;
         }
         double a = 0.0;
 #pragma unroll
-        for (int u = 0; u < PER; ++u) a += pv[u];
+        for (int u = 0; u < PER2; ++u) a += pv[u];
         const double t = block_sum<kCgThreads>(a, s_tmp);
         if (threadIdx.x == 0) s_tot[k] = t;
     }

@@ -132,8 +132,11 @@ struct Tuning {
     uint32_t cg_blocks = 1024;     // grid cap of the CG kernels (64..4096)
     uint32_t graph = 0;            // 1: replay each batch of iterations as a captured HIP graph
     uint32_t xcd_map = 1;          // SELL slices: 0 grid stride, 1 XCD-contiguous chunks per wave
+    uint32_t fold_group = 32;      // blocks per group of the two-level grid fold (0: one level)
 };
 Tuning& tuning();
+// Group size of the two-level grid fold of the CG kernels (0, 8, 16, 32, 64).
+void set_fold_group(uint32_t g);
 
 // CG on SELL matrix S (values V) with vectors in precision V.  A64 (CSR,
 // FP64) gives the true residuals.  x: FP64 solution (in/out).  With V =

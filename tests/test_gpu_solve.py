@@ -206,6 +206,20 @@ def test_graph_replay_and_xcd_map(solver):
             assert r0["true_rel"] <= 1e-8
             print(f"{solver} xcd_map={xm}: iters={r0['iters']} solve={r0['solve_ms']:.1f}ms "
                   f"graph solve={r1['solve_ms']:.1f}ms")
+        # reduction: one level and two levels of every group size converge
+        base = None
+        for fg in (0, 8, 16, 32, 64):
+            stream_amd.tune("graph", 0)
+            stream_amd.tune("fold_group", fg)
+            r = st.run(solver, rtol=1e-8)
+            assert r["true_rel"] <= 1e-8
+            base = base or r["iters"]
+            assert abs(r["iters"] - base) <= max(3, base // 50), (fg, r["iters"], base)
+            stream_amd.tune("graph", 1)
+            r2 = st.run(solver, rtol=1e-8)
+            assert r2["iters"] == r["iters"]
+            print(f"{solver} fold_group={fg}: iters={r['iters']} solve={r['solve_ms']:.1f}ms")
     finally:
         stream_amd.tune("graph", 0)
         stream_amd.tune("xcd_map", 1)
+        stream_amd.tune("fold_group", 32)
