@@ -398,7 +398,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_residual_kernel(
 // vector are exact in floating point (Sterbenz), so the FP32 product keeps
 // its relative accuracy on the smooth modes that standard-form summation
 // loses to cancellation.
-template <class V, bool DIFF>
+template <class V, bool DIFF, int CH>
 __global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(SellArgs<V> M, const V2<V>* __restrict__ cur,
                                                             V2<V>* __restrict__ nxt, V* __restrict__ q, uint32_t qs,
                                                             CgScalars* sc, double* partials, uint32_t* counter) {
@@ -416,20 +416,20 @@ __global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(SellArgs<V> M, const
         const uint32_t W = M.width[s];
         const uint32_t base = M.off[s] + lane;
         V acc = M.diag[ii] * dn;
-        for (uint32_t k = 0; k < W; k += 4) {
-            uint32_t c[4];
-            V a[4];
+        for (uint32_t k = 0; k < W; k += CH) {
+            uint32_t c[CH];
+            V a[CH];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < CH; ++u) {
                 const bool ok = k + u < L;
                 c[u] = ok ? M.col[base + (k + u) * kSlice] : ii;
                 a[u] = ok ? M.val[base + (k + u) * kSlice] : (V)0;
             }
-            V2<V> g[4];
+            V2<V> g[CH];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) g[u] = cur[c[u]];
+            for (int u = 0; u < CH; ++u) g[u] = cur[c[u]];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < CH; ++u) {
                 const V gd = g[u].x + beta * g[u].y;
                 if constexpr (DIFF)
                     acc += a[u] * (gd - dn);
@@ -500,7 +500,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_update_kernel(uint32_t n, XV* _
 // mode 0: iteration (beta, iter, done); mode 1: init, mode 2: after a
 // residual replacement (r given in rq.x, alpha = 0, x and rq_nxt untouched;
 // mode 2 keeps beta as in cg_residual_kernel).
-template <class V, class XV>
+template <class V, class XV, int CH>
 __global__ __launch_bounds__(kCgThreads) void llt_update_kernel(SellArgs<V> U, XV* __restrict__ x,
                                                                 const V2<V>* __restrict__ pd,
                                                                 const V2<V>* __restrict__ rq,
@@ -524,20 +524,20 @@ __global__ __launch_bounds__(kCgThreads) void llt_update_kernel(SellArgs<V> U, X
         const V2<V> own = rq[ii];
         const V ri = it ? own.x - av * own.y : own.x;
         V acc = U.diag[ii] * ri;
-        for (uint32_t k = 0; k < W; k += 4) {
-            uint32_t c[4];
-            V a[4];
+        for (uint32_t k = 0; k < W; k += CH) {
+            uint32_t c[CH];
+            V a[CH];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < CH; ++u) {
                 const bool ok = k + u < L;
                 c[u] = ok ? U.col[base + (k + u) * kSlice] : ii;
                 a[u] = ok ? U.val[base + (k + u) * kSlice] : (V)0;
             }
-            V2<V> g[4];
+            V2<V> g[CH];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) g[u] = rq[c[u]];
+            for (int u = 0; u < CH; ++u) g[u] = rq[c[u]];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) acc += a[u] * (it ? g[u].x - av * g[u].y : g[u].x);
+            for (int u = 0; u < CH; ++u) acc += a[u] * (it ? g[u].x - av * g[u].y : g[u].x);
         }
         if (valid) {
             if (it) {
@@ -574,7 +574,7 @@ __global__ __launch_bounds__(kCgThreads) void llt_update_kernel(SellArgs<V> U, X
 // Second half of the apply: z = L w (z_i = L_ii w_i + sum_{j<i} L_ij w_j)
 // into P[i].x.  force: run even when the solve is flagged done (init and
 // replacement passes).
-template <class V>
+template <class V, int CH>
 __global__ __launch_bounds__(kCgThreads) void llt_lo_kernel(SellArgs<V> M, const V* __restrict__ w,
                                                             V2<V>* __restrict__ P, const CgScalars* sc, int force) {
     if (!force && sc->done) return;
@@ -587,17 +587,17 @@ __global__ __launch_bounds__(kCgThreads) void llt_lo_kernel(SellArgs<V> M, const
         const uint32_t W = M.width[s];
         const uint32_t base = M.off[s] + lane;
         V acc = M.diag[ii] * w[ii];
-        for (uint32_t k = 0; k < W; k += 4) {
-            uint32_t c[4];
-            V a[4];
+        for (uint32_t k = 0; k < W; k += CH) {
+            uint32_t c[CH];
+            V a[CH];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < CH; ++u) {
                 const bool ok = k + u < L;
                 c[u] = ok ? M.col[base + (k + u) * kSlice] : ii;
                 a[u] = ok ? M.val[base + (k + u) * kSlice] : (V)0;
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) acc += a[u] * w[c[u]];
+            for (int u = 0; u < CH; ++u) acc += a[u] * w[c[u]];
         }
         if (valid) P[i].x = acc;
     }
@@ -692,7 +692,7 @@ template <class V> CgVecs<V> carve(CgWork& w, uint32_t n) {
 // updates go to x itself (V = double) or to an FP32 correction merged into
 // x at each residual replacement (V = float).
 // ---------------------------------------------------------------------------
-template <class V, bool DIFF, int PREC>
+template <class V, bool DIFF, int PREC, int CH>
 void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const LowerFactor<V>* L, const double* b,
               double* x, const CgOptions& o, CgWork& w, SolveStats& st, hipStream_t s) {
     using XV = V;
@@ -729,9 +729,9 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
             hipLaunchKernelGGL((cg_residual_kernel<V, PREC>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p,
                                b, x, reinterpret_cast<V*>(rqc), 2u, cur, idiag, w.sc.p, w.partials.p, w.counter.p,
                                mode, rtol2);
-            hipLaunchKernelGGL((llt_update_kernel<V, XV>), grid, blk, 0, s, MU, xacc, cur, rqc, rqc, vv.w, w.sc.p,
+            hipLaunchKernelGGL((llt_update_kernel<V, XV, CH>), grid, blk, 0, s, MU, xacc, cur, rqc, rqc, vv.w, w.sc.p,
                                w.partials.p, w.counter.p, mode == 0 ? 1 : 2);
-            hipLaunchKernelGGL((llt_lo_kernel<V>), grid, blk, 0, s, ML, vv.w, cur, w.sc.p, 1);
+            hipLaunchKernelGGL((llt_lo_kernel<V, CH>), grid, blk, 0, s, ML, vv.w, cur, w.sc.p, 1);
         } else {
             hipLaunchKernelGGL((cg_residual_kernel<V, PREC>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p,
                                b, x, vv.r, 1u, cur, idiag, w.sc.p, w.partials.p, w.counter.p, mode, rtol2);
@@ -750,13 +750,13 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
             if constexpr (PREC == kPrecLLt) {
                 V2<V>* rqc = vv.RQ[it & 1];
                 V2<V>* rqn = vv.RQ[(it + 1) & 1];
-                hipLaunchKernelGGL((cg_dir_kernel<V, DIFF>), grid, blk, 0, s, MA, cur, nxt,
+                hipLaunchKernelGGL((cg_dir_kernel<V, DIFF, CH>), grid, blk, 0, s, MA, cur, nxt,
                                    reinterpret_cast<V*>(rqc) + 1, 2u, w.sc.p, w.partials.p, w.counter.p);
-                hipLaunchKernelGGL((llt_update_kernel<V, XV>), grid, blk, 0, s, MU, xacc, nxt, rqc, rqn, vv.w,
+                hipLaunchKernelGGL((llt_update_kernel<V, XV, CH>), grid, blk, 0, s, MU, xacc, nxt, rqc, rqn, vv.w,
                                    w.sc.p, w.partials.p, w.counter.p, 0);
-                hipLaunchKernelGGL((llt_lo_kernel<V>), grid, blk, 0, s, ML, vv.w, nxt, w.sc.p, 0);
+                hipLaunchKernelGGL((llt_lo_kernel<V, CH>), grid, blk, 0, s, ML, vv.w, nxt, w.sc.p, 0);
             } else {
-                hipLaunchKernelGGL((cg_dir_kernel<V, DIFF>), grid, blk, 0, s, MA, cur, nxt, vv.q, 1u, w.sc.p,
+                hipLaunchKernelGGL((cg_dir_kernel<V, DIFF, CH>), grid, blk, 0, s, MA, cur, nxt, vv.q, 1u, w.sc.p,
                                    w.partials.p, w.counter.p);
                 hipLaunchKernelGGL((cg_update_kernel<V, XV, PREC>), grid, blk, 0, s, n, xacc, vv.r, nxt, vv.q, idiag,
                                    w.sc.p, w.partials.p, w.counter.p);
@@ -842,27 +842,39 @@ void true_residual(const DevCsr<double>& A, const double* b, const double* x, do
     d2h(out2, w.out2.p, 2);
 }
 
+// SELL entries loaded per step of the row loop (all loads of a step are
+// issued before their gathers).
+template <class V, int CH>
+void cg_solve_ch(const DevCsr<double>& A64, const Sell<V>& S, bool diff, int32_t prec, const V* idiag,
+                 const LowerFactor<V>* L, const double* b, double* x, const CgOptions& o, CgWork& w,
+                 SolveStats& st, hipStream_t s) {
+    if (diff) {
+        if (prec == kPrecLLt)
+            cg_drive<V, true, kPrecLLt, CH>(A64, S, idiag, L, b, x, o, w, st, s);
+        else if (prec == kPrecJacobi)
+            cg_drive<V, true, kPrecJacobi, CH>(A64, S, idiag, L, b, x, o, w, st, s);
+        else
+            cg_drive<V, true, kPrecNone, CH>(A64, S, idiag, L, b, x, o, w, st, s);
+    } else {
+        if (prec == kPrecLLt)
+            cg_drive<V, false, kPrecLLt, CH>(A64, S, idiag, L, b, x, o, w, st, s);
+        else if (prec == kPrecJacobi)
+            cg_drive<V, false, kPrecJacobi, CH>(A64, S, idiag, L, b, x, o, w, st, s);
+        else
+            cg_drive<V, false, kPrecNone, CH>(A64, S, idiag, L, b, x, o, w, st, s);
+    }
+}
+
 template <class V>
 void cg_solve(const DevCsr<double>& A64, const Sell<V>& S, bool diff, int32_t prec, const V* idiag,
               const LowerFactor<V>* L, const double* b, double* x, const CgOptions& o, CgWork& w, SolveStats& st,
               hipStream_t s) {
     if (prec == kPrecLLt && !L) throw std::invalid_argument("L L^T preconditioner requested without a factor");
     if (prec == kPrecJacobi && !idiag) throw std::invalid_argument("Jacobi preconditioner requested without a diagonal");
-    if (diff) {
-        if (prec == kPrecLLt)
-            cg_drive<V, true, kPrecLLt>(A64, S, idiag, L, b, x, o, w, st, s);
-        else if (prec == kPrecJacobi)
-            cg_drive<V, true, kPrecJacobi>(A64, S, idiag, L, b, x, o, w, st, s);
-        else
-            cg_drive<V, true, kPrecNone>(A64, S, idiag, L, b, x, o, w, st, s);
-    } else {
-        if (prec == kPrecLLt)
-            cg_drive<V, false, kPrecLLt>(A64, S, idiag, L, b, x, o, w, st, s);
-        else if (prec == kPrecJacobi)
-            cg_drive<V, false, kPrecJacobi>(A64, S, idiag, L, b, x, o, w, st, s);
-        else
-            cg_drive<V, false, kPrecNone>(A64, S, idiag, L, b, x, o, w, st, s);
-    }
+    if (tuning().sell_chunk == 4)
+        cg_solve_ch<V, 4>(A64, S, diff, prec, idiag, L, b, x, o, w, st, s);
+    else
+        cg_solve_ch<V, 8>(A64, S, diff, prec, idiag, L, b, x, o, w, st, s);
 }
 template void cg_solve<float>(const DevCsr<double>&, const Sell<float>&, bool, int32_t, const float*,
                               const LowerFactor<float>*, const double*, double*, const CgOptions&, CgWork&,
@@ -889,11 +901,18 @@ template <class V> double time_dir_kernel(const Sell<V>& S, bool diff, uint32_t 
     auto launch = [&](uint32_t k) {
         V2<V>* cur = vv.P[k & 1];
         V2<V>* nxt = vv.P[(k + 1) & 1];
-        if (diff)
-            hipLaunchKernelGGL((cg_dir_kernel<V, true>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
+        const bool c4 = tuning().sell_chunk == 4;
+        if (diff && c4)
+            hipLaunchKernelGGL((cg_dir_kernel<V, true, 4>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
+                               w.partials.p, w.counter.p);
+        else if (diff)
+            hipLaunchKernelGGL((cg_dir_kernel<V, true, 8>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
+                               w.partials.p, w.counter.p);
+        else if (c4)
+            hipLaunchKernelGGL((cg_dir_kernel<V, false, 4>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
                                w.partials.p, w.counter.p);
         else
-            hipLaunchKernelGGL((cg_dir_kernel<V, false>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
+            hipLaunchKernelGGL((cg_dir_kernel<V, false, 8>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
                                w.partials.p, w.counter.p);
     };
     for (uint32_t k = 0; k < 3; ++k) launch(k);

@@ -219,7 +219,18 @@ def test_graph_replay_and_xcd_map(solver):
             r2 = st.run(solver, rtol=1e-8)
             assert r2["iters"] == r["iters"]
             print(f"{solver} fold_group={fg}: iters={r['iters']} solve={r['solve_ms']:.1f}ms")
+        # SELL load steps of 4 or 8 entries add the same terms in the same
+        # order: identical bits
+        stream_amd.tune("graph", 0)
+        stream_amd.tune("fold_group", 32)
+        out = []
+        for ch in (4, 8):
+            stream_amd.tune("sell_chunk", ch)
+            r = st.run(solver, rtol=1e-8)
+            out.append((r["iters"], st.solution(morton=True)))
+        assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1])
     finally:
         stream_amd.tune("graph", 0)
         stream_amd.tune("xcd_map", 1)
         stream_amd.tune("fold_group", 32)
+        stream_amd.tune("sell_chunk", 8)
