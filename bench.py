@@ -143,7 +143,7 @@ def components(step, pts, al, net):
     for solver in ("fp64", "mixed", "fsai") + (("learned",) if net is not None else ()):
         step.run(solver, rtol=1e-8)
         r = step.run(solver, rtol=1e-8)
-        out[f"step_{solver}"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        out[f"step_{solver}"] = {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items()}
     out["dir_kernel_fp32_ms"] = round(step.time_dir_kernel(fp32=True, reps=200), 5)
     out["dir_kernel_fp64_ms"] = round(step.time_dir_kernel(fp32=False, reps=200), 5)
     xy = np.ascontiguousarray(pts, np.float32)

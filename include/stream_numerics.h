@@ -59,7 +59,7 @@ int32_t SolverCG_solve_mixed(SolverCG* solver, const LinSys* sys, const PrecJaco
 /* Process-wide solver knobs: "replace_factor" (mixed precision residual
  * replacement threshold, default 0.3), "batch" (iterations between host
  * checks, default 32), "cg_blocks" (grid cap of the CG kernels, 64..4096,
- * default 1024), "graph" (1: replay each batch as a captured HIP graph,
+ * default 2048), "graph" (1: replay each batch as a captured HIP graph,
  * default 0), "xcd_map" (SELL slices in XCD-contiguous chunks, default 1)
  * "fold_group" (blocks per group of the two-level reduction, 0 = one
  * level; 8, 16, 32 (default) or 64) and "sell_chunk" (SELL entries per load

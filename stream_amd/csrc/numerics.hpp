@@ -129,7 +129,7 @@ struct CgOptions {
 struct Tuning {
     double replace_factor = 3e-1;  // mixed: residual replacement when ||r|| fell by this factor
     uint32_t batch = 32;           // CG iterations between host checks
-    uint32_t cg_blocks = 1024;     // grid cap of the CG kernels (64..4096)
+    uint32_t cg_blocks = 2048;     // grid cap of the CG kernels (64..4096)
     uint32_t graph = 0;            // 1: replay each batch of iterations as a captured HIP graph
     uint32_t xcd_map = 1;          // SELL slices: 0 grid stride, 1 XCD-contiguous chunks per wave
     uint32_t fold_group = 32;      // blocks per group of the two-level grid fold (0: one level)
