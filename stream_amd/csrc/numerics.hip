@@ -289,11 +289,6 @@ template void build_sell<double>(uint32_t, const uint32_t*, const uint32_t*, con
 
 namespace {
 
-// Streamed matrix data (column indices, values, lengths, diagonal) is read
-// once per kernel: non-temporal loads keep it from evicting the vector
-// entries that neighbouring rows gather again.
-template <class T> __device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_load(p); }
-
 // Matrix arguments of the SELL kernels.  map selects how slices are dealt
 // to wavefronts (slice_span).
 template <class V> struct SellArgs {
@@ -417,18 +412,18 @@ __global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(SellArgs<V> M, const
         const uint32_t ii = valid ? i : 0u;
         const V2<V> p = cur[ii];
         const V dn = p.x + beta * p.y;
-        const uint32_t L = valid ? (uint32_t)ldnt(&M.len[i]) : 0u;
+        const uint32_t L = valid ? (uint32_t)M.len[i] : 0u;
         const uint32_t W = M.width[s];
         const uint32_t base = M.off[s] + lane;
-        V acc = ldnt(&M.diag[ii]) * dn;
+        V acc = M.diag[ii] * dn;
         for (uint32_t k = 0; k < W; k += CH) {
             uint32_t c[CH];
             V a[CH];
 #pragma unroll
             for (int u = 0; u < CH; ++u) {
                 const bool ok = k + u < L;
-                c[u] = ok ? ldnt(&M.col[base + (k + u) * kSlice]) : ii;
-                a[u] = ok ? ldnt(&M.val[base + (k + u) * kSlice]) : (V)0;
+                c[u] = ok ? M.col[base + (k + u) * kSlice] : ii;
+                a[u] = ok ? M.val[base + (k + u) * kSlice] : (V)0;
             }
             V2<V> g[CH];
 #pragma unroll
@@ -523,20 +518,20 @@ __global__ __launch_bounds__(kCgThreads) void llt_update_kernel(SellArgs<V> U, X
         const uint32_t i = s * kSlice + lane;
         const bool valid = i < U.n;
         const uint32_t ii = valid ? i : 0u;
-        const uint32_t L = valid ? (uint32_t)ldnt(&U.len[i]) : 0u;
+        const uint32_t L = valid ? (uint32_t)U.len[i] : 0u;
         const uint32_t W = U.width[s];
         const uint32_t base = U.off[s] + lane;
         const V2<V> own = rq[ii];
         const V ri = it ? own.x - av * own.y : own.x;
-        V acc = ldnt(&U.diag[ii]) * ri;
+        V acc = U.diag[ii] * ri;
         for (uint32_t k = 0; k < W; k += CH) {
             uint32_t c[CH];
             V a[CH];
 #pragma unroll
             for (int u = 0; u < CH; ++u) {
                 const bool ok = k + u < L;
-                c[u] = ok ? ldnt(&U.col[base + (k + u) * kSlice]) : ii;
-                a[u] = ok ? ldnt(&U.val[base + (k + u) * kSlice]) : (V)0;
+                c[u] = ok ? U.col[base + (k + u) * kSlice] : ii;
+                a[u] = ok ? U.val[base + (k + u) * kSlice] : (V)0;
             }
             V2<V> g[CH];
 #pragma unroll
@@ -588,18 +583,18 @@ __global__ __launch_bounds__(kCgThreads) void llt_lo_kernel(SellArgs<V> M, const
         const uint32_t i = s * kSlice + lane;
         const bool valid = i < M.n;
         const uint32_t ii = valid ? i : 0u;
-        const uint32_t L = valid ? (uint32_t)ldnt(&M.len[i]) : 0u;
+        const uint32_t L = valid ? (uint32_t)M.len[i] : 0u;
         const uint32_t W = M.width[s];
         const uint32_t base = M.off[s] + lane;
-        V acc = ldnt(&M.diag[ii]) * w[ii];
+        V acc = M.diag[ii] * w[ii];
         for (uint32_t k = 0; k < W; k += CH) {
             uint32_t c[CH];
             V a[CH];
 #pragma unroll
             for (int u = 0; u < CH; ++u) {
                 const bool ok = k + u < L;
-                c[u] = ok ? ldnt(&M.col[base + (k + u) * kSlice]) : ii;
-                a[u] = ok ? ldnt(&M.val[base + (k + u) * kSlice]) : (V)0;
+                c[u] = ok ? M.col[base + (k + u) * kSlice] : ii;
+                a[u] = ok ? M.val[base + (k + u) * kSlice] : (V)0;
             }
 #pragma unroll
             for (int u = 0; u < CH; ++u) acc += a[u] * w[c[u]];
