@@ -359,9 +359,16 @@ def main():
         net.bind_step(step)
         gms, gflops = net.time_gemm(reps=100)
         tf = gflops / (gms * 1e-3) / 1e12
-        mfma = {"bound": "mfma", "kernel": "gemm_rows_kernel<64, 64, 128, 0> (layer-0 [P|Q] = H W_pq, bf16)",
+        # the layer GEMM streams FP32 activations: H [n][64] in, [P|Q] [n][128] out
+        gbytes = n * (64 + 128) * 4
+        gbs = gbytes / (gms * 1e-3) / 1e9
+        mfma = {"kernel": "gemm_rows_kernel<64, 64, 128, 0> (layer-0 [P|Q] = H W_pq, bf16 MFMA, fp32 accumulate)",
                 "achieved": round(tf, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(tf / BF16_PEAK_TFLOPS, 5), "kernel_ms": round(gms, 5), "flops_per_launch": gflops}
+                "frac": round(tf / BF16_PEAK_TFLOPS, 5), "kernel_ms": round(gms, 5), "flops_per_launch": gflops,
+                "arith_intensity": round(gflops / gbytes, 2),
+                "bound": "hbm (arithmetic intensity below the bf16 ridge of 312 FLOP/B)",
+                "bytes_per_launch": int(gbytes), "achieved_gbs": round(gbs, 1),
+                "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
     tank = None
     if args.tank3d:
