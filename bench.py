@@ -87,15 +87,18 @@ def cpu_baseline(pts, al, f, fp64_iters):
     }
 
 
-def sell_dir_bytes(n, nnz, value_bytes):
-    """Algorithmic bytes of one CG direction launch (d = z + beta d_old, q = A d,
-    d.q) on the SELL-64 matrix: per row the length (2 B), the diagonal, the
-    own {z, d_old} pair, d and q written; per off-diagonal entry its column
-    (4 B) and value; per 64-row slice its offset and width (6 B).  Each
-    vector entry is counted once (neighbour re-reads are cache traffic)."""
+def sell_dir_bytes(n, nnz, value_bytes, llt=False):
+    """Algorithmic bytes of one CG direction launch on the SELL-64 matrix.
+    Jacobi path (d = z + beta d_old, q = A d, d.q): per row the length (2 B),
+    the diagonal, the own {z, d_old} pair, d and q written.  L L^T path
+    (q = A d on a ready-made d, d.q): per row the length, the diagonal, d
+    read and q written.  Per off-diagonal entry its column (4 B) and value;
+    per 64-row slice its offset and width (6 B).  Each vector entry is
+    counted once (neighbour re-reads are cache traffic)."""
     vb = value_bytes
     nslices = (n + 63) // 64
-    return n * (2 + vb + 2 * vb + vb + vb) + (nnz - n) * (4 + vb) + nslices * 6
+    row = 2 + vb + vb + vb if llt else 2 + vb + 2 * vb + vb + vb
+    return n * row + (nnz - n) * (4 + vb) + nslices * 6
 
 
 def tank3d_leg(steps=2):
@@ -145,6 +148,7 @@ def components(step, pts, al, net):
         r = step.run(solver, rtol=1e-8)
         out[f"step_{solver}"] = {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items()}
     out["dir_kernel_fp32_ms"] = round(step.time_dir_kernel(fp32=True, reps=200), 5)
+    out["dir_kernel_llt_ms"] = round(step.time_dir_kernel(fp32=True, reps=200, llt=True), 5)
     out["dir_kernel_fp64_ms"] = round(step.time_dir_kernel(fp32=False, reps=200), 5)
     xy = np.ascontiguousarray(pts, np.float32)
     rad = np.full(len(xy), np.float32(0.5) * np.float32(al[0]), np.float32)
@@ -222,17 +226,18 @@ def sequence3d(steps, weights):
 
 
 COPY_BYTES = 256 << 20
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cg_dir_kernel_dam1M_traffic.txt")
+def traffic_file(kernel):
+    return os.path.join(ROOT, "profiles", f"{kernel}_dam1M_traffic.txt")
 
 
-def measured_traffic(n, nnz, fp32):
+def measured_traffic(kernel, n, nnz, fp32):
     """HBM bytes per direction-kernel launch from the committed counter
     summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
     reading of a copy kernel of known size), when it was taken on this
     workload; else None."""
     try:
         kv = {}
-        with open(TRAFFIC_FILE) as fh:
+        with open(traffic_file(kernel)) as fh:
             for line in fh:
                 if "=" in line and not line.startswith("#"):
                     k, v = line.split("=", 1)
@@ -319,7 +324,7 @@ def main():
         import ctypes
 
         r = step.run(args.solver, rtol=1e-8)
-        kms = step.time_dir_kernel(fp32=args.solver != "fp64", reps=20)
+        kms = step.time_dir_kernel(fp32=args.solver != "fp64", reps=20, llt=args.solver in ("learned", "fsai"))
         cms = ctypes.c_double(0)
         stream_amd.check(stream_amd.lib.Stream_copy_calibration(ctypes.c_uint64(COPY_BYTES), 5, ctypes.byref(cms)))
         print(f"[profile] n={pts.shape[0]} nnz={r['nnz']} dir kernel {kms * 1000:.2f} us, "
@@ -340,17 +345,20 @@ def main():
                     solve_ms=inf64[-1]["solve_ms"])
         print(f"[bench] fp64: {ms64:.2f} ms/step iters {fp64['iters']}", file=sys.stderr, flush=True)
 
-    # dominant kernel: the CG direction kernel (d update + SpMV + d.q)
+    # dominant kernel: the CG direction kernel (SpMV + d.q; on the Jacobi
+    # paths also d = z + beta d_old)
     fp32 = args.solver != "fp64"
-    kms = step.time_dir_kernel(fp32=fp32, reps=200)
+    llt = args.solver in ("learned", "fsai")
+    kms = step.time_dir_kernel(fp32=fp32, reps=200, llt=llt)
     n = pts.shape[0]
     nnz = last["nnz"]
-    alg_bytes = sell_dir_bytes(n, nnz, 4 if fp32 else 8)
+    alg_bytes = sell_dir_bytes(n, nnz, 4 if fp32 else 8, llt=llt)
     achieved = alg_bytes / (kms * 1e-3) / 1e9
-    traffic = measured_traffic(n, nnz, fp32)
+    kname = "llt_dir_kernel" if llt else "cg_dir_kernel"
+    traffic = measured_traffic(kname, n, nnz, fp32)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": f"cg_dir_kernel<{'float, true' if fp32 else 'double, false'}>", "kernel_ms": round(kms, 5),
+                "kernel": f"{kname}<{'float, true' if fp32 else 'double, false'}>", "kernel_ms": round(kms, 5),
                 "alg_bytes_per_launch": int(alg_bytes)}
 
     mfma = None

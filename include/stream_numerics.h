@@ -57,7 +57,7 @@ int32_t SolverCG_solve_mixed(SolverCG* solver, const LinSys* sys, const PrecJaco
                              uint32_t maxit, double* stats);
 
 /* Process-wide solver knobs: "replace_factor" (mixed precision residual
- * replacement threshold, default 0.3), "batch" (iterations between host
+ * replacement threshold, default 0.7), "batch" (iterations between host
  * checks, default 32), "cg_blocks" (grid cap of the CG kernels, 64..4096,
  * default 2048), "graph" (1: replay each batch as a captured HIP graph,
  * default 0), "xcd_map" (SELL slices in XCD-contiguous chunks, default 1)
@@ -116,7 +116,10 @@ int32_t PfemStep2D_get_solution_morton(PfemStep2D* step, double* p);
 uint32_t PfemStep2D_nnz(PfemStep2D* step);
 int32_t PfemStep2D_get_system(PfemStep2D* step, uint32_t* row, uint32_t* col, double* val, double* b);
 int32_t PfemStep2D_get_mesh(PfemStep2D* step, uint32_t* elem, uint8_t* node_is_bnd, double* xy, uint32_t* perm);
-int32_t PfemStep2D_time_dir_kernel(PfemStep2D* step, int32_t fp32, uint32_t reps, double* ms);
+/* Average ms of a CG direction kernel on the last step's system (HIP
+ * events on the library stream, `reps` launches): kind 0 = FP64 Jacobi
+ * path, 1 = FP32 row-sum Jacobi path, 2 = FP32 row-sum L L^T path. */
+int32_t PfemStep2D_time_dir_kernel(PfemStep2D* step, int32_t kind, uint32_t reps, double* ms);
 
 #ifdef __cplusplus
 }

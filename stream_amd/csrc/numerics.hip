@@ -338,7 +338,8 @@ __device__ __forceinline__ SliceSpan slice_span(uint32_t nslices, uint32_t map) 
 
 // ---------------------------------------------------------------------------
 // init / replacement: r = b - A64 x64 (FP64), stored in precision V;
-// Jacobi z into the current {z, d} buffer; partial r.r, r.z, b.b.
+// Jacobi / no preconditioner: z into the current {z, d} buffer (P is not
+// touched on the L L^T path); partial r.r, r.z, b.b.
 // mode 0: init (d = 0, beta = 0, iter = 0)
 // mode 1: replacement of the recursive residual by the true one.  The
 //         direction d and beta (computed by the recursion) are kept and only
@@ -357,16 +358,14 @@ __global__ __launch_bounds__(kCgThreads) void cg_residual_kernel(
         const double ri = b[i] - row_dot(row, col, val, x, i);
         const V rv = (V)ri;
         r[(size_t)i * rs] = rv;
-        V2<V> p = P[i];
-        if (mode == 0) p.y = (V)0;
         if constexpr (PREC != kPrecLLt) {
+            V2<V> p = P[i];
+            if (mode == 0) p.y = (V)0;
             const V zv = (PREC == kPrecJacobi) ? (V)(idiag[i] * rv) : rv;
             p.x = zv;
+            P[i] = p;
             v[1] += (double)rv * (double)zv;
-        } else {
-            if (mode == 0) p.x = (V)0;
         }
-        P[i] = p;
         v[0] += (double)rv * (double)rv;
         v[2] += b[i] * b[i];
     }
@@ -451,6 +450,59 @@ __global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(SellArgs<V> M, const
     }
 }
 
+// Direction kernel of the L L^T path: d is already formed (by the lower
+// apply of the previous iteration, which knows beta), so the row loop
+// gathers 4-byte d_j instead of {z, d} pairs and writes nothing but q.
+// q = A d (row-sum form when DIFF), q into the {r, q} pair, partial d.q.
+template <class V, bool DIFF, int CH>
+__global__ __launch_bounds__(kCgThreads) void llt_dir_kernel(SellArgs<V> M, const V* __restrict__ d,
+                                                             V* __restrict__ q, uint32_t qs, CgScalars* sc,
+                                                             double* partials, uint32_t* counter) {
+    if (sc->done) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    double v[1] = {0.0};
+    SELL_FOR_SLICES(s) {
+        const uint32_t i = s * kSlice + lane;
+        const bool valid = i < M.n;
+        const uint32_t ii = valid ? i : 0u;
+        const V dn = d[ii];
+        const uint32_t L = valid ? (uint32_t)M.len[i] : 0u;
+        const uint32_t W = M.width[s];
+        const uint32_t base = M.off[s] + lane;
+        V acc = M.diag[ii] * dn;
+        for (uint32_t k = 0; k < W; k += CH) {
+            uint32_t c[CH];
+            V a[CH];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const bool ok = k + u < L;
+                c[u] = ok ? M.col[base + (k + u) * kSlice] : ii;
+                a[u] = ok ? M.val[base + (k + u) * kSlice] : (V)0;
+            }
+            V g[CH];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) g[u] = d[c[u]];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                if constexpr (DIFF)
+                    acc += a[u] * (g[u] - dn);
+                else
+                    acc += a[u] * g[u];
+            }
+        }
+        if (valid) {
+            q[(size_t)i * qs] = acc;
+            v[0] += (double)dn * (double)acc;
+        }
+    }
+    double tot[1];
+    if (!grid_fold<1>(v, partials, counter, tot)) return;
+    if (threadIdx.x == 0) {
+        sc->dq = tot[0];
+        sc->alpha = sc->rz / tot[0];
+    }
+}
+
 // x += alpha d; r -= alpha q; z = M r (Jacobi / none) into nxt.x;
 // partial r.z, r.r
 template <class V, class XV, int PREC>
@@ -502,7 +554,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_update_kernel(uint32_t n, XV* _
 // mode 2 keeps beta as in cg_residual_kernel).
 template <class V, class XV, int CH>
 __global__ __launch_bounds__(kCgThreads) void llt_update_kernel(SellArgs<V> U, XV* __restrict__ x,
-                                                                const V2<V>* __restrict__ pd,
+                                                                const V* __restrict__ d,
                                                                 const V2<V>* __restrict__ rq,
                                                                 V2<V>* __restrict__ rq_nxt, V* __restrict__ w,
                                                                 CgScalars* sc, double* partials, uint32_t* counter,
@@ -541,7 +593,7 @@ __global__ __launch_bounds__(kCgThreads) void llt_update_kernel(SellArgs<V> U, X
         }
         if (valid) {
             if (it) {
-                x[i] += (XV)(alpha * (double)pd[i].y);
+                x[i] += (XV)(alpha * (double)d[i]);
                 rq_nxt[i].x = ri;
             }
             w[i] = acc;
@@ -571,13 +623,20 @@ __global__ __launch_bounds__(kCgThreads) void llt_update_kernel(SellArgs<V> U, X
     }
 }
 
-// Second half of the apply: z = L w (z_i = L_ii w_i + sum_{j<i} L_ij w_j)
-// into P[i].x.  force: run even when the solve is flagged done (init and
-// replacement passes).
+// Second half of the apply, z = L w (z_i = L_ii w_i + sum_{j<i} L_ij w_j),
+// and the next direction d_new = z + beta d_old: beta = (r'.z)/(r.z) was
+// set by the update kernel's fold, so the direction kernel that follows
+// reads d ready-made and z is never stored.
+// mode 0: iteration (skipped when the solve is flagged done);
+// mode 1: init (d_new = z, d_old not read);
+// mode 2: after a residual replacement (run even when done; beta is the
+//         recursion's, as in cg_residual_kernel).
 template <class V, int CH>
 __global__ __launch_bounds__(kCgThreads) void llt_lo_kernel(SellArgs<V> M, const V* __restrict__ w,
-                                                            V2<V>* __restrict__ P, const CgScalars* sc, int force) {
-    if (!force && sc->done) return;
+                                                            const V* __restrict__ d_old, V* __restrict__ d_new,
+                                                            const CgScalars* sc, int mode) {
+    if (mode == 0 && sc->done) return;
+    const V beta = mode == 1 ? (V)0 : (V)sc->beta;
     const uint32_t lane = threadIdx.x & 63u;
     SELL_FOR_SLICES(s) {
         const uint32_t i = s * kSlice + lane;
@@ -599,7 +658,7 @@ __global__ __launch_bounds__(kCgThreads) void llt_lo_kernel(SellArgs<V> M, const
 #pragma unroll
             for (int u = 0; u < CH; ++u) acc += a[u] * w[c[u]];
         }
-        if (valid) P[i].x = acc;
+        if (valid) d_new[i] = mode == 1 ? acc : acc + beta * d_old[i];
     }
 }
 
@@ -657,8 +716,9 @@ void clear_done(CgWork& w, hipStream_t s) {
 
 // Vectors of one solve, carved from the grow-only workspace.
 template <class V> struct CgVecs {
-    V2<V>* P[2];   // {z, d} ping-pong
+    V2<V>* P[2];   // {z, d} ping-pong (Jacobi / no preconditioner)
     V2<V>* RQ[2];  // {r, q} ping-pong (L L^T path)
+    V* D[2];       // d ping-pong (L L^T path)
     V* r;
     V* q;
     V* w;
@@ -666,7 +726,7 @@ template <class V> struct CgVecs {
 };
 template <class V> CgVecs<V> carve(CgWork& w, uint32_t n) {
     const size_t nv = ((size_t)n + 63) & ~(size_t)63;
-    w.vbuf.resize(nv * (4 * sizeof(V2<V>) + 3 * sizeof(V) + sizeof(float)) + 256);
+    w.vbuf.resize(nv * (4 * sizeof(V2<V>) + 5 * sizeof(V) + sizeof(float)) + 256);
     uint8_t* p = w.vbuf.p;
     CgVecs<V> v;
     v.P[0] = reinterpret_cast<V2<V>*>(p);
@@ -677,6 +737,10 @@ template <class V> CgVecs<V> carve(CgWork& w, uint32_t n) {
     p += nv * sizeof(V2<V>);
     v.RQ[1] = reinterpret_cast<V2<V>*>(p);
     p += nv * sizeof(V2<V>);
+    v.D[0] = reinterpret_cast<V*>(p);
+    p += nv * sizeof(V);
+    v.D[1] = reinterpret_cast<V*>(p);
+    p += nv * sizeof(V);
     v.r = reinterpret_cast<V*>(p);
     p += nv * sizeof(V);
     v.q = reinterpret_cast<V*>(p);
@@ -710,6 +774,7 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
     const SellArgs<V> MA = sell_args(S);
     SellArgs<V> ML{}, MU{};
     if constexpr (PREC == kPrecLLt) {
+        fill<V>(vv.D[1], n, (V)0, s);  // d_{-1}: defined values even where beta = 0
         ML = sell_args(L->lo);
         MU = sell_args(L->up);
         ML.diag = L->diag.p;
@@ -722,6 +787,10 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
 
     publish_options(w, o, s);
     const double rtol2 = o.abs_tol2 >= 0.0 ? -o.abs_tol2 : o.rtol * o.rtol;
+    // L L^T buffers: iteration `it` reads d from D[it & 1] and r from
+    // RQ[it & 1], and leaves d and r for it + 1 in the other halves; a
+    // replacement after iteration it - 1 rebuilds D[it & 1] from the
+    // direction D[(it + 1) & 1] it was made from.
     auto residual = [&](int mode, uint32_t it) {
         V2<V>* cur = vv.P[it & 1];
         if constexpr (PREC == kPrecLLt) {
@@ -729,9 +798,10 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
             hipLaunchKernelGGL((cg_residual_kernel<V, PREC>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p,
                                b, x, reinterpret_cast<V*>(rqc), 2u, cur, idiag, w.sc.p, w.partials.p, w.counter.p,
                                mode, rtol2);
-            hipLaunchKernelGGL((llt_update_kernel<V, XV, CH>), grid, blk, 0, s, MU, xacc, cur, rqc, rqc, vv.w, w.sc.p,
-                               w.partials.p, w.counter.p, mode == 0 ? 1 : 2);
-            hipLaunchKernelGGL((llt_lo_kernel<V, CH>), grid, blk, 0, s, ML, vv.w, cur, w.sc.p, 1);
+            hipLaunchKernelGGL((llt_update_kernel<V, XV, CH>), grid, blk, 0, s, MU, xacc, vv.D[it & 1], rqc, rqc,
+                               vv.w, w.sc.p, w.partials.p, w.counter.p, mode == 0 ? 1 : 2);
+            hipLaunchKernelGGL((llt_lo_kernel<V, CH>), grid, blk, 0, s, ML, vv.w, vv.D[(it + 1) & 1], vv.D[it & 1],
+                               w.sc.p, mode == 0 ? 1 : 2);
         } else {
             hipLaunchKernelGGL((cg_residual_kernel<V, PREC>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p,
                                b, x, vv.r, 1u, cur, idiag, w.sc.p, w.partials.p, w.counter.p, mode, rtol2);
@@ -750,11 +820,13 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
             if constexpr (PREC == kPrecLLt) {
                 V2<V>* rqc = vv.RQ[it & 1];
                 V2<V>* rqn = vv.RQ[(it + 1) & 1];
-                hipLaunchKernelGGL((cg_dir_kernel<V, DIFF, CH>), grid, blk, 0, s, MA, cur, nxt,
+                V* dc = vv.D[it & 1];
+                hipLaunchKernelGGL((llt_dir_kernel<V, DIFF, CH>), grid, blk, 0, s, MA, dc,
                                    reinterpret_cast<V*>(rqc) + 1, 2u, w.sc.p, w.partials.p, w.counter.p);
-                hipLaunchKernelGGL((llt_update_kernel<V, XV, CH>), grid, blk, 0, s, MU, xacc, nxt, rqc, rqn, vv.w,
+                hipLaunchKernelGGL((llt_update_kernel<V, XV, CH>), grid, blk, 0, s, MU, xacc, dc, rqc, rqn, vv.w,
                                    w.sc.p, w.partials.p, w.counter.p, 0);
-                hipLaunchKernelGGL((llt_lo_kernel<V, CH>), grid, blk, 0, s, ML, vv.w, nxt, w.sc.p, 0);
+                hipLaunchKernelGGL((llt_lo_kernel<V, CH>), grid, blk, 0, s, ML, vv.w, dc, vv.D[(it + 1) & 1], w.sc.p,
+                                   0);
             } else {
                 hipLaunchKernelGGL((cg_dir_kernel<V, DIFF, CH>), grid, blk, 0, s, MA, cur, nxt, vv.q, 1u, w.sc.p,
                                    w.partials.p, w.counter.p);
@@ -883,10 +955,13 @@ template void cg_solve<double>(const DevCsr<double>&, const Sell<double>&, bool,
                                const LowerFactor<double>*, const double*, double*, const CgOptions&, CgWork&,
                                SolveStats&, hipStream_t);
 
-// Average duration (ms, HIP events on the context stream) of the CG
-// direction kernel over `reps` back-to-back launches on S.  The vectors are
-// scratch (filled with ones).
-template <class V> double time_dir_kernel(const Sell<V>& S, bool diff, uint32_t reps, CgWork& w, hipStream_t s) {
+// Average duration (ms, HIP events on the context stream) of a CG
+// direction kernel over `reps` back-to-back launches on S: llt = false, the
+// {z, d}-pair kernel of the Jacobi path (row-sum form when diff); llt = true,
+// the L L^T path's kernel on a ready-made d.  The vectors are scratch (filled
+// with ones).
+template <class V>
+double time_dir_kernel(const Sell<V>& S, bool diff, bool llt, uint32_t reps, CgWork& w, hipStream_t s) {
     const uint32_t n = S.n;
     w.resize(n);
     CgVecs<V> vv = carve<V>(w, n);
@@ -895,14 +970,27 @@ template <class V> double time_dir_kernel(const Sell<V>& S, bool diff, uint32_t 
     h.rz = 1.0;
     SA_HIP(hipMemcpyAsync(w.sc.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
     const size_t nv = ((size_t)n + 63) & ~(size_t)63;
-    fill<V>(reinterpret_cast<V*>(vv.P[0]), 4 * nv, (V)1, s);
+    fill<V>(reinterpret_cast<V*>(vv.P[0]), 10 * nv, (V)1, s);  // P, RQ and D
     const dim3 grid(w.blocks), blk(kCgThreads);
     const SellArgs<V> M = sell_args(S);
+    const bool c4 = tuning().sell_chunk == 4;
     auto launch = [&](uint32_t k) {
         V2<V>* cur = vv.P[k & 1];
         V2<V>* nxt = vv.P[(k + 1) & 1];
-        const bool c4 = tuning().sell_chunk == 4;
-        if (diff && c4)
+        V* q = reinterpret_cast<V*>(vv.RQ[k & 1]) + 1;
+        if (llt && diff && c4)
+            hipLaunchKernelGGL((llt_dir_kernel<V, true, 4>), grid, blk, 0, s, M, vv.D[k & 1], q, 2u, w.sc.p,
+                               w.partials.p, w.counter.p);
+        else if (llt && diff)
+            hipLaunchKernelGGL((llt_dir_kernel<V, true, 8>), grid, blk, 0, s, M, vv.D[k & 1], q, 2u, w.sc.p,
+                               w.partials.p, w.counter.p);
+        else if (llt && c4)
+            hipLaunchKernelGGL((llt_dir_kernel<V, false, 4>), grid, blk, 0, s, M, vv.D[k & 1], q, 2u, w.sc.p,
+                               w.partials.p, w.counter.p);
+        else if (llt)
+            hipLaunchKernelGGL((llt_dir_kernel<V, false, 8>), grid, blk, 0, s, M, vv.D[k & 1], q, 2u, w.sc.p,
+                               w.partials.p, w.counter.p);
+        else if (diff && c4)
             hipLaunchKernelGGL((cg_dir_kernel<V, true, 4>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
                                w.partials.p, w.counter.p);
         else if (diff)
@@ -929,7 +1017,7 @@ template <class V> double time_dir_kernel(const Sell<V>& S, bool diff, uint32_t 
     SA_HIP(hipEventDestroy(e1));
     return reps ? (double)ms / reps : 0.0;
 }
-template double time_dir_kernel<float>(const Sell<float>&, bool, uint32_t, CgWork&, hipStream_t);
-template double time_dir_kernel<double>(const Sell<double>&, bool, uint32_t, CgWork&, hipStream_t);
+template double time_dir_kernel<float>(const Sell<float>&, bool, bool, uint32_t, CgWork&, hipStream_t);
+template double time_dir_kernel<double>(const Sell<double>&, bool, bool, uint32_t, CgWork&, hipStream_t);
 
 }  // namespace stream_amd

@@ -127,7 +127,7 @@ struct CgOptions {
 
 // Process-wide solver knobs (set through Stream_tune).
 struct Tuning {
-    double replace_factor = 3e-1;  // mixed: residual replacement when ||r|| fell by this factor
+    double replace_factor = 7e-1;  // mixed: residual replacement when ||r|| fell by this factor
     uint32_t batch = 32;           // CG iterations between host checks
     uint32_t cg_blocks = 2048;     // grid cap of the CG kernels (64..4096)
     uint32_t graph = 0;            // 1: replay each batch of iterations as a captured HIP graph
@@ -151,7 +151,9 @@ void cg_solve(const DevCsr<double>& A64, const Sell<V>& S, bool diff, int32_t pr
 // ||b - A x||_2^2 and ||b||_2^2 in FP64.
 void true_residual(const DevCsr<double>& A, const double* b, const double* x, double* out2, CgWork& w, hipStream_t s);
 
-// Average duration (ms) of the CG direction kernel over `reps` launches.
-template <class V> double time_dir_kernel(const Sell<V>& S, bool diff, uint32_t reps, CgWork& w, hipStream_t s);
+// Average duration (ms) of a CG direction kernel over `reps` launches (llt:
+// the L L^T path's kernel, else the Jacobi path's).
+template <class V>
+double time_dir_kernel(const Sell<V>& S, bool diff, bool llt, uint32_t reps, CgWork& w, hipStream_t s);
 
 }  // namespace stream_amd

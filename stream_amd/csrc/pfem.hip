@@ -904,18 +904,20 @@ int32_t PfemStep2D_run(PfemStep2D* h, int32_t solver, double rtol, uint32_t maxi
     });
 }
 
-// Average duration in ms of the CG direction kernel (d update + SpMV +
-// d.q), FP64 (fp32 = 0) or FP32 (fp32 = 1) values, on the system of the
-// last step; timed with HIP events on the library stream over `reps`
-// launches.
-int32_t PfemStep2D_time_dir_kernel(PfemStep2D* h, int32_t fp32, uint32_t reps, double* ms) {
+// Average duration in ms of a CG direction kernel on the system of the last
+// step, timed with HIP events on the library stream over `reps` launches:
+// kind 0 = FP64 Jacobi path (d = z + beta d_old, q = A d, d.q), 1 = FP32
+// row-sum Jacobi path, 2 = FP32 row-sum L L^T path (q = A d on a
+// ready-made d, d.q).
+int32_t PfemStep2D_time_dir_kernel(PfemStep2D* h, int32_t kind, uint32_t reps, double* ms) {
     return g([&] {
+        if (kind < 0 || kind > 2) throw std::invalid_argument("time_dir_kernel: kind must be 0, 1 or 2");
         CgWork w;
-        h->s.prepare(fp32 ? 1 : 0, ctx().stream);
-        if (fp32)
-            *ms = time_dir_kernel<float>(h->s.S32, true, reps, w, ctx().stream);
+        h->s.prepare(kind ? 1 : 0, ctx().stream);
+        if (kind)
+            *ms = time_dir_kernel<float>(h->s.S32, true, kind == 2, reps, w, ctx().stream);
         else
-            *ms = time_dir_kernel<double>(h->s.S64, false, reps, w, ctx().stream);
+            *ms = time_dir_kernel<double>(h->s.S64, false, false, reps, w, ctx().stream);
     });
 }
 

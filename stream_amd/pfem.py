@@ -104,10 +104,14 @@ class PfemStep2D:
     def nnz(self):
         return int(PfemStep2D_nnz(self.h))
 
-    def time_dir_kernel(self, fp32: bool, reps: int = 200) -> float:
-        """Average ms of the CG direction kernel on the current system."""
+    def time_dir_kernel(self, fp32: bool, reps: int = 200, llt: bool = False) -> float:
+        """Average ms of a CG direction kernel on the current system: FP64
+        Jacobi path, FP32 row-sum Jacobi path, or (llt) the FP32 L L^T path's
+        kernel on a ready-made direction."""
+        if llt and not fp32:
+            raise ValueError("the L L^T direction kernel is FP32")
         ms = ctypes.c_double(0.0)
-        check(PfemStep2D_time_dir_kernel(self.h, int(fp32), reps, ctypes.byref(ms)))
+        check(PfemStep2D_time_dir_kernel(self.h, 2 if llt else int(fp32), reps, ctypes.byref(ms)))
         return ms.value
 
     def solution(self, morton=False):

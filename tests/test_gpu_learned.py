@@ -109,6 +109,31 @@ def test_zero_head_is_jacobi(small):
     assert abs(loss) < 1e-6
 
 
+def test_zero_head_learned_pcg_is_jacobi_pcg():
+    """With zero output weights the learned factor is L = D^-1/2, so the
+    L L^T path (direction on a ready-made d, update + L^T apply, L apply +
+    next direction, residual replacements) runs Jacobi PCG through its own
+    kernels: the Jacobi path's iteration count up to FP32 rounding (z_i =
+    L_ii (L_ii r_i) against (1/A_ii) r_i) and the same solution."""
+    from stream_amd.learned import PrecLearned
+    from stream_amd.pfem import PfemStep2D
+
+    pts, al = clouds.dam_break_scaled(60_000)
+    f = clouds.divergence_source(pts, clouds.fourier_modes(11))
+    st = PfemStep2D(pts, al, f)
+    st.set_learned(PrecLearned())
+    rl = st.run("learned", rtol=1e-8)
+    xl = st.solution(morton=True)
+    rj = st.run("mixed", rtol=1e-8)
+    xj = st.solution(morton=True)
+    print(f"zero-head learned: {rl['iters']} iterations ({rl['replacements']} replacements), "
+          f"Jacobi path {rj['iters']} ({rj['replacements']})")
+    assert not rl["fallback"]
+    assert rl["true_rel"] <= 1e-8 and rj["true_rel"] <= 1e-8
+    assert abs(rl["iters"] - rj["iters"]) <= max(3, rj["iters"] // 50)
+    assert np.linalg.norm(xl - xj) / np.linalg.norm(xj) < 1e-5
+
+
 def test_gradient_matches_oracle(small):
     from stream_amd.learned import PrecLearned
 
