@@ -92,13 +92,15 @@ def sell_dir_bytes(n, nnz, value_bytes, llt=False):
     Jacobi path (d = z + beta d_old, q = A d, d.q): per row the length (2 B),
     the diagonal, the own {z, d_old} pair, d and q written.  L L^T path
     (q = A d on a ready-made d, d.q): per row the length, the diagonal, d
-    read and q written.  Per off-diagonal entry its column (4 B) and value;
-    per 64-row slice its offset and width (6 B).  Each vector entry is
-    counted once (neighbour re-reads are cache traffic)."""
+    read and q written.  Per off-diagonal entry its 16-bit column offset
+    and value (the 4-byte columns of the entries whose offset does not fit,
+    one in a few hundred, are not counted); per 64-row slice its offset and
+    width (6 B).  Each vector entry is counted once (neighbour re-reads are
+    cache traffic)."""
     vb = value_bytes
     nslices = (n + 63) // 64
     row = 2 + vb + vb + vb if llt else 2 + vb + 2 * vb + vb + vb
-    return n * row + (nnz - n) * (4 + vb) + nslices * 6
+    return n * row + (nnz - n) * (2 + vb) + nslices * 6
 
 
 def tank3d_leg(steps=2):

@@ -229,8 +229,8 @@ __global__ __launch_bounds__(256) void sell_fill_kernel(uint32_t n, const uint32
                                                         const float* __restrict__ val32, int form, int part,
                                                         const uint32_t* __restrict__ off,
                                                         const uint16_t* __restrict__ width,
-                                                        uint32_t* __restrict__ scol, V* __restrict__ sval,
-                                                        V* __restrict__ diag) {
+                                                        uint32_t* __restrict__ scol, int16_t* __restrict__ scol16,
+                                                        V* __restrict__ sval, V* __restrict__ diag) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = i / kSlice, lane = i % kSlice;
@@ -243,13 +243,16 @@ __global__ __launch_bounds__(256) void sell_fill_kernel(uint32_t n, const uint32
         rowsum += v;
         if (c == i) dv = v;
         if (in_part(c, i, part)) {
+            const int32_t o = (int32_t)c - (int32_t)i;
             scol[base + k * kSlice] = c;
+            scol16[base + k * kSlice] = (o >= -32767 && o <= 32767) ? (int16_t)o : kColEscape;
             sval[base + k * kSlice] = (V)v;
             ++k;
         }
     }
     for (; k < width[s]; ++k) {
         scol[base + k * kSlice] = i;
+        scol16[base + k * kSlice] = 0;
         sval[base + k * kSlice] = (V)0;
     }
     diag[i] = (V)(form == kSellRowSum ? rowsum : dv);
@@ -278,9 +281,10 @@ void build_sell(uint32_t n, const uint32_t* row, const uint32_t* col, const doub
     exclusive_scan_u32(S.tmp.p, S.off.p, S.nslices, S.scan_ws.p, s);
     S.entries = read_scalar(S.off.p + S.nslices);
     S.col.resize(S.entries ? S.entries : 1);
+    S.col16.resize(S.entries ? S.entries : 1);
     S.val.resize(S.entries ? S.entries : 1);
     hipLaunchKernelGGL((sell_fill_kernel<V>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, row, col, val64, val32,
-                       (int)form, (int)part, S.off.p, S.width.p, S.col.p, S.val.p, S.diag.p);
+                       (int)form, (int)part, S.off.p, S.width.p, S.col.p, S.col16.p, S.val.p, S.diag.p);
 }
 template void build_sell<float>(uint32_t, const uint32_t*, const uint32_t*, const double*, const float*, SellForm,
                                 SellPart, Sell<float>&, hipStream_t);
@@ -297,12 +301,33 @@ template <class V> struct SellArgs {
     const uint16_t* width;
     const uint16_t* len;
     const uint32_t* col;
+    const int16_t* col16;
     const V* val;
     const V* diag;
     uint32_t map;
 };
 template <class V> SellArgs<V> sell_args(const Sell<V>& S) {
-    return {S.n, S.nslices, S.off.p, S.width.p, S.len.p, S.col.p, S.val.p, S.diag.p, tuning().xcd_map};
+    return {S.n, S.nslices, S.off.p, S.width.p, S.len.p, S.col.p, S.col16.p, S.val.p, S.diag.p, tuning().xcd_map};
+}
+
+// Entries k .. k + CH - 1 of a row (slot base + e * kSlice, row length L,
+// row ii): values and columns, the padding as column ii with value 0.  The
+// 16-bit offsets and the values are all loaded before any escaped column
+// (one entry in a few hundred on Morton-ordered meshes) is fetched from the
+// 32-bit copy.
+template <class V, int CH>
+__device__ __forceinline__ void sell_load(const SellArgs<V>& M, uint32_t base, uint32_t k, uint32_t L, uint32_t ii,
+                                          uint32_t (&c)[CH], V (&a)[CH]) {
+    int16_t o[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+        const bool ok = k + u < L;
+        o[u] = ok ? M.col16[base + (k + u) * kSlice] : (int16_t)0;
+        a[u] = ok ? M.val[base + (k + u) * kSlice] : (V)0;
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u)
+        c[u] = o[u] == kColEscape ? M.col[base + (k + u) * kSlice] : (uint32_t)((int32_t)ii + (int32_t)o[u]);
 }
 
 // Slices of one wavefront: [s, e) with stride step.
@@ -418,12 +443,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(SellArgs<V> M, const
         for (uint32_t k = 0; k < W; k += CH) {
             uint32_t c[CH];
             V a[CH];
-#pragma unroll
-            for (int u = 0; u < CH; ++u) {
-                const bool ok = k + u < L;
-                c[u] = ok ? M.col[base + (k + u) * kSlice] : ii;
-                a[u] = ok ? M.val[base + (k + u) * kSlice] : (V)0;
-            }
+            sell_load<V, CH>(M, base, k, L, ii, c, a);
             V2<V> g[CH];
 #pragma unroll
             for (int u = 0; u < CH; ++u) g[u] = cur[c[u]];
@@ -473,12 +493,7 @@ __global__ __launch_bounds__(kCgThreads) void llt_dir_kernel(SellArgs<V> M, cons
         for (uint32_t k = 0; k < W; k += CH) {
             uint32_t c[CH];
             V a[CH];
-#pragma unroll
-            for (int u = 0; u < CH; ++u) {
-                const bool ok = k + u < L;
-                c[u] = ok ? M.col[base + (k + u) * kSlice] : ii;
-                a[u] = ok ? M.val[base + (k + u) * kSlice] : (V)0;
-            }
+            sell_load<V, CH>(M, base, k, L, ii, c, a);
             V g[CH];
 #pragma unroll
             for (int u = 0; u < CH; ++u) g[u] = d[c[u]];
@@ -579,12 +594,7 @@ __global__ __launch_bounds__(kCgThreads) void llt_update_kernel(SellArgs<V> U, X
         for (uint32_t k = 0; k < W; k += CH) {
             uint32_t c[CH];
             V a[CH];
-#pragma unroll
-            for (int u = 0; u < CH; ++u) {
-                const bool ok = k + u < L;
-                c[u] = ok ? U.col[base + (k + u) * kSlice] : ii;
-                a[u] = ok ? U.val[base + (k + u) * kSlice] : (V)0;
-            }
+            sell_load<V, CH>(U, base, k, L, ii, c, a);
             V2<V> g[CH];
 #pragma unroll
             for (int u = 0; u < CH; ++u) g[u] = rq[c[u]];
@@ -649,12 +659,7 @@ __global__ __launch_bounds__(kCgThreads) void llt_lo_kernel(SellArgs<V> M, const
         for (uint32_t k = 0; k < W; k += CH) {
             uint32_t c[CH];
             V a[CH];
-#pragma unroll
-            for (int u = 0; u < CH; ++u) {
-                const bool ok = k + u < L;
-                c[u] = ok ? M.col[base + (k + u) * kSlice] : ii;
-                a[u] = ok ? M.val[base + (k + u) * kSlice] : (V)0;
-            }
+            sell_load<V, CH>(M, base, k, L, ii, c, a);
 #pragma unroll
             for (int u = 0; u < CH; ++u) acc += a[u] * w[c[u]];
         }

@@ -39,6 +39,10 @@ template <class V> struct DevCsr {
 
 constexpr uint32_t kSlice = 64;
 
+// SELL column offsets: entries whose column is more than 32767 rows away
+// from their row hold kColEscape in col16 and are read from col.
+constexpr int16_t kColEscape = -32768;
+
 template <class V> struct Sell {
     uint32_t n = 0;
     uint32_t nslices = 0;
@@ -47,6 +51,7 @@ template <class V> struct Sell {
     DBuf<uint16_t> width;    // [nslices]
     DBuf<uint16_t> len;      // [n]
     DBuf<uint32_t> col;      // [entries]
+    DBuf<int16_t> col16;     // [entries] column - row, or kColEscape where that does not fit
     DBuf<V> val;             // [entries]
     DBuf<V> diag;            // [n]
     DBuf<uint32_t> tmp;      // build scratch
