@@ -62,8 +62,9 @@ int32_t SolverCG_solve_mixed(SolverCG* solver, const LinSys* sys, const PrecJaco
  * default 2048), "graph" (1: replay each batch as a captured HIP graph,
  * default 0), "xcd_map" (SELL slices in XCD-contiguous chunks, default 1)
  * "fold_group" (blocks per group of the two-level reduction, 0 = one
- * level; 8, 16, 32 (default) or 64) and "sell_chunk" (SELL entries per load
- * step of the CG row loops, 4 or 8 (default)). */
+ * level; 8, 16, 32 (default) or 64), "sell_chunk" (SELL entries per load
+ * step of the CG row loops, 4 or 8 (default)) and "sell_rows" (SELL slices a
+ * wavefront works on at once, 1 or 2). */
 int32_t Stream_tune(const char* key, double value);
 
 /* Mesh-and-solve step on device-resident points: LBVH + 2D alpha-shape +

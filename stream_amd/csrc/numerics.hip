@@ -355,11 +355,66 @@ __device__ __forceinline__ SliceSpan slice_span(uint32_t nslices, uint32_t map) 
     return {s0, min(s0 + chunk, nslices), 1u};
 }
 
-// Loop over the slices of this wavefront (the kernel's matrix is M).
-#define SELL_FOR_SLICES(s)                                                                        \
-    for (SliceSpan _span = slice_span(M.nslices, M.map), _once = {0, 1, 1}; _once.s < _once.e; \
-         ++_once.s)                                                                               \
-        for (uint32_t s = _span.s; s < _span.e; s += _span.step)
+// R slices of one wavefront processed together (s, s + step, ...): this
+// lane's row in each, its length, SELL slot base, and the widest slice.
+// Working on R slices at once puts R rows' loads in flight before any of
+// their gathers, so the chain of dependent loads (slice offset -> columns ->
+// gathers) is paid once per R slices.
+template <int R> struct SliceRows {
+    uint32_t i[R], ii[R], L[R], base[R];
+    bool valid[R];
+    uint32_t W;
+};
+template <class V, int R>
+__device__ __forceinline__ SliceRows<R> slice_rows(const SellArgs<V>& M, uint32_t s, const SliceSpan& sp,
+                                                   uint32_t lane) {
+    SliceRows<R> q;
+    q.W = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t sr = s + r * sp.step;
+        const bool has = sr < sp.e;
+        q.i[r] = sr * kSlice + lane;
+        q.valid[r] = has && q.i[r] < M.n;
+        q.ii[r] = q.valid[r] ? q.i[r] : 0u;
+        q.L[r] = q.valid[r] ? (uint32_t)M.len[q.i[r]] : 0u;
+        q.base[r] = has ? M.off[sr] + lane : 0u;
+        q.W = max(q.W, has ? (uint32_t)M.width[sr] : 0u);
+    }
+    return q;
+}
+
+// acc[r] = term(r, acc[r], a_k, fetch(c_k)) over the entries k of this
+// lane's R rows in ascending order (the padding past a row's length as
+// column ii, value 0).  Per step the CH entries of all R rows are loaded,
+// then all their gathers are issued, then the terms are added.
+template <class V, int R, int CH, class G, class Fetch, class Term>
+__device__ __forceinline__ void sell_rows_product(const SellArgs<V>& M, const SliceRows<R>& q, V (&acc)[R],
+                                                  Fetch fetch, Term term) {
+    for (uint32_t k = 0; k < q.W; k += CH) {
+        uint32_t c[R][CH];
+        V a[R][CH];
+#pragma unroll
+        for (int r = 0; r < R; ++r) sell_load<V, CH>(M, q.base[r], k, q.L[r], q.ii[r], c[r], a[r]);
+        G g[R][CH];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int u = 0; u < CH; ++u) g[r][u] = fetch(c[r][u]);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int u = 0; u < CH; ++u) acc[r] = term(r, acc[r], a[r][u], g[r][u]);
+    }
+}
+
+// Loop over the slices of this wavefront, R at a time (the kernel's matrix
+// is M; q is the SliceRows of the current group).
+#define SELL_FOR_SLICE_ROWS(R, q)                                                                    \
+    for (SliceSpan _span = slice_span(M.nslices, M.map), _once = {0, 1, 1}; _once.s < _once.e;    \
+         ++_once.s)                                                                                  \
+        for (uint32_t _s = _span.s; _s < _span.e; _s += (R) * _span.step)                          \
+            if (const SliceRows<R> q = slice_rows<V, R>(M, _s, _span, lane); true)
 
 // ---------------------------------------------------------------------------
 // init / replacement: r = b - A64 x64 (FP64), stored in precision V;
@@ -422,7 +477,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_residual_kernel(
 // vector are exact in floating point (Sterbenz), so the FP32 product keeps
 // its relative accuracy on the smooth modes that standard-form summation
 // loses to cancellation.
-template <class V, bool DIFF, int CH>
+template <class V, bool DIFF, int CH, int R>
 __global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(SellArgs<V> M, const V2<V>* __restrict__ cur,
                                                             V2<V>* __restrict__ nxt, V* __restrict__ q, uint32_t qs,
                                                             CgScalars* sc, double* partials, uint32_t* counter) {
@@ -430,37 +485,31 @@ __global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(SellArgs<V> M, const
     const V beta = (V)sc->beta;
     const uint32_t lane = threadIdx.x & 63u;
     double v[1] = {0.0};
-    SELL_FOR_SLICES(s) {
-        const uint32_t i = s * kSlice + lane;
-        const bool valid = i < M.n;
-        const uint32_t ii = valid ? i : 0u;
-        const V2<V> p = cur[ii];
-        const V dn = p.x + beta * p.y;
-        const uint32_t L = valid ? (uint32_t)M.len[i] : 0u;
-        const uint32_t W = M.width[s];
-        const uint32_t base = M.off[s] + lane;
-        V acc = M.diag[ii] * dn;
-        for (uint32_t k = 0; k < W; k += CH) {
-            uint32_t c[CH];
-            V a[CH];
-            sell_load<V, CH>(M, base, k, L, ii, c, a);
-            V2<V> g[CH];
+    SELL_FOR_SLICE_ROWS(R, rows) {
+        V dn[R], acc[R];
 #pragma unroll
-            for (int u = 0; u < CH; ++u) g[u] = cur[c[u]];
-#pragma unroll
-            for (int u = 0; u < CH; ++u) {
-                const V gd = g[u].x + beta * g[u].y;
+        for (int r = 0; r < R; ++r) {
+            const V2<V> p = cur[rows.ii[r]];
+            dn[r] = p.x + beta * p.y;
+            acc[r] = M.diag[rows.ii[r]] * dn[r];
+        }
+        sell_rows_product<V, R, CH, V2<V>>(
+            M, rows, acc, [&](uint32_t c) { return cur[c]; },
+            [&](int r, V ac, V a, V2<V> g) {
+                const V gd = g.x + beta * g.y;
                 if constexpr (DIFF)
-                    acc += a[u] * (gd - dn);
+                    return ac + a * (gd - dn[r]);
                 else
-                    acc += a[u] * gd;
+                    return ac + a * gd;
+            });
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (rows.valid[r]) {
+                const uint32_t i = rows.i[r];
+                nxt[i].y = dn[r];
+                q[(size_t)i * qs] = acc[r];
+                v[0] += (double)dn[r] * (double)acc[r];
             }
-        }
-        if (valid) {
-            nxt[i].y = dn;
-            q[(size_t)i * qs] = acc;
-            v[0] += (double)dn * (double)acc;
-        }
     }
     double tot[1];
     if (!grid_fold<1>(v, partials, counter, tot)) return;
@@ -474,41 +523,34 @@ __global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(SellArgs<V> M, const
 // apply of the previous iteration, which knows beta), so the row loop
 // gathers 4-byte d_j instead of {z, d} pairs and writes nothing but q.
 // q = A d (row-sum form when DIFF), q into the {r, q} pair, partial d.q.
-template <class V, bool DIFF, int CH>
+template <class V, bool DIFF, int CH, int R>
 __global__ __launch_bounds__(kCgThreads) void llt_dir_kernel(SellArgs<V> M, const V* __restrict__ d,
                                                              V* __restrict__ q, uint32_t qs, CgScalars* sc,
                                                              double* partials, uint32_t* counter) {
     if (sc->done) return;
     const uint32_t lane = threadIdx.x & 63u;
     double v[1] = {0.0};
-    SELL_FOR_SLICES(s) {
-        const uint32_t i = s * kSlice + lane;
-        const bool valid = i < M.n;
-        const uint32_t ii = valid ? i : 0u;
-        const V dn = d[ii];
-        const uint32_t L = valid ? (uint32_t)M.len[i] : 0u;
-        const uint32_t W = M.width[s];
-        const uint32_t base = M.off[s] + lane;
-        V acc = M.diag[ii] * dn;
-        for (uint32_t k = 0; k < W; k += CH) {
-            uint32_t c[CH];
-            V a[CH];
-            sell_load<V, CH>(M, base, k, L, ii, c, a);
-            V g[CH];
+    SELL_FOR_SLICE_ROWS(R, rows) {
+        V dn[R], acc[R];
 #pragma unroll
-            for (int u = 0; u < CH; ++u) g[u] = d[c[u]];
-#pragma unroll
-            for (int u = 0; u < CH; ++u) {
+        for (int r = 0; r < R; ++r) {
+            dn[r] = d[rows.ii[r]];
+            acc[r] = M.diag[rows.ii[r]] * dn[r];
+        }
+        sell_rows_product<V, R, CH, V>(
+            M, rows, acc, [&](uint32_t c) { return d[c]; },
+            [&](int r, V ac, V a, V g) {
                 if constexpr (DIFF)
-                    acc += a[u] * (g[u] - dn);
+                    return ac + a * (g - dn[r]);
                 else
-                    acc += a[u] * g[u];
+                    return ac + a * g;
+            });
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (rows.valid[r]) {
+                q[(size_t)rows.i[r] * qs] = acc[r];
+                v[0] += (double)dn[r] * (double)acc[r];
             }
-        }
-        if (valid) {
-            q[(size_t)i * qs] = acc;
-            v[0] += (double)dn * (double)acc;
-        }
     }
     double tot[1];
     if (!grid_fold<1>(v, partials, counter, tot)) return;
@@ -567,7 +609,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_update_kernel(uint32_t n, XV* _
 // mode 0: iteration (beta, iter, done); mode 1: init, mode 2: after a
 // residual replacement (r given in rq.x, alpha = 0, x and rq_nxt untouched;
 // mode 2 keeps beta as in cg_residual_kernel).
-template <class V, class XV, int CH>
+template <class V, class XV, int CH, int R>
 __global__ __launch_bounds__(kCgThreads) void llt_update_kernel(SellArgs<V> U, XV* __restrict__ x,
                                                                 const V* __restrict__ d,
                                                                 const V2<V>* __restrict__ rq,
@@ -581,35 +623,29 @@ __global__ __launch_bounds__(kCgThreads) void llt_update_kernel(SellArgs<V> U, X
     const bool it = mode == 0;
     const uint32_t lane = threadIdx.x & 63u;
     double v[2] = {0.0, 0.0};
-    SELL_FOR_SLICES(s) {
-        const uint32_t i = s * kSlice + lane;
-        const bool valid = i < U.n;
-        const uint32_t ii = valid ? i : 0u;
-        const uint32_t L = valid ? (uint32_t)U.len[i] : 0u;
-        const uint32_t W = U.width[s];
-        const uint32_t base = U.off[s] + lane;
-        const V2<V> own = rq[ii];
-        const V ri = it ? own.x - av * own.y : own.x;
-        V acc = U.diag[ii] * ri;
-        for (uint32_t k = 0; k < W; k += CH) {
-            uint32_t c[CH];
-            V a[CH];
-            sell_load<V, CH>(U, base, k, L, ii, c, a);
-            V2<V> g[CH];
+    SELL_FOR_SLICE_ROWS(R, rows) {
+        V ri[R], acc[R];
 #pragma unroll
-            for (int u = 0; u < CH; ++u) g[u] = rq[c[u]];
-#pragma unroll
-            for (int u = 0; u < CH; ++u) acc += a[u] * (it ? g[u].x - av * g[u].y : g[u].x);
+        for (int r = 0; r < R; ++r) {
+            const V2<V> own = rq[rows.ii[r]];
+            ri[r] = it ? own.x - av * own.y : own.x;
+            acc[r] = U.diag[rows.ii[r]] * ri[r];
         }
-        if (valid) {
-            if (it) {
-                x[i] += (XV)(alpha * (double)d[i]);
-                rq_nxt[i].x = ri;
+        sell_rows_product<V, R, CH, V2<V>>(
+            U, rows, acc, [&](uint32_t c) { return rq[c]; },
+            [&](int, V ac, V a, V2<V> g) { return ac + a * (it ? g.x - av * g.y : g.x); });
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (rows.valid[r]) {
+                const uint32_t i = rows.i[r];
+                if (it) {
+                    x[i] += (XV)(alpha * (double)d[i]);
+                    rq_nxt[i].x = ri[r];
+                }
+                w[i] = acc[r];
+                v[0] += (double)acc[r] * (double)acc[r];
+                v[1] += (double)ri[r] * (double)ri[r];
             }
-            w[i] = acc;
-            v[0] += (double)acc * (double)acc;
-            v[1] += (double)ri * (double)ri;
-        }
     }
     double tot[2];
     if (!grid_fold<2>(v, partials, counter, tot)) return;
@@ -641,29 +677,25 @@ __global__ __launch_bounds__(kCgThreads) void llt_update_kernel(SellArgs<V> U, X
 // mode 1: init (d_new = z, d_old not read);
 // mode 2: after a residual replacement (run even when done; beta is the
 //         recursion's, as in cg_residual_kernel).
-template <class V, int CH>
+template <class V, int CH, int R>
 __global__ __launch_bounds__(kCgThreads) void llt_lo_kernel(SellArgs<V> M, const V* __restrict__ w,
                                                             const V* __restrict__ d_old, V* __restrict__ d_new,
                                                             const CgScalars* sc, int mode) {
     if (mode == 0 && sc->done) return;
     const V beta = mode == 1 ? (V)0 : (V)sc->beta;
     const uint32_t lane = threadIdx.x & 63u;
-    SELL_FOR_SLICES(s) {
-        const uint32_t i = s * kSlice + lane;
-        const bool valid = i < M.n;
-        const uint32_t ii = valid ? i : 0u;
-        const uint32_t L = valid ? (uint32_t)M.len[i] : 0u;
-        const uint32_t W = M.width[s];
-        const uint32_t base = M.off[s] + lane;
-        V acc = M.diag[ii] * w[ii];
-        for (uint32_t k = 0; k < W; k += CH) {
-            uint32_t c[CH];
-            V a[CH];
-            sell_load<V, CH>(M, base, k, L, ii, c, a);
+    SELL_FOR_SLICE_ROWS(R, rows) {
+        V acc[R];
 #pragma unroll
-            for (int u = 0; u < CH; ++u) acc += a[u] * w[c[u]];
-        }
-        if (valid) d_new[i] = mode == 1 ? acc : acc + beta * d_old[i];
+        for (int r = 0; r < R; ++r) acc[r] = M.diag[rows.ii[r]] * w[rows.ii[r]];
+        sell_rows_product<V, R, CH, V>(
+            M, rows, acc, [&](uint32_t c) { return w[c]; }, [&](int, V ac, V a, V g) { return ac + a * g; });
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (rows.valid[r]) {
+                const uint32_t i = rows.i[r];
+                d_new[i] = mode == 1 ? acc[r] : acc[r] + beta * d_old[i];
+            }
     }
 }
 
@@ -761,7 +793,7 @@ template <class V> CgVecs<V> carve(CgWork& w, uint32_t n) {
 // updates go to x itself (V = double) or to an FP32 correction merged into
 // x at each residual replacement (V = float).
 // ---------------------------------------------------------------------------
-template <class V, bool DIFF, int PREC, int CH>
+template <class V, bool DIFF, int PREC, int CH, int R>
 void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const LowerFactor<V>* L, const double* b,
               double* x, const CgOptions& o, CgWork& w, SolveStats& st, hipStream_t s) {
     using XV = V;
@@ -803,9 +835,9 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
             hipLaunchKernelGGL((cg_residual_kernel<V, PREC>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p,
                                b, x, reinterpret_cast<V*>(rqc), 2u, cur, idiag, w.sc.p, w.partials.p, w.counter.p,
                                mode, rtol2);
-            hipLaunchKernelGGL((llt_update_kernel<V, XV, CH>), grid, blk, 0, s, MU, xacc, vv.D[it & 1], rqc, rqc,
+            hipLaunchKernelGGL((llt_update_kernel<V, XV, CH, R>), grid, blk, 0, s, MU, xacc, vv.D[it & 1], rqc, rqc,
                                vv.w, w.sc.p, w.partials.p, w.counter.p, mode == 0 ? 1 : 2);
-            hipLaunchKernelGGL((llt_lo_kernel<V, CH>), grid, blk, 0, s, ML, vv.w, vv.D[(it + 1) & 1], vv.D[it & 1],
+            hipLaunchKernelGGL((llt_lo_kernel<V, CH, R>), grid, blk, 0, s, ML, vv.w, vv.D[(it + 1) & 1], vv.D[it & 1],
                                w.sc.p, mode == 0 ? 1 : 2);
         } else {
             hipLaunchKernelGGL((cg_residual_kernel<V, PREC>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p,
@@ -826,14 +858,14 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
                 V2<V>* rqc = vv.RQ[it & 1];
                 V2<V>* rqn = vv.RQ[(it + 1) & 1];
                 V* dc = vv.D[it & 1];
-                hipLaunchKernelGGL((llt_dir_kernel<V, DIFF, CH>), grid, blk, 0, s, MA, dc,
+                hipLaunchKernelGGL((llt_dir_kernel<V, DIFF, CH, R>), grid, blk, 0, s, MA, dc,
                                    reinterpret_cast<V*>(rqc) + 1, 2u, w.sc.p, w.partials.p, w.counter.p);
-                hipLaunchKernelGGL((llt_update_kernel<V, XV, CH>), grid, blk, 0, s, MU, xacc, dc, rqc, rqn, vv.w,
+                hipLaunchKernelGGL((llt_update_kernel<V, XV, CH, R>), grid, blk, 0, s, MU, xacc, dc, rqc, rqn, vv.w,
                                    w.sc.p, w.partials.p, w.counter.p, 0);
-                hipLaunchKernelGGL((llt_lo_kernel<V, CH>), grid, blk, 0, s, ML, vv.w, dc, vv.D[(it + 1) & 1], w.sc.p,
+                hipLaunchKernelGGL((llt_lo_kernel<V, CH, R>), grid, blk, 0, s, ML, vv.w, dc, vv.D[(it + 1) & 1], w.sc.p,
                                    0);
             } else {
-                hipLaunchKernelGGL((cg_dir_kernel<V, DIFF, CH>), grid, blk, 0, s, MA, cur, nxt, vv.q, 1u, w.sc.p,
+                hipLaunchKernelGGL((cg_dir_kernel<V, DIFF, CH, R>), grid, blk, 0, s, MA, cur, nxt, vv.q, 1u, w.sc.p,
                                    w.partials.p, w.counter.p);
                 hipLaunchKernelGGL((cg_update_kernel<V, XV, PREC>), grid, blk, 0, s, n, xacc, vv.r, nxt, vv.q, idiag,
                                    w.sc.p, w.partials.p, w.counter.p);
@@ -920,25 +952,25 @@ void true_residual(const DevCsr<double>& A, const double* b, const double* x, do
 }
 
 // SELL entries loaded per step of the row loop (all loads of a step are
-// issued before their gathers).
-template <class V, int CH>
+// issued before their gathers) and slices per wavefront step.
+template <class V, int CH, int R>
 void cg_solve_ch(const DevCsr<double>& A64, const Sell<V>& S, bool diff, int32_t prec, const V* idiag,
                  const LowerFactor<V>* L, const double* b, double* x, const CgOptions& o, CgWork& w,
                  SolveStats& st, hipStream_t s) {
     if (diff) {
         if (prec == kPrecLLt)
-            cg_drive<V, true, kPrecLLt, CH>(A64, S, idiag, L, b, x, o, w, st, s);
+            cg_drive<V, true, kPrecLLt, CH, R>(A64, S, idiag, L, b, x, o, w, st, s);
         else if (prec == kPrecJacobi)
-            cg_drive<V, true, kPrecJacobi, CH>(A64, S, idiag, L, b, x, o, w, st, s);
+            cg_drive<V, true, kPrecJacobi, CH, R>(A64, S, idiag, L, b, x, o, w, st, s);
         else
-            cg_drive<V, true, kPrecNone, CH>(A64, S, idiag, L, b, x, o, w, st, s);
+            cg_drive<V, true, kPrecNone, CH, R>(A64, S, idiag, L, b, x, o, w, st, s);
     } else {
         if (prec == kPrecLLt)
-            cg_drive<V, false, kPrecLLt, CH>(A64, S, idiag, L, b, x, o, w, st, s);
+            cg_drive<V, false, kPrecLLt, CH, R>(A64, S, idiag, L, b, x, o, w, st, s);
         else if (prec == kPrecJacobi)
-            cg_drive<V, false, kPrecJacobi, CH>(A64, S, idiag, L, b, x, o, w, st, s);
+            cg_drive<V, false, kPrecJacobi, CH, R>(A64, S, idiag, L, b, x, o, w, st, s);
         else
-            cg_drive<V, false, kPrecNone, CH>(A64, S, idiag, L, b, x, o, w, st, s);
+            cg_drive<V, false, kPrecNone, CH, R>(A64, S, idiag, L, b, x, o, w, st, s);
     }
 }
 
@@ -948,10 +980,15 @@ void cg_solve(const DevCsr<double>& A64, const Sell<V>& S, bool diff, int32_t pr
               hipStream_t s) {
     if (prec == kPrecLLt && !L) throw std::invalid_argument("L L^T preconditioner requested without a factor");
     if (prec == kPrecJacobi && !idiag) throw std::invalid_argument("Jacobi preconditioner requested without a diagonal");
-    if (tuning().sell_chunk == 4)
-        cg_solve_ch<V, 4>(A64, S, diff, prec, idiag, L, b, x, o, w, st, s);
+    const bool c4 = tuning().sell_chunk == 4, r2 = tuning().sell_rows == 2;
+    if (c4 && r2)
+        cg_solve_ch<V, 4, 2>(A64, S, diff, prec, idiag, L, b, x, o, w, st, s);
+    else if (c4)
+        cg_solve_ch<V, 4, 1>(A64, S, diff, prec, idiag, L, b, x, o, w, st, s);
+    else if (r2)
+        cg_solve_ch<V, 8, 2>(A64, S, diff, prec, idiag, L, b, x, o, w, st, s);
     else
-        cg_solve_ch<V, 8>(A64, S, diff, prec, idiag, L, b, x, o, w, st, s);
+        cg_solve_ch<V, 8, 1>(A64, S, diff, prec, idiag, L, b, x, o, w, st, s);
 }
 template void cg_solve<float>(const DevCsr<double>&, const Sell<float>&, bool, int32_t, const float*,
                               const LowerFactor<float>*, const double*, double*, const CgOptions&, CgWork&,
@@ -959,6 +996,27 @@ template void cg_solve<float>(const DevCsr<double>&, const Sell<float>&, bool, i
 template void cg_solve<double>(const DevCsr<double>&, const Sell<double>&, bool, int32_t, const double*,
                                const LowerFactor<double>*, const double*, double*, const CgOptions&, CgWork&,
                                SolveStats&, hipStream_t);
+
+// One launch of a direction kernel on scratch vectors (ping-pong by k).
+template <class V, int CH, int R>
+void dir_launch(const SellArgs<V>& M, bool diff, bool llt, CgVecs<V>& vv, CgWork& w, uint32_t k, hipStream_t s) {
+    const dim3 grid(w.blocks), blk(kCgThreads);
+    V2<V>* cur = vv.P[k & 1];
+    V2<V>* nxt = vv.P[(k + 1) & 1];
+    V* q = reinterpret_cast<V*>(vv.RQ[k & 1]) + 1;
+    if (llt && diff)
+        hipLaunchKernelGGL((llt_dir_kernel<V, true, CH, R>), grid, blk, 0, s, M, vv.D[k & 1], q, 2u, w.sc.p,
+                           w.partials.p, w.counter.p);
+    else if (llt)
+        hipLaunchKernelGGL((llt_dir_kernel<V, false, CH, R>), grid, blk, 0, s, M, vv.D[k & 1], q, 2u, w.sc.p,
+                           w.partials.p, w.counter.p);
+    else if (diff)
+        hipLaunchKernelGGL((cg_dir_kernel<V, true, CH, R>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
+                           w.partials.p, w.counter.p);
+    else
+        hipLaunchKernelGGL((cg_dir_kernel<V, false, CH, R>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
+                           w.partials.p, w.counter.p);
+}
 
 // Average duration (ms, HIP events on the context stream) of a CG
 // direction kernel over `reps` back-to-back launches on S: llt = false, the
@@ -976,37 +1034,17 @@ double time_dir_kernel(const Sell<V>& S, bool diff, bool llt, uint32_t reps, CgW
     SA_HIP(hipMemcpyAsync(w.sc.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
     const size_t nv = ((size_t)n + 63) & ~(size_t)63;
     fill<V>(reinterpret_cast<V*>(vv.P[0]), 10 * nv, (V)1, s);  // P, RQ and D
-    const dim3 grid(w.blocks), blk(kCgThreads);
     const SellArgs<V> M = sell_args(S);
-    const bool c4 = tuning().sell_chunk == 4;
+    const bool c4 = tuning().sell_chunk == 4, r2 = tuning().sell_rows == 2;
     auto launch = [&](uint32_t k) {
-        V2<V>* cur = vv.P[k & 1];
-        V2<V>* nxt = vv.P[(k + 1) & 1];
-        V* q = reinterpret_cast<V*>(vv.RQ[k & 1]) + 1;
-        if (llt && diff && c4)
-            hipLaunchKernelGGL((llt_dir_kernel<V, true, 4>), grid, blk, 0, s, M, vv.D[k & 1], q, 2u, w.sc.p,
-                               w.partials.p, w.counter.p);
-        else if (llt && diff)
-            hipLaunchKernelGGL((llt_dir_kernel<V, true, 8>), grid, blk, 0, s, M, vv.D[k & 1], q, 2u, w.sc.p,
-                               w.partials.p, w.counter.p);
-        else if (llt && c4)
-            hipLaunchKernelGGL((llt_dir_kernel<V, false, 4>), grid, blk, 0, s, M, vv.D[k & 1], q, 2u, w.sc.p,
-                               w.partials.p, w.counter.p);
-        else if (llt)
-            hipLaunchKernelGGL((llt_dir_kernel<V, false, 8>), grid, blk, 0, s, M, vv.D[k & 1], q, 2u, w.sc.p,
-                               w.partials.p, w.counter.p);
-        else if (diff && c4)
-            hipLaunchKernelGGL((cg_dir_kernel<V, true, 4>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
-                               w.partials.p, w.counter.p);
-        else if (diff)
-            hipLaunchKernelGGL((cg_dir_kernel<V, true, 8>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
-                               w.partials.p, w.counter.p);
+        if (c4 && r2)
+            dir_launch<V, 4, 2>(M, diff, llt, vv, w, k, s);
         else if (c4)
-            hipLaunchKernelGGL((cg_dir_kernel<V, false, 4>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
-                               w.partials.p, w.counter.p);
+            dir_launch<V, 4, 1>(M, diff, llt, vv, w, k, s);
+        else if (r2)
+            dir_launch<V, 8, 2>(M, diff, llt, vv, w, k, s);
         else
-            hipLaunchKernelGGL((cg_dir_kernel<V, false, 8>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
-                               w.partials.p, w.counter.p);
+            dir_launch<V, 8, 1>(M, diff, llt, vv, w, k, s);
     };
     for (uint32_t k = 0; k < 3; ++k) launch(k);
     hipEvent_t e0, e1;

@@ -139,6 +139,7 @@ struct Tuning {
     uint32_t xcd_map = 1;          // SELL slices: 0 grid stride, 1 XCD-contiguous chunks per wave
     uint32_t fold_group = 32;      // blocks per group of the two-level grid fold (0: one level)
     uint32_t sell_chunk = 8;       // SELL entries per load step of the CG row loops (4 or 8)
+    uint32_t sell_rows = 1;        // SELL slices a wavefront works on at once (1 or 2)
 };
 Tuning& tuning();
 // Group size of the two-level grid fold of the CG kernels (0, 8, 16, 32, 64).

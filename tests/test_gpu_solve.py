@@ -212,18 +212,23 @@ def test_dam_break_1m_fp64_and_mixed():
     assert (A != A.T).nnz == 0
 
 
-@pytest.mark.parametrize("solver", ["fp64", "mixed", "fsai"])
+@pytest.mark.parametrize("solver", ["fp64", "mixed", "fsai", "learned_zero"])
 def test_graph_replay_and_xcd_map(solver):
     """CG batches replayed as captured HIP graphs give the eager run's bits
     (every batch is a replay, with host checks between replays); both slice
-    mappings converge to the tolerance."""
+    mappings converge to the tolerance.  learned_zero: the L L^T path with
+    the zero-head network's factor (L = D^-1/2)."""
     import stream_amd
+    from stream_amd.learned import PrecLearned
     from stream_amd.pfem import PfemStep2D
 
     _say(f"graph replay / xcd map {solver}")
     pts, al = clouds.dam_break_scaled(60_000)
     f = clouds.divergence_source(pts, clouds.fourier_modes(11))
     st = PfemStep2D(pts, al, f)
+    if solver == "learned_zero":
+        st.set_learned(PrecLearned())
+        solver = "learned"
     try:
         for xm in (0, 1):
             stream_amd.tune("xcd_map", xm)
@@ -252,18 +257,23 @@ def test_graph_replay_and_xcd_map(solver):
             r2 = st.run(solver, rtol=1e-8)
             assert r2["iters"] == r["iters"]
             print(f"{solver} fold_group={fg}: iters={r['iters']} solve={r['solve_ms']:.1f}ms")
-        # SELL load steps of 4 or 8 entries add the same terms in the same
-        # order: identical bits
+        # SELL load steps of 4 or 8 entries, one or two slices per wavefront
+        # step: the same terms added in the same order, identical bits
         stream_amd.tune("graph", 0)
         stream_amd.tune("fold_group", 32)
         out = []
         for ch in (4, 8):
-            stream_amd.tune("sell_chunk", ch)
-            r = st.run(solver, rtol=1e-8)
-            out.append((r["iters"], st.solution(morton=True)))
-        assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1])
+            for rows in (1, 2):
+                stream_amd.tune("sell_chunk", ch)
+                stream_amd.tune("sell_rows", rows)
+                r = st.run(solver, rtol=1e-8)
+                out.append((r["iters"], st.solution(morton=True)))
+                print(f"{solver} sell_chunk={ch} sell_rows={rows}: iters={r['iters']} solve={r['solve_ms']:.1f}ms")
+        for it, x in out[1:]:
+            assert it == out[0][0] and np.array_equal(x, out[0][1])
     finally:
         stream_amd.tune("graph", 0)
         stream_amd.tune("xcd_map", 1)
         stream_amd.tune("fold_group", 32)
         stream_amd.tune("sell_chunk", 8)
+        stream_amd.tune("sell_rows", 1)
