@@ -63,8 +63,9 @@ int32_t SolverCG_solve_mixed(SolverCG* solver, const LinSys* sys, const PrecJaco
  * default 0), "xcd_map" (SELL slices in XCD-contiguous chunks, default 1)
  * "fold_group" (blocks per group of the two-level reduction, 0 = one
  * level; 8, 16, 32 (default) or 64), "sell_chunk" (SELL entries per load
- * step of the CG row loops, 4 or 8 (default)) and "sell_rows" (SELL slices a
- * wavefront works on at once, 1 or 2). */
+ * step of the CG row loops, 4 or 8 (default)) and "llt_threads" (block size
+ * of the L L^T path's kernels, 256, 512 (default) or 1024; "cg_blocks"
+ * counts 256-thread blocks, so the thread count is the same). */
 int32_t Stream_tune(const char* key, double value);
 
 /* Mesh-and-solve step on device-resident points: LBVH + 2D alpha-shape +

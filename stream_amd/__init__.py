@@ -62,7 +62,7 @@ def synchronize() -> None:
 
 def tune(key: str, value: float) -> None:
     """Set a solver knob: 'replace_factor', 'batch', 'cg_blocks', 'graph',
-    'xcd_map', 'fold_group', 'sell_chunk' or 'sell_rows'."""
+    'xcd_map', 'fold_group', 'sell_chunk' or 'llt_threads'."""
     check(lib.Stream_tune(key.encode(), float(value)))
 
 

@@ -7,6 +7,9 @@
 
 namespace stream_amd {
 
+// Block size of the Jacobi-path CG kernels; the L L^T path's kernels are
+// instantiated for 256, 512 and 1024 (Tuning::llt_threads).  Every kernel
+// takes its block size NT as a template argument.
 constexpr int kCgThreads = 256;
 // Grid of every CG kernel: at most Tuning::cg_blocks blocks (<= 4096) with
 // grid-stride rows; the last block folds at most 16 partials per thread,
@@ -21,9 +24,16 @@ Tuning& tuning() {
     return t;
 }
 
-void CgWork::resize(uint32_t n) {
-    blocks = (n + kCgThreads - 1) / kCgThreads;
-    const uint32_t cap = tuning().cg_blocks < kMaxCgBlocks ? tuning().cg_blocks : kMaxCgBlocks;
+void CgWork::resize(uint32_t n, uint32_t nthreads) {
+    if (!(nthreads == 256 || nthreads == 512 || nthreads == 1024))
+        throw std::invalid_argument("CG block size must be 256, 512 or 1024");
+    threads = nthreads;
+    blocks = (n + threads - 1) / threads;
+    // cg_blocks caps the grid in 256-thread units: the same number of
+    // threads whatever the block size
+    uint32_t cap = tuning().cg_blocks < kMaxCgBlocks ? tuning().cg_blocks : kMaxCgBlocks;
+    cap = cap * kCgThreads / threads;
+    if (cap == 0) cap = 1;
     if (blocks > cap) blocks = cap;
     if (blocks == 0) blocks = 1;
     sc.resize(1);
@@ -49,8 +59,10 @@ void set_fold_group(uint32_t g) {
 
 namespace {
 
+// Block sum of an NT-thread block (s_tmp: NT / kWave doubles), valid in
+// thread 0.
 template <int NT> __device__ __forceinline__ double block_sum(double v, double* s_tmp) {
-    constexpr int NW = NT / kWave;
+    constexpr uint32_t nw = NT / kWave;
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
     const uint32_t w = threadIdx.x / kWave;
@@ -58,9 +70,9 @@ template <int NT> __device__ __forceinline__ double block_sum(double v, double* 
     __syncthreads();
     double t = 0.0;
     if (threadIdx.x == 0)
-        for (int k = 0; k < NW; ++k) t += s_tmp[k];
+        for (uint32_t k = 0; k < nw; ++k) t += s_tmp[k];
     __syncthreads();
-    return t;  // valid in thread 0
+    return t;
 }
 
 // Fold K per-thread values over the grid.  Every block stores its K block
@@ -73,17 +85,17 @@ template <int NT> __device__ __forceinline__ double block_sum(double v, double* 
 // fold serialises G atomics on one address.  The last block returns true in
 // all its threads with the totals in tot[]; counters are reset by the blocks
 // that finish them.  Every order is fixed, so the totals are reproducible.
-template <int K>
+template <int K, int NT>
 __device__ bool grid_fold(const double (&v)[K], double* partials, uint32_t* counter, double (&tot)[K]) {
-    __shared__ double s_tmp[kCgThreads / kWave];
+    __shared__ double s_tmp[NT / kWave];
     __shared__ double s_tot[K];
     __shared__ bool s_last;
     const uint32_t G = gridDim.x;  // <= kMaxCgBlocks
     const uint32_t grp = c_fold_group;
     double bs[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) bs[k] = block_sum<kCgThreads>(v[k], s_tmp);
-    constexpr int PER = kMaxCgBlocks / kCgThreads;
+    for (int k = 0; k < K; ++k) bs[k] = block_sum<NT>(v[k], s_tmp);
+    constexpr int PER = kMaxCgBlocks / NT;
     if (grp == 0) {
         if (threadIdx.x == 0) {
 #pragma unroll
@@ -101,13 +113,13 @@ __device__ bool grid_fold(const double (&v)[K], double* partials, uint32_t* coun
             double pv[PER];
 #pragma unroll
             for (int u = 0; u < PER; ++u) {
-                const uint32_t b = threadIdx.x + u * kCgThreads;
+                const uint32_t b = threadIdx.x + u * NT;
                 pv[u] = b < G ? coherent_load(&partials[(size_t)k * G + b]) : 0.0;
             }
             double a = 0.0;
 #pragma unroll
             for (int u = 0; u < PER; ++u) a += pv[u];
-            const double t = block_sum<kCgThreads>(a, s_tmp);
+            const double t = block_sum<NT>(a, s_tmp);
             if (threadIdx.x == 0) s_tot[k] = t;
         }
         if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -136,7 +148,7 @@ __device__ bool grid_fold(const double (&v)[K], double* partials, uint32_t* coun
     for (int k = 0; k < K; ++k) {
         const uint32_t b = g * grp + threadIdx.x;
         const double a = threadIdx.x < gsize ? coherent_load(&partials[(size_t)k * G + b]) : 0.0;
-        const double t = block_sum<kCgThreads>(a, s_tmp);
+        const double t = block_sum<NT>(a, s_tmp);
         if (threadIdx.x == 0) s_tot[k] = t;
     }
     if (threadIdx.x == 0) {
@@ -150,19 +162,19 @@ __device__ bool grid_fold(const double (&v)[K], double* partials, uint32_t* coun
     __syncthreads();
     if (!s_last) return false;
     vmem_acquire_order();
-    constexpr int PER2 = kMaxFoldGroups / kCgThreads;
+    constexpr int PER2 = (kMaxFoldGroups + NT - 1) / NT;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         double pv[PER2];
 #pragma unroll
         for (int u = 0; u < PER2; ++u) {
-            const uint32_t q = threadIdx.x + u * kCgThreads;
+            const uint32_t q = threadIdx.x + u * NT;
             pv[u] = q < ng ? coherent_load(&gpart[(size_t)k * ng + q]) : 0.0;
         }
         double a = 0.0;
 #pragma unroll
         for (int u = 0; u < PER2; ++u) a += pv[u];
-        const double t = block_sum<kCgThreads>(a, s_tmp);
+        const double t = block_sum<NT>(a, s_tmp);
         if (threadIdx.x == 0) s_tot[k] = t;
     }
     if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -170,6 +182,137 @@ __device__ bool grid_fold(const double (&v)[K], double* partials, uint32_t* coun
 #pragma unroll
     for (int k = 0; k < K; ++k) tot[k] = s_tot[k];
     return true;
+}
+
+// Block partials without a grid fold: each block stores its K block sums
+// at p[k * stride + blockIdx.x]; the next kernel sums them (sum_partials).
+// The producer ends with no ticket atomics and no last-block tail.  The
+// wave sums of all K values cross LDS behind one barrier.
+template <int K, int NT>
+__device__ void store_partials(const double (&v)[K], double* __restrict__ p, uint32_t stride) {
+    constexpr uint32_t nw = NT / kWave;
+    __shared__ double s_w[K][nw];
+    double a[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        a[k] = v[k];
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) a[k] += __shfl_xor(a[k], o, kWave);
+    }
+    const uint32_t wv = threadIdx.x / kWave;
+    if (lane_id() == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) s_w[k][wv] = a[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double t = 0.0;
+            for (uint32_t q = 0; q < nw; ++q) t += s_w[k][q];
+            p[(size_t)k * stride + blockIdx.x] = t;
+        }
+}
+
+// The K x G partials a previous kernel stored with store_partials, read as
+// one index space e = k G + b: thread t adds entries t, t + NT, ...
+// of every k in ascending order, then the lanes' sums meet in an xor
+// butterfly (every lane gets the same bits, additions being commutative)
+// and the waves' in LDS behind one barrier, in wave order.  Every block
+// that sums the same partials therefore gets the same bits, so each block
+// of a kernel can derive the same scalar (alpha, beta) without a grid-wide
+// fold.
+template <int K> __device__ __forceinline__ double partial_at(const double* p, uint32_t stride, uint32_t G,
+                                                              uint32_t e) {
+    uint32_t kk = 0;
+#pragma unroll
+    for (int k = 1; k < K; ++k) kk += e >= k * G ? 1u : 0u;
+    return p[(size_t)kk * stride + (e - kk * G)];
+}
+template <int K> __device__ __forceinline__ void add_partial(double (&a)[K], uint32_t G, uint32_t e, double v) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (e >= k * G && e < (k + 1) * G) a[k] += v;
+}
+template <int K, int NT> __device__ void finish_partials(double (&a)[K], double (&tot)[K]) {
+    constexpr uint32_t nw = NT / kWave;
+    __shared__ double s_w[K][nw];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) a[k] += __shfl_xor(a[k], o, kWave);
+    const uint32_t wv = threadIdx.x / kWave;
+    if (lane_id() == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) s_w[k][wv] = a[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double t = 0.0;
+        for (uint32_t q = 0; q < nw; ++q) t += s_w[k][q];
+        tot[k] = t;
+    }
+}
+
+// Sum of the K x G partials (G <= kMaxCgBlocks), LOADS loads per thread in
+// flight per round.  Result in every thread.
+template <int K, int NT, int LOADS = 16>
+__device__ void sum_partials(const double* __restrict__ p, uint32_t stride, uint32_t G, double (&tot)[K]) {
+    double a[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) a[k] = 0.0;
+    const uint32_t total = K * G;
+    for (uint32_t e0 = 0; e0 < total; e0 += LOADS * NT) {
+        double pv[LOADS];
+#pragma unroll
+        for (int u = 0; u < LOADS; ++u) {
+            const uint32_t e = e0 + threadIdx.x + u * NT;
+            pv[u] = e < total ? partial_at<K>(p, stride, G, e) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < LOADS; ++u) add_partial<K>(a, G, e0 + threadIdx.x + u * NT, pv[u]);
+    }
+    finish_partials<K, NT>(a, tot);
+}
+
+// The same sum in two halves: the loads issued early (prefetch_partials,
+// when K x G <= kPrefetch NT, kPrefetch doubles per thread), the adds after
+// other work (sum_prefetched); larger grids take sum_partials at the
+// point of use, two loads per round so that the caller's registers stay
+// few.  Both orders give sum_partials' bits.
+constexpr int kPrefetch = 4;
+struct PartialsPrefetch {
+    double v[kPrefetch];
+    bool whole;
+};
+template <int K, int NT>
+__device__ __forceinline__ PartialsPrefetch prefetch_partials(const double* __restrict__ p, uint32_t stride,
+                                                              uint32_t G) {
+    PartialsPrefetch f;
+    const uint32_t total = K * G;
+    f.whole = total <= kPrefetch * NT;
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+        const uint32_t e = threadIdx.x + u * NT;
+        f.v[u] = (f.whole && e < total) ? partial_at<K>(p, stride, G, e) : 0.0;
+    }
+    return f;
+}
+template <int K, int NT>
+__device__ void sum_prefetched(const PartialsPrefetch& f, const double* __restrict__ p, uint32_t stride, uint32_t G,
+                               double (&tot)[K]) {
+    if (!f.whole) {
+        sum_partials<K, NT, 2>(p, stride, G, tot);
+        return;
+    }
+    double a[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) a[k] = 0.0;
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+        const uint32_t e = threadIdx.x + u * NT;
+        if (e < K * G) add_partial<K>(a, G, e, f.v[u]);
+    }
+    finish_partials<K, NT>(a, tot);
 }
 
 template <class MV, class XV>
@@ -182,7 +325,8 @@ __device__ __forceinline__ double row_dot(const uint32_t* __restrict__ row, cons
 }
 
 
-#define CG_ROWS(i) for (uint32_t i = blockIdx.x * kCgThreads + threadIdx.x; i < n; i += gridDim.x * kCgThreads)
+// rows of an NT-thread kernel, grid stride
+#define CG_ROWS(i) for (uint32_t i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT)
 
 template <class V> struct V2T;
 template <> struct V2T<float> {
@@ -427,8 +571,8 @@ __device__ __forceinline__ void sell_rows_product(const SellArgs<V>& M, const Sl
 //         below the true one, this degrades to a restart instead of scaling
 //         the old direction by r_true.z / r_rec.z.
 // ---------------------------------------------------------------------------
-template <class V, int PREC>
-__global__ __launch_bounds__(kCgThreads) void cg_residual_kernel(
+template <class V, int PREC, int NT>
+__global__ __launch_bounds__(NT) void cg_residual_kernel(
     uint32_t n, const uint32_t* __restrict__ row, const uint32_t* __restrict__ col, const double* __restrict__ val,
     const double* __restrict__ b, const double* __restrict__ x, V* __restrict__ r, uint32_t rs,
     V2<V>* __restrict__ P, const V* __restrict__ idiag, CgScalars* sc, double* partials, uint32_t* counter, int mode,
@@ -450,7 +594,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_residual_kernel(
         v[2] += b[i] * b[i];
     }
     double tot[3];
-    if (!grid_fold<3>(v, partials, counter, tot)) return;
+    if (!grid_fold<3, NT>(v, partials, counter, tot)) return;
     if (threadIdx.x == 0) {
         sc->rr = tot[0];
         sc->aux = tot[0];
@@ -477,8 +621,8 @@ __global__ __launch_bounds__(kCgThreads) void cg_residual_kernel(
 // vector are exact in floating point (Sterbenz), so the FP32 product keeps
 // its relative accuracy on the smooth modes that standard-form summation
 // loses to cancellation.
-template <class V, bool DIFF, int CH, int R>
-__global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(SellArgs<V> M, const V2<V>* __restrict__ cur,
+template <class V, bool DIFF, int CH, int R, int NT>
+__global__ __launch_bounds__(NT) void cg_dir_kernel(SellArgs<V> M, const V2<V>* __restrict__ cur,
                                                             V2<V>* __restrict__ nxt, V* __restrict__ q, uint32_t qs,
                                                             CgScalars* sc, double* partials, uint32_t* counter) {
     if (sc->done) return;
@@ -512,7 +656,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(SellArgs<V> M, const
             }
     }
     double tot[1];
-    if (!grid_fold<1>(v, partials, counter, tot)) return;
+    if (!grid_fold<1, NT>(v, partials, counter, tot)) return;
     if (threadIdx.x == 0) {
         sc->dq = tot[0];
         sc->alpha = sc->rz / tot[0];
@@ -522,11 +666,13 @@ __global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(SellArgs<V> M, const
 // Direction kernel of the L L^T path: d is already formed (by the lower
 // apply of the previous iteration, which knows beta), so the row loop
 // gathers 4-byte d_j instead of {z, d} pairs and writes nothing but q.
-// q = A d (row-sum form when DIFF), q into the {r, q} pair, partial d.q.
-template <class V, bool DIFF, int CH, int R>
-__global__ __launch_bounds__(kCgThreads) void llt_dir_kernel(SellArgs<V> M, const V* __restrict__ d,
-                                                             V* __restrict__ q, uint32_t qs, CgScalars* sc,
-                                                             double* partials, uint32_t* counter) {
+// q = A d (row-sum form when DIFF), q into the {r, q} pair; the block
+// partials of d.q go to partials[0][block] and the update kernel derives
+// alpha from them.
+template <class V, bool DIFF, int CH, int R, int NT>
+__global__ __launch_bounds__(NT) void llt_dir_kernel(SellArgs<V> M, const V* __restrict__ d,
+                                                             V* __restrict__ q, uint32_t qs, const CgScalars* sc,
+                                                             double* partials) {
     if (sc->done) return;
     const uint32_t lane = threadIdx.x & 63u;
     double v[1] = {0.0};
@@ -552,18 +698,13 @@ __global__ __launch_bounds__(kCgThreads) void llt_dir_kernel(SellArgs<V> M, cons
                 v[0] += (double)dn[r] * (double)acc[r];
             }
     }
-    double tot[1];
-    if (!grid_fold<1>(v, partials, counter, tot)) return;
-    if (threadIdx.x == 0) {
-        sc->dq = tot[0];
-        sc->alpha = sc->rz / tot[0];
-    }
+    store_partials<1, NT>(v, partials, kMaxCgBlocks);
 }
 
 // x += alpha d; r -= alpha q; z = M r (Jacobi / none) into nxt.x;
 // partial r.z, r.r
-template <class V, class XV, int PREC>
-__global__ __launch_bounds__(kCgThreads) void cg_update_kernel(uint32_t n, XV* __restrict__ x, V* __restrict__ r,
+template <class V, class XV, int PREC, int NT>
+__global__ __launch_bounds__(NT) void cg_update_kernel(uint32_t n, XV* __restrict__ x, V* __restrict__ r,
                                                                V2<V>* __restrict__ nxt, const V* __restrict__ q,
                                                                const V* __restrict__ idiag, CgScalars* sc,
                                                                double* partials, uint32_t* counter) {
@@ -585,7 +726,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_update_kernel(uint32_t n, XV* _
         }
     }
     double tot[2];
-    if (!grid_fold<2>(v, partials, counter, tot)) return;
+    if (!grid_fold<2, NT>(v, partials, counter, tot)) return;
     if (threadIdx.x == 0) {
         sc->rr = tot[1];
         if constexpr (PREC != kPrecLLt) {
@@ -604,21 +745,32 @@ __global__ __launch_bounds__(kCgThreads) void cg_update_kernel(uint32_t n, XV* _
 // L_ji r'_j, U = the upper SELL).  r and q live interleaved ({r, q} per
 // row, written by the direction kernel), so the r' of each upper neighbour
 // is recomputed from one 8-byte load with the same operations its owner
-// uses; r' goes to the other {r, q} buffer.  Partials w.w (= r'.z, since
-// z = L w) and r'.r'.
-// mode 0: iteration (beta, iter, done); mode 1: init, mode 2: after a
-// residual replacement (r given in rq.x, alpha = 0, x and rq_nxt untouched;
-// mode 2 keeps beta as in cg_residual_kernel).
-template <class V, class XV, int CH, int R>
-__global__ __launch_bounds__(kCgThreads) void llt_update_kernel(SellArgs<V> U, XV* __restrict__ x,
+// uses; r' goes to the other {r, q} buffer.  Every block derives alpha =
+// r.z / d.q from the direction kernel's partials (block 0 records alpha,
+// d.q and r.z as rz_old); the block partials of w.w (= r'.z, since
+// z = L w) and r'.r' go to partials[1..2][block] for the lower apply.
+// mode 0: iteration; mode 1: init, mode 2: after a residual replacement
+// (r given in rq.x, alpha = 0, x and rq_nxt untouched).
+template <class V, class XV, int CH, int R, int NT>
+__global__ __launch_bounds__(NT) void llt_update_kernel(SellArgs<V> U, XV* __restrict__ x,
                                                                 const V* __restrict__ d,
                                                                 const V2<V>* __restrict__ rq,
                                                                 V2<V>* __restrict__ rq_nxt, V* __restrict__ w,
-                                                                CgScalars* sc, double* partials, uint32_t* counter,
-                                                                int mode) {
+                                                                CgScalars* sc, double* partials, int mode) {
     if (mode == 0 && sc->done) return;
     const SellArgs<V>& M = U;  // slice loop
-    const double alpha = mode == 0 ? sc->alpha : 0.0;
+    double alpha = 0.0;
+    if (mode == 0) {
+        double dq[1];
+        sum_partials<1, NT>(partials, kMaxCgBlocks, gridDim.x, dq);
+        const double rz = sc->rz;
+        alpha = rz / dq[0];
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->dq = dq[0];
+            sc->alpha = alpha;
+            sc->rz_old = rz;
+        }
+    }
     const V av = (V)alpha;
     const bool it = mode == 0;
     const uint32_t lane = threadIdx.x & 63u;
@@ -647,60 +799,82 @@ __global__ __launch_bounds__(kCgThreads) void llt_update_kernel(SellArgs<V> U, X
                 v[1] += (double)ri[r] * (double)ri[r];
             }
     }
-    double tot[2];
-    if (!grid_fold<2>(v, partials, counter, tot)) return;
-    if (threadIdx.x == 0) {
-        if (mode == 0) {
-            sc->rr = tot[1];
-            sc->rz_old = sc->rz;
-            sc->beta = tot[0] / sc->rz;
-            sc->rz = tot[0];
-            sc->iter += 1;
-            const double cv = sc->crit ? tot[0] : tot[1];
-            sc->done = (cv <= sc->tol2 || sc->iter >= sc->maxit) ? 1u : 0u;
-        } else if (mode == 1) {
-            sc->rz = tot[0];
-            sc->rz_old = tot[0];
-            const double cv = sc->crit ? tot[0] : sc->rr;
-            sc->done = (cv <= sc->tol2 || sc->iter >= sc->maxit) ? 1u : 0u;
-        } else {
-            sc->rz = tot[0];
-        }
-    }
+    store_partials<2, NT>(v, partials + kMaxCgBlocks, kMaxCgBlocks);
 }
 
 // Second half of the apply, z = L w (z_i = L_ii w_i + sum_{j<i} L_ij w_j),
-// and the next direction d_new = z + beta d_old: beta = (r'.z)/(r.z) was
-// set by the update kernel's fold, so the direction kernel that follows
-// reads d ready-made and z is never stored.
-// mode 0: iteration (skipped when the solve is flagged done);
-// mode 1: init (d_new = z, d_old not read);
-// mode 2: after a residual replacement (run even when done; beta is the
-//         recursion's, as in cg_residual_kernel).
-template <class V, int CH, int R>
-__global__ __launch_bounds__(kCgThreads) void llt_lo_kernel(SellArgs<V> M, const V* __restrict__ w,
-                                                            const V* __restrict__ d_old, V* __restrict__ d_new,
-                                                            const CgScalars* sc, int mode) {
+// and the next direction d_new = z + beta d_old; z is never stored.  Every
+// block sums the update kernel's partials: mode 0 (iteration) beta =
+// (r'.z) / (r.z) with r.z = rz_old, and block 0 records r.z, r.r, beta,
+// the iteration count and the stop flag; mode 1 (init) d_new = z (d_old
+// not read), block 0 records r.z and the stop flag; mode 2 (after a
+// residual replacement) beta is the recursion's, as in cg_residual_kernel,
+// and block 0 records r.z.  mode 0 is skipped once the solve is flagged
+// done; the iteration that sets the flag may leave d_new partly written
+// (blocks that start after block 0 wrote it return), which is harmless:
+// a residual replacement rebuilds that buffer from d_old before it is read.
+template <class V, int CH, int R, int NT>
+__global__ __launch_bounds__(NT) void llt_lo_kernel(SellArgs<V> M, const V* __restrict__ w,
+                                                               const V* __restrict__ d_old, V* __restrict__ d_new,
+                                                               CgScalars* sc, const double* partials, int mode) {
     if (mode == 0 && sc->done) return;
-    const V beta = mode == 1 ? (V)0 : (V)sc->beta;
+    // the partials' loads go out first; their sum (beta) is taken after the
+    // wavefront's first slice group, whose loads cover their latency
+    const PartialsPrefetch pf = prefetch_partials<2, NT>(partials + kMaxCgBlocks, kMaxCgBlocks, gridDim.x);
     const uint32_t lane = threadIdx.x & 63u;
-    SELL_FOR_SLICE_ROWS(R, rows) {
-        V acc[R];
+    const SliceSpan sp = slice_span(M.nslices, M.map);
+    auto product = [&](const SliceRows<R>& rows, V (&acc)[R]) {
 #pragma unroll
         for (int r = 0; r < R; ++r) acc[r] = M.diag[rows.ii[r]] * w[rows.ii[r]];
         sell_rows_product<V, R, CH, V>(
             M, rows, acc, [&](uint32_t c) { return w[c]; }, [&](int, V ac, V a, V g) { return ac + a * g; });
+    };
+    auto store = [&](const SliceRows<R>& rows, const V (&acc)[R], V beta) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (rows.valid[r]) {
                 const uint32_t i = rows.i[r];
                 d_new[i] = mode == 1 ? acc[r] : acc[r] + beta * d_old[i];
             }
+    };
+    // first group (empty when the wavefront has no slices: no rows valid)
+    const SliceRows<R> first = slice_rows<V, R>(M, sp.s, sp, lane);
+    V acc0[R];
+    product(first, acc0);
+    double tot[2];
+    sum_prefetched<2, NT>(pf, partials + kMaxCgBlocks, kMaxCgBlocks, gridDim.x, tot);
+    double bd = 0.0;
+    if (mode == 0)
+        bd = tot[0] / sc->rz_old;
+    else if (mode == 2)
+        bd = sc->beta;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->rz = tot[0];
+        if (mode == 0) {
+            sc->rr = tot[1];
+            sc->beta = bd;
+            sc->iter += 1;
+            const double cv = sc->crit ? tot[0] : tot[1];
+            sc->done = (cv <= sc->tol2 || sc->iter >= sc->maxit) ? 1u : 0u;
+        } else if (mode == 1) {
+            sc->rz_old = tot[0];
+            const double cv = sc->crit ? tot[0] : sc->rr;
+            sc->done = (cv <= sc->tol2 || sc->iter >= sc->maxit) ? 1u : 0u;
+        }
+    }
+    const V beta = (V)bd;
+    store(first, acc0, beta);
+    for (uint32_t s0 = sp.s + R * sp.step; s0 < sp.e; s0 += R * sp.step) {
+        const SliceRows<R> rows = slice_rows<V, R>(M, s0, sp, lane);
+        V acc[R];
+        product(rows, acc);
+        store(rows, acc, beta);
     }
 }
 
 // x64 += delta32; delta32 = 0
-__global__ __launch_bounds__(kCgThreads) void merge_delta_kernel(uint32_t n, double* __restrict__ x,
+template <int NT>
+__global__ __launch_bounds__(NT) void merge_delta_kernel(uint32_t n, double* __restrict__ x,
                                                                  float* __restrict__ delta) {
     CG_ROWS(i) {
         x[i] += (double)delta[i];
@@ -708,7 +882,8 @@ __global__ __launch_bounds__(kCgThreads) void merge_delta_kernel(uint32_t n, dou
     }
 }
 
-__global__ __launch_bounds__(kCgThreads) void true_residual_kernel(uint32_t n, const uint32_t* __restrict__ row,
+template <int NT>
+__global__ __launch_bounds__(NT) void true_residual_kernel(uint32_t n, const uint32_t* __restrict__ row,
                                                                    const uint32_t* __restrict__ col,
                                                                    const double* __restrict__ val,
                                                                    const double* __restrict__ b,
@@ -721,7 +896,7 @@ __global__ __launch_bounds__(kCgThreads) void true_residual_kernel(uint32_t n, c
         v[1] += b[i] * b[i];
     }
     double tot[2];
-    if (!grid_fold<2>(v, partials, counter, tot)) return;
+    if (!grid_fold<2, NT>(v, partials, counter, tot)) return;
     if (threadIdx.x == 0) {
         out2[0] = tot[0];
         out2[1] = tot[1];
@@ -793,13 +968,13 @@ template <class V> CgVecs<V> carve(CgWork& w, uint32_t n) {
 // updates go to x itself (V = double) or to an FP32 correction merged into
 // x at each residual replacement (V = float).
 // ---------------------------------------------------------------------------
-template <class V, bool DIFF, int PREC, int CH, int R>
+template <class V, bool DIFF, int PREC, int CH, int R, int NT>
 void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const LowerFactor<V>* L, const double* b,
               double* x, const CgOptions& o, CgWork& w, SolveStats& st, hipStream_t s) {
     using XV = V;
     const uint32_t n = A64.n;
-    w.resize(n);
-    const dim3 grid(w.blocks), blk(kCgThreads);
+    w.resize(n, NT);
+    const dim3 grid(w.blocks), blk(NT);
     CgVecs<V> vv = carve<V>(w, n);
     XV* xacc;
     if constexpr (sizeof(V) == 4) {
@@ -832,20 +1007,20 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
         V2<V>* cur = vv.P[it & 1];
         if constexpr (PREC == kPrecLLt) {
             V2<V>* rqc = vv.RQ[it & 1];
-            hipLaunchKernelGGL((cg_residual_kernel<V, PREC>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p,
+            hipLaunchKernelGGL((cg_residual_kernel<V, PREC, NT>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p,
                                b, x, reinterpret_cast<V*>(rqc), 2u, cur, idiag, w.sc.p, w.partials.p, w.counter.p,
                                mode, rtol2);
-            hipLaunchKernelGGL((llt_update_kernel<V, XV, CH, R>), grid, blk, 0, s, MU, xacc, vv.D[it & 1], rqc, rqc,
-                               vv.w, w.sc.p, w.partials.p, w.counter.p, mode == 0 ? 1 : 2);
-            hipLaunchKernelGGL((llt_lo_kernel<V, CH, R>), grid, blk, 0, s, ML, vv.w, vv.D[(it + 1) & 1], vv.D[it & 1],
-                               w.sc.p, mode == 0 ? 1 : 2);
+            hipLaunchKernelGGL((llt_update_kernel<V, XV, CH, R, NT>), grid, blk, 0, s, MU, xacc, vv.D[it & 1], rqc, rqc,
+                               vv.w, w.sc.p, w.partials.p, mode == 0 ? 1 : 2);
+            hipLaunchKernelGGL((llt_lo_kernel<V, CH, R, NT>), grid, blk, 0, s, ML, vv.w, vv.D[(it + 1) & 1], vv.D[it & 1],
+                               w.sc.p, w.partials.p, mode == 0 ? 1 : 2);
         } else {
-            hipLaunchKernelGGL((cg_residual_kernel<V, PREC>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p,
+            hipLaunchKernelGGL((cg_residual_kernel<V, PREC, NT>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p,
                                b, x, vv.r, 1u, cur, idiag, w.sc.p, w.partials.p, w.counter.p, mode, rtol2);
         }
     };
     auto merge = [&] {
-        if constexpr (sizeof(V) == 4) hipLaunchKernelGGL(merge_delta_kernel, grid, blk, 0, s, n, x, vv.delta);
+        if constexpr (sizeof(V) == 4) hipLaunchKernelGGL(merge_delta_kernel<NT>, grid, blk, 0, s, n, x, vv.delta);
     };
     // one batch of o.batch iterations starting at iteration it0 (kernels
     // only: a captured batch contains no memset or copy nodes)
@@ -858,16 +1033,16 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
                 V2<V>* rqc = vv.RQ[it & 1];
                 V2<V>* rqn = vv.RQ[(it + 1) & 1];
                 V* dc = vv.D[it & 1];
-                hipLaunchKernelGGL((llt_dir_kernel<V, DIFF, CH, R>), grid, blk, 0, s, MA, dc,
-                                   reinterpret_cast<V*>(rqc) + 1, 2u, w.sc.p, w.partials.p, w.counter.p);
-                hipLaunchKernelGGL((llt_update_kernel<V, XV, CH, R>), grid, blk, 0, s, MU, xacc, dc, rqc, rqn, vv.w,
-                                   w.sc.p, w.partials.p, w.counter.p, 0);
-                hipLaunchKernelGGL((llt_lo_kernel<V, CH, R>), grid, blk, 0, s, ML, vv.w, dc, vv.D[(it + 1) & 1], w.sc.p,
-                                   0);
+                hipLaunchKernelGGL((llt_dir_kernel<V, DIFF, CH, R, NT>), grid, blk, 0, s, MA, dc,
+                                   reinterpret_cast<V*>(rqc) + 1, 2u, w.sc.p, w.partials.p);
+                hipLaunchKernelGGL((llt_update_kernel<V, XV, CH, R, NT>), grid, blk, 0, s, MU, xacc, dc, rqc, rqn, vv.w,
+                                   w.sc.p, w.partials.p, 0);
+                hipLaunchKernelGGL((llt_lo_kernel<V, CH, R, NT>), grid, blk, 0, s, ML, vv.w, dc, vv.D[(it + 1) & 1], w.sc.p,
+                                   w.partials.p, 0);
             } else {
-                hipLaunchKernelGGL((cg_dir_kernel<V, DIFF, CH, R>), grid, blk, 0, s, MA, cur, nxt, vv.q, 1u, w.sc.p,
+                hipLaunchKernelGGL((cg_dir_kernel<V, DIFF, CH, R, NT>), grid, blk, 0, s, MA, cur, nxt, vv.q, 1u, w.sc.p,
                                    w.partials.p, w.counter.p);
-                hipLaunchKernelGGL((cg_update_kernel<V, XV, PREC>), grid, blk, 0, s, n, xacc, vv.r, nxt, vv.q, idiag,
+                hipLaunchKernelGGL((cg_update_kernel<V, XV, PREC, NT>), grid, blk, 0, s, n, xacc, vv.r, nxt, vv.q, idiag,
                                    w.sc.p, w.partials.p, w.counter.p);
             }
         }
@@ -923,7 +1098,7 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
     }
     merge();
     double tr[2];
-    hipLaunchKernelGGL(true_residual_kernel, grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p, b, x, w.partials.p,
+    hipLaunchKernelGGL(true_residual_kernel<NT>, grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p, b, x, w.partials.p,
                        w.counter.p, w.out2.p);
     d2h(tr, w.out2.p, 2);
     SA_HIP(hipEventRecord(e1, s));
@@ -946,31 +1121,44 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
 void true_residual(const DevCsr<double>& A, const double* b, const double* x, double* out2, CgWork& w,
                    hipStream_t s) {
     w.resize(A.n);
-    hipLaunchKernelGGL(true_residual_kernel, dim3(w.blocks), dim3(kCgThreads), 0, s, A.n, A.row.p, A.col.p, A.val.p,
+    hipLaunchKernelGGL(true_residual_kernel<kCgThreads>, dim3(w.blocks), dim3(kCgThreads), 0, s, A.n, A.row.p, A.col.p, A.val.p,
                        b, x, w.partials.p, w.counter.p, w.out2.p);
     d2h(out2, w.out2.p, 2);
 }
 
 // SELL entries loaded per step of the row loop (all loads of a step are
-// issued before their gathers) and slices per wavefront step.
-template <class V, int CH, int R>
+// issued before their gathers); block size of the L L^T kernels.
+template <class V, bool DIFF, int CH>
+void cg_solve_llt(const DevCsr<double>& A64, const Sell<V>& S, const LowerFactor<V>* L, const double* b, double* x,
+                  const CgOptions& o, CgWork& w, SolveStats& st, hipStream_t s) {
+    const uint32_t nt = tuning().llt_threads;
+    if (nt == 256)
+        cg_drive<V, DIFF, kPrecLLt, CH, 1, 256>(A64, S, nullptr, L, b, x, o, w, st, s);
+    else if (nt == 512)
+        cg_drive<V, DIFF, kPrecLLt, CH, 1, 512>(A64, S, nullptr, L, b, x, o, w, st, s);
+    else
+        cg_drive<V, DIFF, kPrecLLt, CH, 1, 1024>(A64, S, nullptr, L, b, x, o, w, st, s);
+}
+
+template <class V, int CH>
 void cg_solve_ch(const DevCsr<double>& A64, const Sell<V>& S, bool diff, int32_t prec, const V* idiag,
                  const LowerFactor<V>* L, const double* b, double* x, const CgOptions& o, CgWork& w,
                  SolveStats& st, hipStream_t s) {
+    constexpr int NT = kCgThreads;
     if (diff) {
         if (prec == kPrecLLt)
-            cg_drive<V, true, kPrecLLt, CH, R>(A64, S, idiag, L, b, x, o, w, st, s);
+            cg_solve_llt<V, true, CH>(A64, S, L, b, x, o, w, st, s);
         else if (prec == kPrecJacobi)
-            cg_drive<V, true, kPrecJacobi, CH, R>(A64, S, idiag, L, b, x, o, w, st, s);
+            cg_drive<V, true, kPrecJacobi, CH, 1, NT>(A64, S, idiag, L, b, x, o, w, st, s);
         else
-            cg_drive<V, true, kPrecNone, CH, R>(A64, S, idiag, L, b, x, o, w, st, s);
+            cg_drive<V, true, kPrecNone, CH, 1, NT>(A64, S, idiag, L, b, x, o, w, st, s);
     } else {
         if (prec == kPrecLLt)
-            cg_drive<V, false, kPrecLLt, CH, R>(A64, S, idiag, L, b, x, o, w, st, s);
+            cg_solve_llt<V, false, CH>(A64, S, L, b, x, o, w, st, s);
         else if (prec == kPrecJacobi)
-            cg_drive<V, false, kPrecJacobi, CH, R>(A64, S, idiag, L, b, x, o, w, st, s);
+            cg_drive<V, false, kPrecJacobi, CH, 1, NT>(A64, S, idiag, L, b, x, o, w, st, s);
         else
-            cg_drive<V, false, kPrecNone, CH, R>(A64, S, idiag, L, b, x, o, w, st, s);
+            cg_drive<V, false, kPrecNone, CH, 1, NT>(A64, S, idiag, L, b, x, o, w, st, s);
     }
 }
 
@@ -980,15 +1168,10 @@ void cg_solve(const DevCsr<double>& A64, const Sell<V>& S, bool diff, int32_t pr
               hipStream_t s) {
     if (prec == kPrecLLt && !L) throw std::invalid_argument("L L^T preconditioner requested without a factor");
     if (prec == kPrecJacobi && !idiag) throw std::invalid_argument("Jacobi preconditioner requested without a diagonal");
-    const bool c4 = tuning().sell_chunk == 4, r2 = tuning().sell_rows == 2;
-    if (c4 && r2)
-        cg_solve_ch<V, 4, 2>(A64, S, diff, prec, idiag, L, b, x, o, w, st, s);
-    else if (c4)
-        cg_solve_ch<V, 4, 1>(A64, S, diff, prec, idiag, L, b, x, o, w, st, s);
-    else if (r2)
-        cg_solve_ch<V, 8, 2>(A64, S, diff, prec, idiag, L, b, x, o, w, st, s);
+    if (tuning().sell_chunk == 4)
+        cg_solve_ch<V, 4>(A64, S, diff, prec, idiag, L, b, x, o, w, st, s);
     else
-        cg_solve_ch<V, 8, 1>(A64, S, diff, prec, idiag, L, b, x, o, w, st, s);
+        cg_solve_ch<V, 8>(A64, S, diff, prec, idiag, L, b, x, o, w, st, s);
 }
 template void cg_solve<float>(const DevCsr<double>&, const Sell<float>&, bool, int32_t, const float*,
                               const LowerFactor<float>*, const double*, double*, const CgOptions&, CgWork&,
@@ -997,25 +1180,26 @@ template void cg_solve<double>(const DevCsr<double>&, const Sell<double>&, bool,
                                const LowerFactor<double>*, const double*, double*, const CgOptions&, CgWork&,
                                SolveStats&, hipStream_t);
 
-// One launch of a direction kernel on scratch vectors (ping-pong by k).
-template <class V, int CH, int R>
+// One launch of a direction kernel on scratch vectors (ping-pong by k):
+// the L L^T path's with NT-thread blocks, or the Jacobi path's.
+template <class V, int CH, int NT>
 void dir_launch(const SellArgs<V>& M, bool diff, bool llt, CgVecs<V>& vv, CgWork& w, uint32_t k, hipStream_t s) {
-    const dim3 grid(w.blocks), blk(kCgThreads);
+    const dim3 grid(w.blocks), blk(w.threads);
     V2<V>* cur = vv.P[k & 1];
     V2<V>* nxt = vv.P[(k + 1) & 1];
     V* q = reinterpret_cast<V*>(vv.RQ[k & 1]) + 1;
     if (llt && diff)
-        hipLaunchKernelGGL((llt_dir_kernel<V, true, CH, R>), grid, blk, 0, s, M, vv.D[k & 1], q, 2u, w.sc.p,
-                           w.partials.p, w.counter.p);
+        hipLaunchKernelGGL((llt_dir_kernel<V, true, CH, 1, NT>), grid, blk, 0, s, M, vv.D[k & 1], q, 2u, w.sc.p,
+                           w.partials.p);
     else if (llt)
-        hipLaunchKernelGGL((llt_dir_kernel<V, false, CH, R>), grid, blk, 0, s, M, vv.D[k & 1], q, 2u, w.sc.p,
-                           w.partials.p, w.counter.p);
+        hipLaunchKernelGGL((llt_dir_kernel<V, false, CH, 1, NT>), grid, blk, 0, s, M, vv.D[k & 1], q, 2u, w.sc.p,
+                           w.partials.p);
     else if (diff)
-        hipLaunchKernelGGL((cg_dir_kernel<V, true, CH, R>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
-                           w.partials.p, w.counter.p);
+        hipLaunchKernelGGL((cg_dir_kernel<V, true, CH, 1, kCgThreads>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u,
+                           w.sc.p, w.partials.p, w.counter.p);
     else
-        hipLaunchKernelGGL((cg_dir_kernel<V, false, CH, R>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u, w.sc.p,
-                           w.partials.p, w.counter.p);
+        hipLaunchKernelGGL((cg_dir_kernel<V, false, CH, 1, kCgThreads>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u,
+                           w.sc.p, w.partials.p, w.counter.p);
 }
 
 // Average duration (ms, HIP events on the context stream) of a CG
@@ -1026,7 +1210,7 @@ void dir_launch(const SellArgs<V>& M, bool diff, bool llt, CgVecs<V>& vv, CgWork
 template <class V>
 double time_dir_kernel(const Sell<V>& S, bool diff, bool llt, uint32_t reps, CgWork& w, hipStream_t s) {
     const uint32_t n = S.n;
-    w.resize(n);
+    w.resize(n, llt ? tuning().llt_threads : (uint32_t)kCgThreads);
     CgVecs<V> vv = carve<V>(w, n);
     CgScalars h{};
     h.maxit = 0xffffffffu;
@@ -1035,16 +1219,21 @@ double time_dir_kernel(const Sell<V>& S, bool diff, bool llt, uint32_t reps, CgW
     const size_t nv = ((size_t)n + 63) & ~(size_t)63;
     fill<V>(reinterpret_cast<V*>(vv.P[0]), 10 * nv, (V)1, s);  // P, RQ and D
     const SellArgs<V> M = sell_args(S);
-    const bool c4 = tuning().sell_chunk == 4, r2 = tuning().sell_rows == 2;
+    const bool c4 = tuning().sell_chunk == 4;
+    const uint32_t nt = w.threads;
     auto launch = [&](uint32_t k) {
-        if (c4 && r2)
-            dir_launch<V, 4, 2>(M, diff, llt, vv, w, k, s);
+        if (c4 && nt == 256)
+            dir_launch<V, 4, 256>(M, diff, llt, vv, w, k, s);
+        else if (c4 && nt == 512)
+            dir_launch<V, 4, 512>(M, diff, llt, vv, w, k, s);
         else if (c4)
-            dir_launch<V, 4, 1>(M, diff, llt, vv, w, k, s);
-        else if (r2)
-            dir_launch<V, 8, 2>(M, diff, llt, vv, w, k, s);
+            dir_launch<V, 4, 1024>(M, diff, llt, vv, w, k, s);
+        else if (nt == 256)
+            dir_launch<V, 8, 256>(M, diff, llt, vv, w, k, s);
+        else if (nt == 512)
+            dir_launch<V, 8, 512>(M, diff, llt, vv, w, k, s);
         else
-            dir_launch<V, 8, 1>(M, diff, llt, vv, w, k, s);
+            dir_launch<V, 8, 1024>(M, diff, llt, vv, w, k, s);
     };
     for (uint32_t k = 0; k < 3; ++k) launch(k);
     hipEvent_t e0, e1;

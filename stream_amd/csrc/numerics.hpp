@@ -97,7 +97,9 @@ struct CgWork {
     DBuf<double> out2;       // true-residual norms
     DBuf<uint8_t> vbuf;      // solver vectors (grow-only)
     uint32_t blocks = 0;
-    void resize(uint32_t n);
+    uint32_t threads = 256;  // block size of the last resize
+    // grid for n rows with blocks of nthreads (256, 512 or 1024)
+    void resize(uint32_t n, uint32_t nthreads = 256);
 };
 
 // Preconditioner kind of the fused solvers.
@@ -134,12 +136,12 @@ struct CgOptions {
 struct Tuning {
     double replace_factor = 7e-1;  // mixed: residual replacement when ||r|| fell by this factor
     uint32_t batch = 32;           // CG iterations between host checks
-    uint32_t cg_blocks = 2048;     // grid cap of the CG kernels (64..4096)
+    uint32_t cg_blocks = 2048;     // grid cap of the CG kernels in 256-thread blocks (64..4096)
     uint32_t graph = 0;            // 1: replay each batch of iterations as a captured HIP graph
     uint32_t xcd_map = 1;          // SELL slices: 0 grid stride, 1 XCD-contiguous chunks per wave
     uint32_t fold_group = 32;      // blocks per group of the two-level grid fold (0: one level)
     uint32_t sell_chunk = 8;       // SELL entries per load step of the CG row loops (4 or 8)
-    uint32_t sell_rows = 1;        // SELL slices a wavefront works on at once (1 or 2)
+    uint32_t llt_threads = 512;    // block size of the L L^T path's kernels (256, 512 or 1024)
 };
 Tuning& tuning();
 // Group size of the two-level grid fold of the CG kernels (0, 8, 16, 32, 64).

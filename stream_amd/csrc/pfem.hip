@@ -732,8 +732,8 @@ int32_t SolverCG_solve_mixed(SolverCG* sol, const LinSys* sys, const PrecJacobi_
 // default 32), "cg_blocks" (grid cap of the CG kernels), "graph" (replay
 // each batch as a captured HIP graph), "xcd_map" (SELL slice mapping),
 // "fold_group" (two-level reduction group size, 0 = one level),
-// "sell_chunk" (SELL entries per load step, 4 or 8) and "sell_rows" (SELL
-// slices a wavefront works on at once, 1 or 2).
+// "sell_chunk" (SELL entries per load step, 4 or 8) and "llt_threads"
+// (block size of the L L^T path's kernels, 256, 512 or 1024).
 int32_t Stream_tune(const char* key, double value) {
     return g([&] {
         const std::string k(key ? key : "");
@@ -752,9 +752,10 @@ int32_t Stream_tune(const char* key, double value) {
         } else if (k == "sell_chunk") {
             if (!(value == 4.0 || value == 8.0)) throw std::invalid_argument("sell_chunk must be 4 or 8");
             tuning().sell_chunk = (uint32_t)value;
-        } else if (k == "sell_rows") {
-            if (!(value == 1.0 || value == 2.0)) throw std::invalid_argument("sell_rows must be 1 or 2");
-            tuning().sell_rows = (uint32_t)value;
+        } else if (k == "llt_threads") {
+            if (!(value == 256.0 || value == 512.0 || value == 1024.0))
+                throw std::invalid_argument("llt_threads must be 256, 512 or 1024");
+            tuning().llt_threads = (uint32_t)value;
         } else if (k == "fold_group") {
             set_fold_group((uint32_t)value);
         } else if (k == "xcd_map") {
