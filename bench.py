@@ -103,6 +103,18 @@ def sell_dir_bytes(n, nnz, value_bytes, llt=False):
     return n * row + (nnz - n) * (2 + vb) + nslices * 6
 
 
+def sell_update_bytes(n, nnz):
+    """Algorithmic bytes of one L L^T update launch (x += alpha d, r' = r -
+    alpha q, w = L^T r', partials of w.w and r'.r') on the upper SELL of the
+    factor: per row the length (2 B), L_ii (4 B), the own {r, q} pair (8 B),
+    d (4 B), the FP32 x correction read and written (8 B), r' and w written
+    (8 B); per strict-upper entry (nnz - n) / 2 its 16-bit column offset and
+    FP32 value (6 B); per 64-row slice its offset and width (6 B).  Each
+    vector entry is counted once (neighbour re-reads are cache traffic)."""
+    nslices = (n + 63) // 64
+    return n * 34 + (nnz - n) // 2 * 6 + nslices * 6
+
+
 def tank3d_leg(steps=2):
     """The 3D step on the sloshing-tank cloud (1,996,920 FP64 points,
     alpha = 1.3h): LBVH + alpha-shape tetrahedra + P1 + CG to rtol 1e-8,
@@ -151,6 +163,7 @@ def components(step, pts, al, net):
         out[f"step_{solver}"] = {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items()}
     out["dir_kernel_fp32_ms"] = round(step.time_dir_kernel(fp32=True, reps=200), 5)
     out["dir_kernel_llt_ms"] = round(step.time_dir_kernel(fp32=True, reps=200, llt=True), 5)
+    out["update_kernel_llt_ms"] = round(step.time_update_kernel(reps=200), 5)
     out["dir_kernel_fp64_ms"] = round(step.time_dir_kernel(fp32=False, reps=200), 5)
     xy = np.ascontiguousarray(pts, np.float32)
     rad = np.full(len(xy), np.float32(0.5) * np.float32(al[0]), np.float32)
@@ -326,7 +339,10 @@ def main():
         import ctypes
 
         r = step.run(args.solver, rtol=1e-8)
-        kms = step.time_dir_kernel(fp32=args.solver != "fp64", reps=20, llt=args.solver in ("learned", "fsai"))
+        llt = args.solver in ("learned", "fsai")
+        kms = step.time_dir_kernel(fp32=args.solver != "fp64", reps=20, llt=llt)
+        if llt:
+            step.time_update_kernel(reps=20)
         cms = ctypes.c_double(0)
         stream_amd.check(stream_amd.lib.Stream_copy_calibration(ctypes.c_uint64(COPY_BYTES), 5, ctypes.byref(cms)))
         print(f"[profile] n={pts.shape[0]} nnz={r['nnz']} dir kernel {kms * 1000:.2f} us, "
@@ -347,21 +363,31 @@ def main():
                     solve_ms=inf64[-1]["solve_ms"])
         print(f"[bench] fp64: {ms64:.2f} ms/step iters {fp64['iters']}", file=sys.stderr, flush=True)
 
-    # dominant kernel: the CG direction kernel (SpMV + d.q; on the Jacobi
-    # paths also d = z + beta d_old)
+    # dominant kernel: on the L L^T path the update kernel (update + L^T
+    # apply), reported beside the direction kernel; on the Jacobi paths the
+    # direction kernel (d = z + beta d_old, SpMV, d.q)
     fp32 = args.solver != "fp64"
     llt = args.solver in ("learned", "fsai")
-    kms = step.time_dir_kernel(fp32=fp32, reps=200, llt=llt)
     n = pts.shape[0]
     nnz = last["nnz"]
-    alg_bytes = sell_dir_bytes(n, nnz, 4 if fp32 else 8, llt=llt)
-    achieved = alg_bytes / (kms * 1e-3) / 1e9
-    kname = "llt_dir_kernel" if llt else "cg_dir_kernel"
-    traffic = measured_traffic(kname, n, nnz, fp32)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": f"{kname}<{'float, true' if fp32 else 'double, false'}>", "kernel_ms": round(kms, 5),
-                "alg_bytes_per_launch": int(alg_bytes)}
+
+    def roof(kname, kms, alg_bytes, traffic):
+        achieved = alg_bytes / (kms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
+                "kernel_ms": round(kms, 5), "alg_bytes_per_launch": int(alg_bytes)}
+
+    kms = step.time_dir_kernel(fp32=fp32, reps=200, llt=llt)
+    dname = "llt_dir_kernel" if llt else "cg_dir_kernel"
+    dir_roof = roof(f"{dname}<{'float, true' if fp32 else 'double, false'}>", kms,
+                    sell_dir_bytes(n, nnz, 4 if fp32 else 8, llt=llt), measured_traffic(dname, n, nnz, fp32))
+    if llt:
+        ums = step.time_update_kernel(reps=200)
+        roofline = roof("llt_update_kernel<float, float>", ums, sell_update_bytes(n, nnz),
+                        measured_traffic("llt_update_kernel", n, nnz, fp32))
+        roofline["direction_kernel"] = dir_roof
+    else:
+        roofline = dir_roof
 
     mfma = None
     if net is not None:

@@ -118,9 +118,11 @@ int32_t PfemStep2D_get_solution_morton(PfemStep2D* step, double* p);
 uint32_t PfemStep2D_nnz(PfemStep2D* step);
 int32_t PfemStep2D_get_system(PfemStep2D* step, uint32_t* row, uint32_t* col, double* val, double* b);
 int32_t PfemStep2D_get_mesh(PfemStep2D* step, uint32_t* elem, uint8_t* node_is_bnd, double* xy, uint32_t* perm);
-/* Average ms of a CG direction kernel on the last step's system (HIP
- * events on the library stream, `reps` launches): kind 0 = FP64 Jacobi
- * path, 1 = FP32 row-sum Jacobi path, 2 = FP32 row-sum L L^T path. */
+/* Average ms of a CG kernel on the last step's system (HIP events on the
+ * library stream, `reps` launches): direction kernel of kind 0 = FP64
+ * Jacobi path, 1 = FP32 row-sum Jacobi path, 2 = FP32 row-sum L L^T path;
+ * kind 3 = the L L^T update kernel on the learned factor (FSAI without a
+ * network). */
 int32_t PfemStep2D_time_dir_kernel(PfemStep2D* step, int32_t kind, uint32_t reps, double* ms);
 
 #ifdef __cplusplus

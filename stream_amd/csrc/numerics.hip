@@ -1252,4 +1252,61 @@ double time_dir_kernel(const Sell<V>& S, bool diff, bool llt, uint32_t reps, CgW
 template double time_dir_kernel<float>(const Sell<float>&, bool, bool, uint32_t, CgWork&, hipStream_t);
 template double time_dir_kernel<double>(const Sell<double>&, bool, bool, uint32_t, CgWork&, hipStream_t);
 
+// Average duration (ms) of the L L^T update kernel (x += alpha d,
+// r' = r - alpha q, w = L^T r', partials of w.w and r'.r') over `reps`
+// launches on factor L with the configured block size.  The vectors are
+// scratch (ones) and the direction kernel's partials ones, so alpha = 1/G.
+template <class V, int CH, int NT>
+void update_launches(const LowerFactor<V>& L, uint32_t reps, CgVecs<V>& vv, CgWork& w, hipStream_t s) {
+    static_assert(sizeof(V) == 4, "the mixed path's update accumulates x in the FP32 correction");
+    SellArgs<V> MU = sell_args(L.up);
+    MU.diag = L.diag.p;
+    for (uint32_t k = 0; k < reps; ++k)
+        hipLaunchKernelGGL((llt_update_kernel<V, float, CH, 1, NT>), dim3(w.blocks), dim3(NT), 0, s, MU, vv.delta,
+                           vv.D[k & 1], vv.RQ[k & 1], vv.RQ[(k + 1) & 1], vv.w, w.sc.p, w.partials.p, 0);
+}
+template <class V> double time_llt_update_kernel(const LowerFactor<V>& L, uint32_t reps, CgWork& w, hipStream_t s) {
+    const uint32_t n = L.n;
+    w.resize(n, tuning().llt_threads);
+    CgVecs<V> vv = carve<V>(w, n);
+    CgScalars h{};
+    h.maxit = 0xffffffffu;
+    h.rz = 1.0;
+    SA_HIP(hipMemcpyAsync(w.sc.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    const size_t nv = ((size_t)n + 63) & ~(size_t)63;
+    fill<V>(reinterpret_cast<V*>(vv.P[0]), 10 * nv + 3 * nv, (V)1, s);  // P, RQ, D, r, q, w
+    fill<float>(vv.delta, n, 0.0f, s);
+    fill<double>(w.partials.p, kMaxCgBlocks, 1.0, s);
+    const bool c4 = tuning().sell_chunk == 4;
+    const uint32_t nt = w.threads;
+    auto launch = [&](uint32_t r) {
+        if (c4 && nt == 256)
+            update_launches<V, 4, 256>(L, r, vv, w, s);
+        else if (c4 && nt == 512)
+            update_launches<V, 4, 512>(L, r, vv, w, s);
+        else if (c4)
+            update_launches<V, 4, 1024>(L, r, vv, w, s);
+        else if (nt == 256)
+            update_launches<V, 8, 256>(L, r, vv, w, s);
+        else if (nt == 512)
+            update_launches<V, 8, 512>(L, r, vv, w, s);
+        else
+            update_launches<V, 8, 1024>(L, r, vv, w, s);
+    };
+    launch(3);
+    hipEvent_t e0, e1;
+    SA_HIP(hipEventCreate(&e0));
+    SA_HIP(hipEventCreate(&e1));
+    SA_HIP(hipEventRecord(e0, s));
+    launch(reps);
+    SA_HIP(hipEventRecord(e1, s));
+    SA_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SA_HIP(hipEventElapsedTime(&ms, e0, e1));
+    SA_HIP(hipEventDestroy(e0));
+    SA_HIP(hipEventDestroy(e1));
+    return reps ? (double)ms / reps : 0.0;
+}
+template double time_llt_update_kernel<float>(const LowerFactor<float>&, uint32_t, CgWork&, hipStream_t);
+
 }  // namespace stream_amd

@@ -114,6 +114,13 @@ class PfemStep2D:
         check(PfemStep2D_time_dir_kernel(self.h, 2 if llt else int(fp32), reps, ctypes.byref(ms)))
         return ms.value
 
+    def time_update_kernel(self, reps: int = 200) -> float:
+        """Average ms of the L L^T update kernel (x += alpha d, r' = r - alpha
+        q, w = L^T r') on the learned factor (FSAI without a network)."""
+        ms = ctypes.c_double(0.0)
+        check(PfemStep2D_time_dir_kernel(self.h, 3, reps, ctypes.byref(ms)))
+        return ms.value
+
     def solution(self, morton=False):
         p = np.empty(self.n)
         check((PfemStep2D_get_solution_morton if morton else PfemStep2D_get_solution)(self.h, p))
