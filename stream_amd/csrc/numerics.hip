@@ -758,11 +758,26 @@ __global__ __launch_bounds__(NT) void llt_update_kernel(SellArgs<V> U, XV* __res
                                                                 V2<V>* __restrict__ rq_nxt, V* __restrict__ w,
                                                                 CgScalars* sc, double* partials, int mode) {
     if (mode == 0 && sc->done) return;
-    const SellArgs<V>& M = U;  // slice loop
+    const SellArgs<V>& M = U;
+    const bool it = mode == 0;
+    // the direction kernel's partials and the first slice group's own data
+    // are loaded before the partials' sum (alpha) is taken
+    PartialsPrefetch pf{};
+    if (it) pf = prefetch_partials<1, NT>(partials, kMaxCgBlocks, gridDim.x);
+    const uint32_t lane = threadIdx.x & 63u;
+    const SliceSpan sp = slice_span(M.nslices, M.map);
+    const SliceRows<R> first = slice_rows<V, R>(M, sp.s, sp, lane);
+    V2<V> own0[R];
+    V diag0[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        own0[r] = rq[first.ii[r]];
+        diag0[r] = U.diag[first.ii[r]];
+    }
     double alpha = 0.0;
-    if (mode == 0) {
+    if (it) {
         double dq[1];
-        sum_partials<1, NT>(partials, kMaxCgBlocks, gridDim.x, dq);
+        sum_prefetched<1, NT>(pf, partials, kMaxCgBlocks, gridDim.x, dq);
         const double rz = sc->rz;
         alpha = rz / dq[0];
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -772,16 +787,13 @@ __global__ __launch_bounds__(NT) void llt_update_kernel(SellArgs<V> U, XV* __res
         }
     }
     const V av = (V)alpha;
-    const bool it = mode == 0;
-    const uint32_t lane = threadIdx.x & 63u;
     double v[2] = {0.0, 0.0};
-    SELL_FOR_SLICE_ROWS(R, rows) {
+    auto body = [&](const SliceRows<R>& rows, const V2<V> (&own)[R], const V (&dg)[R]) {
         V ri[R], acc[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const V2<V> own = rq[rows.ii[r]];
-            ri[r] = it ? own.x - av * own.y : own.x;
-            acc[r] = U.diag[rows.ii[r]] * ri[r];
+            ri[r] = it ? own[r].x - av * own[r].y : own[r].x;
+            acc[r] = dg[r] * ri[r];
         }
         sell_rows_product<V, R, CH, V2<V>>(
             U, rows, acc, [&](uint32_t c) { return rq[c]; },
@@ -798,6 +810,18 @@ __global__ __launch_bounds__(NT) void llt_update_kernel(SellArgs<V> U, XV* __res
                 v[0] += (double)acc[r] * (double)acc[r];
                 v[1] += (double)ri[r] * (double)ri[r];
             }
+    };
+    body(first, own0, diag0);
+    for (uint32_t s0 = sp.s + R * sp.step; s0 < sp.e; s0 += R * sp.step) {
+        const SliceRows<R> rows = slice_rows<V, R>(M, s0, sp, lane);
+        V2<V> own[R];
+        V dg[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            own[r] = rq[rows.ii[r]];
+            dg[r] = U.diag[rows.ii[r]];
+        }
+        body(rows, own, dg);
     }
     store_partials<2, NT>(v, partials + kMaxCgBlocks, kMaxCgBlocks);
 }

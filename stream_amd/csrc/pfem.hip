@@ -235,6 +235,7 @@ struct Step2D {
     CgWork work;
     SolveStats last;
     double t_mesh = 0, t_asm = 0, t_prep = 0, t_solve = 0;
+    bool solved = false;  // x holds the solution of the current system
     uint32_t n = 0;
     PrecLearned* learned = nullptr;  // network of solver 3 (not owned)
     bool learned_fallback = false;   // the last learned factor was rejected
@@ -269,6 +270,9 @@ struct Step2D {
 
     void set_points(uint32_t n_, const double* h_xy, const double* h_alpha, const double* h_f) {
         n = n_;
+        A.n = 0;  // the assembled system and solution belong to the previous points
+        A.nnz = 0;
+        solved = false;
         hipStream_t s = ctx().stream;
         DBuf<double> dxy, da;
         dxy.resize(2ull * n);
@@ -310,6 +314,7 @@ struct Step2D {
     }
 
     void assemble(hipStream_t s) {
+        solved = false;
         const uint32_t cap = mesh.cap_edges + n;  // nnz <= edges + n
         A.n = n;
         A.row.resize(n + 1);
@@ -393,9 +398,20 @@ struct Step2D {
             throw std::invalid_argument("unknown solver kind");
         }
         t_solve = last.ms;
+        solved = true;
+    }
+
+    // the system of the current points (mesh + assembly done)
+    void require_system() const {
+        if (!A.nnz || A.n != n) throw std::logic_error("no assembled system for the current points (run a step first)");
+    }
+    void require_solution() const {
+        require_system();
+        if (!solved) throw std::logic_error("no solution for the current system (run a step first)");
     }
 
     void solution_input_order(double* h_p) {
+        require_solution();
         hipStream_t s = ctx().stream;
         p_out.resize(n);
         hipLaunchKernelGGL((permute_bwd_kernel<double>), dim3(grid_for(n, 256)), dim3(256), 0, s, x.p,
@@ -919,6 +935,7 @@ int32_t PfemStep2D_run(PfemStep2D* h, int32_t solver, double rtol, uint32_t maxi
 int32_t PfemStep2D_time_dir_kernel(PfemStep2D* h, int32_t kind, uint32_t reps, double* ms) {
     return g([&] {
         if (kind < 0 || kind > 3) throw std::invalid_argument("time_dir_kernel: kind must be 0, 1, 2 or 3");
+        h->s.require_system();
         CgWork w;
         if (kind == 3) {
             h->s.prepare(h->s.learned ? 3 : 2, ctx().stream);
@@ -939,17 +956,26 @@ int32_t PfemStep2D_get_solution(PfemStep2D* h, double* p) {
 }
 
 // Solution in Morton order [N].
-int32_t PfemStep2D_get_solution_morton(PfemStep2D* h, double* p) { return g([&] { d2h(p, h->s.x.p, h->s.n); }); }
+int32_t PfemStep2D_get_solution_morton(PfemStep2D* h, double* p) {
+    return g([&] {
+        h->s.require_solution();
+        d2h(p, h->s.x.p, h->s.n);
+    });
+}
 
 uint32_t PfemStep2D_nnz(PfemStep2D* h) {
     uint32_t r = 0;
-    (void)g([&] { r = read_scalar(h->s.A.row.p + h->s.n); });
+    (void)g([&] {
+        h->s.require_system();
+        r = read_scalar(h->s.A.row.p + h->s.n);
+    });
     return r;
 }
 
 // Assembled system in Morton order: row [N+1], col/val [nnz], b [N].
 int32_t PfemStep2D_get_system(PfemStep2D* h, uint32_t* row, uint32_t* col, double* val, double* b) {
     return g([&] {
+        h->s.require_system();
         const uint32_t n = h->s.n;
         const uint32_t nnz = read_scalar(h->s.A.row.p + n);
         if (row) d2h(row, h->s.A.row.p, n + 1);

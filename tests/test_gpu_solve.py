@@ -88,6 +88,29 @@ def test_step_solve_true_residual_and_direct_error(case, solver):
     assert np.array_equal(p[perm], x)
 
 
+def test_step_entry_points_need_an_assembled_system():
+    """Kernel timing, the system and the solution are refused (host-side
+    error, nothing launched) until a step has meshed and assembled the
+    current points, and again after new points are set."""
+    import stream_amd
+    from stream_amd.pfem import PfemStep2D
+
+    _say("entry points before a step")
+    pts, al = clouds.jittered_square(32, seed=1)
+    f = np.ones(pts.shape[0])
+    st = PfemStep2D(pts, al, f)
+    for call in (lambda: st.time_dir_kernel(True, 2), lambda: st.time_dir_kernel(True, 2, llt=True),
+                 lambda: st.time_update_kernel(2), lambda: st.solution(), lambda: st.system()):
+        with pytest.raises(stream_amd.StreamError):
+            call()
+    st.run("fsai", rtol=1e-8)
+    assert st.time_dir_kernel(True, 2, llt=True) > 0 and st.time_update_kernel(2) > 0
+    assert st.solution().shape == (pts.shape[0],)
+    st.set_points(pts, al, f)
+    with pytest.raises(stream_amd.StreamError):
+        st.time_update_kernel(2)
+
+
 def test_fp64_and_oracle_cg_agree():
     from stream_amd.pfem import PfemStep2D
 
