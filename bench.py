@@ -107,12 +107,12 @@ def sell_update_bytes(n, nnz):
     """Algorithmic bytes of one L L^T update launch (x += alpha d, r' = r -
     alpha q, w = L^T r', partials of w.w and r'.r') on the upper SELL of the
     factor: per row the length (2 B), L_ii (4 B), the own {r, q} pair (8 B),
-    d (4 B), the FP32 x correction read and written (8 B), r' and w written
-    (8 B); per strict-upper entry (nnz - n) / 2 its 16-bit column offset and
-    FP32 value (6 B); per 64-row slice its offset and width (6 B).  Each
-    vector entry is counted once (neighbour re-reads are cache traffic)."""
+    d (4 B), the FP64 x read and written (16 B), r' and w written (8 B); per
+    strict-upper entry (nnz - n) / 2 its 16-bit column offset and FP32 value
+    (6 B); per 64-row slice its offset and width (6 B).  Each vector entry
+    is counted once (neighbour re-reads are cache traffic)."""
     nslices = (n + 63) // 64
-    return n * 34 + (nnz - n) // 2 * 6 + nslices * 6
+    return n * 42 + (nnz - n) // 2 * 6 + nslices * 6
 
 
 def tank3d_leg(steps=2):
@@ -383,7 +383,7 @@ def main():
                     sell_dir_bytes(n, nnz, 4 if fp32 else 8, llt=llt), measured_traffic(dname, n, nnz, fp32))
     if llt:
         ums = step.time_update_kernel(reps=200)
-        roofline = roof("llt_update_kernel<float, float>", ums, sell_update_bytes(n, nnz),
+        roofline = roof("llt_update_kernel<float, double>", ums, sell_update_bytes(n, nnz),
                         measured_traffic("llt_update_kernel", n, nnz, fp32))
         roofline["direction_kernel"] = dir_roof
     else:

@@ -57,7 +57,7 @@ int32_t SolverCG_solve_mixed(SolverCG* solver, const LinSys* sys, const PrecJaco
                              uint32_t maxit, double* stats);
 
 /* Process-wide solver knobs: "replace_factor" (mixed precision residual
- * replacement threshold, default 0.7), "batch" (iterations between host
+ * replacement threshold, default 0.1), "batch" (iterations between host
  * checks, default 32), "cg_blocks" (grid cap of the CG kernels, 64..4096,
  * default 2048), "graph" (1: replay each batch as a captured HIP graph,
  * default 0), "xcd_map" (SELL slices in XCD-contiguous chunks, default 1)

@@ -896,16 +896,6 @@ __global__ __launch_bounds__(NT) void llt_lo_kernel(SellArgs<V> M, const V* __re
     }
 }
 
-// x64 += delta32; delta32 = 0
-template <int NT>
-__global__ __launch_bounds__(NT) void merge_delta_kernel(uint32_t n, double* __restrict__ x,
-                                                                 float* __restrict__ delta) {
-    CG_ROWS(i) {
-        x[i] += (double)delta[i];
-        delta[i] = 0.0f;
-    }
-}
-
 template <int NT>
 __global__ __launch_bounds__(NT) void true_residual_kernel(uint32_t n, const uint32_t* __restrict__ row,
                                                                    const uint32_t* __restrict__ col,
@@ -958,11 +948,10 @@ template <class V> struct CgVecs {
     V* r;
     V* q;
     V* w;
-    float* delta;
 };
 template <class V> CgVecs<V> carve(CgWork& w, uint32_t n) {
     const size_t nv = ((size_t)n + 63) & ~(size_t)63;
-    w.vbuf.resize(nv * (4 * sizeof(V2<V>) + 5 * sizeof(V) + sizeof(float)) + 256);
+    w.vbuf.resize(nv * (4 * sizeof(V2<V>) + 5 * sizeof(V)) + 256);
     uint8_t* p = w.vbuf.p;
     CgVecs<V> v;
     v.P[0] = reinterpret_cast<V2<V>*>(p);
@@ -983,30 +972,26 @@ template <class V> CgVecs<V> carve(CgWork& w, uint32_t n) {
     p += nv * sizeof(V);
     v.w = reinterpret_cast<V*>(p);
     p += nv * sizeof(V);
-    v.delta = reinterpret_cast<float*>(p);
+
     return v;
 }
 
 // ---------------------------------------------------------------------------
-// Driver.  V: iteration precision (matrix values, vectors); the solution
-// updates go to x itself (V = double) or to an FP32 correction merged into
-// x at each residual replacement (V = float).
+// Driver.  V: iteration precision (matrix values, vectors).  The solution
+// updates x += alpha d go to the FP64 x itself in every precision: an FP32
+// correction merged at the residual replacements carries rounding noise that
+// is rough on the mesh scale, which A amplifies against the smooth
+// right-hand side (16% more iterations on the 1M dam-break).
 // ---------------------------------------------------------------------------
 template <class V, bool DIFF, int PREC, int CH, int R, int NT>
 void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const LowerFactor<V>* L, const double* b,
               double* x, const CgOptions& o, CgWork& w, SolveStats& st, hipStream_t s) {
-    using XV = V;
+    using XV = double;
     const uint32_t n = A64.n;
     w.resize(n, NT);
     const dim3 grid(w.blocks), blk(NT);
     CgVecs<V> vv = carve<V>(w, n);
-    XV* xacc;
-    if constexpr (sizeof(V) == 4) {
-        fill<float>(vv.delta, n, 0.0f, s);
-        xacc = (XV*)vv.delta;
-    } else {
-        xacc = (XV*)x;
-    }
+    XV* xacc = x;
     const SellArgs<V> MA = sell_args(S);
     SellArgs<V> ML{}, MU{};
     if constexpr (PREC == kPrecLLt) {
@@ -1042,9 +1027,6 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
             hipLaunchKernelGGL((cg_residual_kernel<V, PREC, NT>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p,
                                b, x, vv.r, 1u, cur, idiag, w.sc.p, w.partials.p, w.counter.p, mode, rtol2);
         }
-    };
-    auto merge = [&] {
-        if constexpr (sizeof(V) == 4) hipLaunchKernelGGL(merge_delta_kernel<NT>, grid, blk, 0, s, n, x, vv.delta);
     };
     // one batch of o.batch iterations starting at iteration it0 (kernels
     // only: a captured batch contains no memset or copy nodes)
@@ -1086,7 +1068,6 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
             if (h.iter >= o.maxit || !o.verify) break;
             // the recursive residual says converged: verify with the FP64
             // true residual, which also replaces the recursive one
-            merge();
             residual(1, h.iter);
             ++reps;
             h = read_sc(w, s);
@@ -1113,14 +1094,12 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
         }
         h = read_sc(w, s);
         if (!h.done && o.replace_factor > 0 && h.rr < o.replace_factor * o.replace_factor * rr_last_rep) {
-            merge();
             residual(1, h.iter);
             ++reps;
             h = read_sc(w, s);
             rr_last_rep = h.aux;
         }
     }
-    merge();
     double tr[2];
     hipLaunchKernelGGL(true_residual_kernel<NT>, grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p, b, x, w.partials.p,
                        w.counter.p, w.out2.p);
@@ -1281,12 +1260,11 @@ template double time_dir_kernel<double>(const Sell<double>&, bool, bool, uint32_
 // launches on factor L with the configured block size.  The vectors are
 // scratch (ones) and the direction kernel's partials ones, so alpha = 1/G.
 template <class V, int CH, int NT>
-void update_launches(const LowerFactor<V>& L, uint32_t reps, CgVecs<V>& vv, CgWork& w, hipStream_t s) {
-    static_assert(sizeof(V) == 4, "the mixed path's update accumulates x in the FP32 correction");
+void update_launches(const LowerFactor<V>& L, uint32_t reps, CgVecs<V>& vv, double* xs, CgWork& w, hipStream_t s) {
     SellArgs<V> MU = sell_args(L.up);
     MU.diag = L.diag.p;
     for (uint32_t k = 0; k < reps; ++k)
-        hipLaunchKernelGGL((llt_update_kernel<V, float, CH, 1, NT>), dim3(w.blocks), dim3(NT), 0, s, MU, vv.delta,
+        hipLaunchKernelGGL((llt_update_kernel<V, double, CH, 1, NT>), dim3(w.blocks), dim3(NT), 0, s, MU, xs,
                            vv.D[k & 1], vv.RQ[k & 1], vv.RQ[(k + 1) & 1], vv.w, w.sc.p, w.partials.p, 0);
 }
 template <class V> double time_llt_update_kernel(const LowerFactor<V>& L, uint32_t reps, CgWork& w, hipStream_t s) {
@@ -1299,23 +1277,25 @@ template <class V> double time_llt_update_kernel(const LowerFactor<V>& L, uint32
     SA_HIP(hipMemcpyAsync(w.sc.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
     const size_t nv = ((size_t)n + 63) & ~(size_t)63;
     fill<V>(reinterpret_cast<V*>(vv.P[0]), 10 * nv + 3 * nv, (V)1, s);  // P, RQ, D, r, q, w
-    fill<float>(vv.delta, n, 0.0f, s);
+    DBuf<double> xs;  // scratch solution
+    xs.resize(n);
+    fill<double>(xs.p, n, 0.0, s);
     fill<double>(w.partials.p, kMaxCgBlocks, 1.0, s);
     const bool c4 = tuning().sell_chunk == 4;
     const uint32_t nt = w.threads;
     auto launch = [&](uint32_t r) {
         if (c4 && nt == 256)
-            update_launches<V, 4, 256>(L, r, vv, w, s);
+            update_launches<V, 4, 256>(L, r, vv, xs.p, w, s);
         else if (c4 && nt == 512)
-            update_launches<V, 4, 512>(L, r, vv, w, s);
+            update_launches<V, 4, 512>(L, r, vv, xs.p, w, s);
         else if (c4)
-            update_launches<V, 4, 1024>(L, r, vv, w, s);
+            update_launches<V, 4, 1024>(L, r, vv, xs.p, w, s);
         else if (nt == 256)
-            update_launches<V, 8, 256>(L, r, vv, w, s);
+            update_launches<V, 8, 256>(L, r, vv, xs.p, w, s);
         else if (nt == 512)
-            update_launches<V, 8, 512>(L, r, vv, w, s);
+            update_launches<V, 8, 512>(L, r, vv, xs.p, w, s);
         else
-            update_launches<V, 8, 1024>(L, r, vv, w, s);
+            update_launches<V, 8, 1024>(L, r, vv, xs.p, w, s);
     };
     launch(3);
     hipEvent_t e0, e1;

@@ -134,7 +134,7 @@ struct CgOptions {
 
 // Process-wide solver knobs (set through Stream_tune).
 struct Tuning {
-    double replace_factor = 7e-1;  // mixed: residual replacement when ||r|| fell by this factor
+    double replace_factor = 1e-1;  // mixed: residual replacement when ||r|| fell by this factor
     uint32_t batch = 32;           // CG iterations between host checks
     uint32_t cg_blocks = 2048;     // grid cap of the CG kernels in 256-thread blocks (64..4096)
     uint32_t graph = 0;            // 1: replay each batch of iterations as a captured HIP graph
@@ -148,9 +148,8 @@ Tuning& tuning();
 void set_fold_group(uint32_t g);
 
 // CG on SELL matrix S (values V) with vectors in precision V.  A64 (CSR,
-// FP64) gives the true residuals.  x: FP64 solution (in/out).  With V =
-// float the updates accumulate in an FP32 correction merged into x at
-// every residual replacement.  diff: S holds the row-sum form.
+// FP64) gives the true residuals.  x: FP64 solution (in/out), updated in
+// FP64 every iteration.  diff: S holds the row-sum form.
 template <class V>
 void cg_solve(const DevCsr<double>& A64, const Sell<V>& S, bool diff, int32_t prec, const V* idiag,
               const LowerFactor<V>* L, const double* b, double* x, const CgOptions& o, CgWork& w, SolveStats& st,

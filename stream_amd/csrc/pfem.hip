@@ -744,7 +744,7 @@ int32_t SolverCG_solve_mixed(SolverCG* sol, const LinSys* sys, const PrecJacobi_
 
 // Process-wide solver knobs: "replace_factor" (mixed precision: replace
 // the recursive residual by the FP64 true residual whenever its norm fell
-// by this factor, default 0.7), "batch" (CG iterations between host checks,
+// by this factor, default 0.1), "batch" (CG iterations between host checks,
 // default 32), "cg_blocks" (grid cap of the CG kernels), "graph" (replay
 // each batch as a captured HIP graph), "xcd_map" (SELL slice mapping),
 // "fold_group" (two-level reduction group size, 0 = one level),
