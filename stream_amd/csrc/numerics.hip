@@ -18,6 +18,12 @@ constexpr uint32_t kMaxCgBlocks = 4096;
 constexpr uint32_t kMaxFoldGroups = kMaxCgBlocks / 8;  // smallest fold group: 8 blocks
 constexpr uint32_t kCounterStride = 32;                // one 128-byte line per ticket counter
 constexpr uint32_t kCounterWords = (1 + kMaxFoldGroups) * kCounterStride;
+// Block partials handed from one kernel to the next (store_partials /
+// sum_partials): the L L^T path's d.q at slot 0 and w.w, r'.r' at slots
+// 1, 2 (stride kMaxCgBlocks); the Jacobi path's d.q after the grid folds'
+// area, which the update kernel's fold must not overwrite while other
+// blocks of it still read them.
+constexpr size_t kJacobiDqSlot = 4ull * kMaxCgBlocks + 4ull * kMaxFoldGroups;
 
 Tuning& tuning() {
     static Tuning t;
@@ -38,7 +44,7 @@ void CgWork::resize(uint32_t n, uint32_t nthreads) {
     if (blocks == 0) blocks = 1;
     sc.resize(1);
     out2.resize(2);
-    partials.resize(4ull * kMaxCgBlocks + 4ull * kMaxFoldGroups);
+    partials.resize(kJacobiDqSlot + kMaxCgBlocks);
     if (!counter.p) {
         counter.resize(kCounterWords);
         fill<uint32_t>(counter.p, kCounterWords, 0u, ctx().stream);
@@ -614,7 +620,8 @@ __global__ __launch_bounds__(NT) void cg_residual_kernel(
     }
 }
 
-// d = z + beta d_old; q = A d; partial d.q.
+// d = z + beta d_old; q = A d; the block partials of d.q go to the update
+// kernel, which derives alpha from them.
 // DIFF: the values are in row-sum form (off-diagonals K_ij, diagonal slot
 // holding the row sum s_i) and q_i = s_i d_i + sum_j K_ij (d_j - d_i).  For
 // a graph Laplacian the differences of neighbouring entries of a smooth
@@ -624,7 +631,7 @@ __global__ __launch_bounds__(NT) void cg_residual_kernel(
 template <class V, bool DIFF, int CH, int R, int NT>
 __global__ __launch_bounds__(NT) void cg_dir_kernel(SellArgs<V> M, const V2<V>* __restrict__ cur,
                                                             V2<V>* __restrict__ nxt, V* __restrict__ q, uint32_t qs,
-                                                            CgScalars* sc, double* partials, uint32_t* counter) {
+                                                            const CgScalars* sc, double* partials) {
     if (sc->done) return;
     const V beta = (V)sc->beta;
     const uint32_t lane = threadIdx.x & 63u;
@@ -655,12 +662,7 @@ __global__ __launch_bounds__(NT) void cg_dir_kernel(SellArgs<V> M, const V2<V>* 
                 v[0] += (double)dn[r] * (double)acc[r];
             }
     }
-    double tot[1];
-    if (!grid_fold<1, NT>(v, partials, counter, tot)) return;
-    if (threadIdx.x == 0) {
-        sc->dq = tot[0];
-        sc->alpha = sc->rz / tot[0];
-    }
+    store_partials<1, NT>(v, partials + kJacobiDqSlot, kMaxCgBlocks);
 }
 
 // Direction kernel of the L L^T path: d is already formed (by the lower
@@ -709,7 +711,15 @@ __global__ __launch_bounds__(NT) void cg_update_kernel(uint32_t n, XV* __restric
                                                                const V* __restrict__ idiag, CgScalars* sc,
                                                                double* partials, uint32_t* counter) {
     if (sc->done) return;
-    const double alpha = sc->alpha;
+    // alpha = r.z / d.q from the direction kernel's block partials, summed
+    // by every block (block 0 records it)
+    double dq[1];
+    sum_partials<1, NT>(partials + kJacobiDqSlot, kMaxCgBlocks, gridDim.x, dq);
+    const double alpha = sc->rz / dq[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->dq = dq[0];
+        sc->alpha = alpha;
+    }
     const V av = (V)alpha;
     double v[2] = {0.0, 0.0};
     CG_ROWS(i) {
@@ -1047,7 +1057,7 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
                                    w.partials.p, 0);
             } else {
                 hipLaunchKernelGGL((cg_dir_kernel<V, DIFF, CH, R, NT>), grid, blk, 0, s, MA, cur, nxt, vv.q, 1u, w.sc.p,
-                                   w.partials.p, w.counter.p);
+                                   w.partials.p);
                 hipLaunchKernelGGL((cg_update_kernel<V, XV, PREC, NT>), grid, blk, 0, s, n, xacc, vv.r, nxt, vv.q, idiag,
                                    w.sc.p, w.partials.p, w.counter.p);
             }
@@ -1199,10 +1209,10 @@ void dir_launch(const SellArgs<V>& M, bool diff, bool llt, CgVecs<V>& vv, CgWork
                            w.partials.p);
     else if (diff)
         hipLaunchKernelGGL((cg_dir_kernel<V, true, CH, 1, kCgThreads>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u,
-                           w.sc.p, w.partials.p, w.counter.p);
+                           w.sc.p, w.partials.p);
     else
         hipLaunchKernelGGL((cg_dir_kernel<V, false, CH, 1, kCgThreads>), grid, blk, 0, s, M, cur, nxt, vv.q, 1u,
-                           w.sc.p, w.partials.p, w.counter.p);
+                           w.sc.p, w.partials.p);
 }
 
 // Average duration (ms, HIP events on the context stream) of a CG
