@@ -399,6 +399,7 @@ __global__ __launch_bounds__(256) void enc_bwd_kernel(uint32_t n, const float* _
 // ---------------------------------------------------------------------------
 // Messages (one wave per node, lane = feature)
 // ---------------------------------------------------------------------------
+constexpr uint32_t kMsgBatch = 8;  // edges whose loads are issued together
 
 __global__ __launch_bounds__(256) void msg_fwd_kernel(uint32_t n, const uint32_t* __restrict__ row,
                                                       const uint32_t* __restrict__ col,
@@ -414,13 +415,29 @@ __global__ __launch_bounds__(256) void msg_fwd_kernel(uint32_t n, const uint32_t
         const uint32_t i = uniform(i0);
         const float p = pq[(size_t)i * kPq + f];
         float acc = 0.f;
-        for (uint32_t e = row[i]; e < row[i + 1]; ++e) {
-            const uint32_t j = col[e];
-            if (j == i) continue;
-            const float2 ef = efeat[e];
-            const float et = ef.x * v0 + ef.y * v1 + (j > i ? v2 : -v2);
-            const float z = p + pq[(size_t)j * kPq + kHid + f] + et + b;
-            acc += fmaxf(z, 0.f);
+        const uint32_t e1 = row[i + 1];
+        // the row's columns, edge features and neighbour rows of up to kMsgBatch
+        // edges are loaded before any is used; the adds keep the edge order
+        for (uint32_t eb = row[i]; eb < e1; eb += kMsgBatch) {
+            uint32_t jj[kMsgBatch];
+            float2 ef[kMsgBatch];
+            float qj[kMsgBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kMsgBatch; ++u) {
+                const bool ok = eb + u < e1;
+                jj[u] = ok ? col[eb + u] : i;
+                ef[u] = ok ? efeat[eb + u] : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kMsgBatch; ++u) qj[u] = pq[(size_t)jj[u] * kPq + kHid + f];
+#pragma unroll
+            for (uint32_t u = 0; u < kMsgBatch; ++u) {
+                const uint32_t j = jj[u];
+                if (j == i) continue;  // the diagonal and the padding
+                const float et = ef[u].x * v0 + ef[u].y * v1 + (j > i ? v2 : -v2);
+                const float z = p + qj[u] + et + b;
+                acc += fmaxf(z, 0.f);
+            }
         }
         const float d = deg[i];
         m[(size_t)i * kHid + f] = d > 0.f ? acc / d : 0.f;
@@ -510,13 +527,29 @@ __global__ __launch_bounds__(256) void head_fwd_kernel(uint32_t n, const uint32_
     for (uint32_t k0 = blockIdx.x * 4 + wave_id(); k0 < n; k0 += gridDim.x * 4) {
         const uint32_t k = uniform(k0);
         const float r = rsd[(size_t)k * kRsd + f];
-        for (uint32_t e = row[k]; e < row[k + 1]; ++e) {
-            const uint32_t j = col[e];
-            if (j <= k) continue;
-            const float2 ef = efeat[e];
-            const float z = r + rsd[(size_t)j * kRsd + kHid + f] + (ef.x * v0 + ef.y * v1 + v2) + bde;
-            const float o = wsumf(bfr(fmaxf(z, 0.f)) * wo);
-            if (f == 0) gval[e] = val[e] != 0.0 ? o : 0.f;
+        const uint32_t e1 = row[k + 1];
+        // loads of up to kMsgBatch edges are issued before any is used
+        for (uint32_t eb = row[k]; eb < e1; eb += kMsgBatch) {
+            uint32_t jj[kMsgBatch];
+            float2 ef[kMsgBatch];
+            float sj[kMsgBatch];
+            bool nz[kMsgBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kMsgBatch; ++u) {
+                const bool ok = eb + u < e1;
+                jj[u] = ok ? col[eb + u] : k;
+                ef[u] = ok ? efeat[eb + u] : make_float2(0.f, 0.f);
+                nz[u] = ok && val[eb + u] != 0.0;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kMsgBatch; ++u) sj[u] = rsd[(size_t)jj[u] * kRsd + kHid + f];
+#pragma unroll
+            for (uint32_t u = 0; u < kMsgBatch; ++u) {
+                if (jj[u] <= k) continue;  // lower part, diagonal and padding
+                const float z = r + sj[u] + (ef[u].x * v0 + ef[u].y * v1 + v2) + bde;
+                const float o = wsumf(bfr(fmaxf(z, 0.f)) * wo);
+                if (f == 0) gval[eb + u] = nz[u] ? o : 0.f;
+            }
         }
         const float zd = rsd[(size_t)k * kRsd + 2 * kHid + f] + bdd;
         const float o = wsumf(bfr(fmaxf(zd, 0.f)) * wd) + kDiagShift;
