@@ -134,6 +134,63 @@ def test_zero_head_learned_pcg_is_jacobi_pcg():
     assert np.linalg.norm(xl - xj) / np.linalg.norm(xj) < 1e-5
 
 
+def test_learned_pcg_matches_fp64_pcg_with_oracle_factor():
+    """End to end: the learned mixed solver of the device step (trained 2D
+    weights) against textbook FP64 PCG preconditioned by the factor of the
+    oracle's forward pass of the same weights on the same system (the
+    assembly is bit-exact; the factors differ by the bf16 rounding-boundary
+    flips only): iteration counts within 3% and the same solution."""
+    import os
+
+    import scipy.sparse as sp
+
+    from stream_amd.learned import PrecLearned
+    from stream_amd.pfem import PfemStep2D
+
+    wpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "stream_amd", "weights", "mpnn_dam2d")
+    w = PrecLearned.load_params(wpath)
+    pts, al = clouds.jittered_square(64, seed=0)
+    f = clouds.divergence_source(pts, clouds.fourier_modes(11))
+    st = PfemStep2D(pts, al, f)
+    st.set_learned(PrecLearned(w))
+    info = st.run("learned", rtol=1e-8)
+    assert not info["fallback"] and info["true_rel"] <= 1e-8
+    row, col, val, b = st.system()
+    r = ol.ashape2d(pts, al, bits=32)
+    orow, ocol, oval, ob = ol.assemble_p1(r["ordered"], r["elem"], r["node_is_bnd"], f[r["perm"]])
+    assert np.array_equal(row, orow) and np.array_equal(col, ocol) and np.array_equal(val, oval)
+    g = mpnn.Graph(row, col, val, r["ordered"], r["node_is_bnd"])
+    gup, gdiag, _ = mpnn.forward(mpnn.unflatten(w.astype(np.float64)), g, emu=True)
+    L = mpnn.factor_csr(g, gup, gdiag)
+    LT = L.T.tocsr()
+    n = len(row) - 1
+    A = sp.csr_matrix((val, col.astype(np.int64), row.astype(np.int64)), shape=(n, n))
+    x = np.zeros(n)
+    rr = b.copy()
+    z = L @ (LT @ rr)
+    d = z.copy()
+    rz = rr @ z
+    bb = b @ b
+    it_ref = None
+    for it in range(1, 20000):
+        q = A @ d
+        alpha = rz / (d @ q)
+        x += alpha * d
+        rr -= alpha * q
+        if rr @ rr <= 1e-16 * bb and (b - A @ x) @ (b - A @ x) <= 1e-16 * bb:
+            it_ref = it
+            break
+        z = L @ (LT @ rr)
+        rz2 = rr @ z
+        d = z + (rz2 / rz) * d
+        rz = rz2
+    xg = st.solution(morton=True)
+    print(f"learned: device {info['iters']} iterations, FP64 PCG with the oracle factor {it_ref}")
+    assert it_ref is not None
+    assert abs(info["iters"] - it_ref) <= max(5, (3 * it_ref) // 100)
+    assert np.linalg.norm(xg - x) / np.linalg.norm(x) < 1e-5
+
+
 def test_gradient_matches_oracle(small):
     from stream_amd.learned import PrecLearned
 

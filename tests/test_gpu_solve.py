@@ -127,6 +127,82 @@ def test_fp64_and_oracle_cg_agree():
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6
 
 
+def _fsai_factor(A):
+    """FSAI by definition, independently of the library: row i of G solves
+    A[S, S] g = e_1 on S = {i} U {j > i in the pattern of A}, scaled so that
+    (G A G^T)_ii = 1; M^-1 = G^T G."""
+    n = A.shape[0]
+    U = sp.triu(A, k=0).tocsr()
+    U.sort_indices()
+    sizes = np.diff(U.indptr)
+    rows_all, cols_all, vals_all = [], [], []
+    for k in np.unique(sizes):
+        rows = np.nonzero(sizes == k)[0]
+        idx = np.stack([U.indices[U.indptr[r]:U.indptr[r] + k] for r in rows])
+        assert np.all(idx[:, 0] == rows)
+        sub = np.empty((len(rows), k, k))
+        for a_ in range(k):
+            sub[:, a_, :] = np.asarray(A[np.repeat(idx[:, a_], k), idx.ravel()]).reshape(len(rows), k)
+        e = np.zeros((len(rows), k, 1))
+        e[:, 0, 0] = 1.0
+        g = np.linalg.solve(sub, e)[..., 0]
+        g /= np.sqrt(g[:, :1])
+        rows_all.append(np.repeat(rows, k))
+        cols_all.append(idx.ravel())
+        vals_all.append(g.ravel())
+    return sp.csr_matrix((np.concatenate(vals_all), (np.concatenate(rows_all), np.concatenate(cols_all))),
+                         shape=(n, n))
+
+
+def _pcg64(A, b, apply_m, rtol=1e-8, maxit=20000):
+    """Textbook FP64 PCG, stopping on ||r|| <= rtol ||b|| (true residual)."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = apply_m(r)
+    d = z.copy()
+    rz = r @ z
+    bb = b @ b
+    for it in range(1, maxit):
+        q = A @ d
+        alpha = rz / (d @ q)
+        x += alpha * d
+        r -= alpha * q
+        if r @ r <= rtol * rtol * bb:
+            r = b - A @ x
+            if r @ r <= rtol * rtol * bb:
+                return x, it
+        z = apply_m(r)
+        rz2 = r @ z
+        d = z + (rz2 / rz) * d
+        rz = rz2
+    raise AssertionError("FP64 PCG did not converge")
+
+
+def test_fsai_pcg_matches_fp64_reference():
+    """The mixed FSAI solver (FSAI factor on the device, FP32 L L^T
+    iteration, FP64 x and residual replacement) against FP64 PCG with an
+    FSAI factor built here from its definition: the same iteration count up
+    to rounding (the FP64 x update keeps the FP32 iteration at the FP64
+    count) and the same solution."""
+    from stream_amd.pfem import PfemStep2D
+
+    _say("fsai vs fp64 reference")
+    pts, al = clouds.dam_break_scaled(60_000)
+    f = clouds.divergence_source(pts, clouds.fourier_modes(11))
+    st = PfemStep2D(pts, al, f)
+    info = st.run("fsai", rtol=1e-8)
+    row, col, val, b = st.system()
+    A = _csr(row, col, val, pts.shape[0])
+    G = _fsai_factor(A)
+    GT = G.T.tocsr()
+    x_ref, it_ref = _pcg64(A, b, lambda r: GT @ (G @ r))
+    x = st.solution(morton=True)
+    print(f"fsai: device {info['iters']} iterations, FP64 reference {it_ref}")
+    assert abs(info["iters"] - it_ref) <= max(5, it_ref // 50)
+    assert info["true_rel"] <= 1e-8
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-5
+
+
 def test_reference_api_diagonal_kat():
     from stream_amd.numerics import DeviceCSR, HostCSR, LinSys, PrecJacobi, SolverCG
 
