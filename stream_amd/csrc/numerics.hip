@@ -321,18 +321,33 @@ __device__ void sum_prefetched(const PartialsPrefetch& f, const double* __restri
     finish_partials<K, NT>(a, tot);
 }
 
-template <class MV, class XV>
-__device__ __forceinline__ double row_dot(const uint32_t* __restrict__ row, const uint32_t* __restrict__ col,
-                                          const MV* __restrict__ val, const XV* __restrict__ x, uint32_t i) {
-    double s = 0.0;
-    const uint32_t e = row[i + 1];
-    for (uint32_t k = row[i]; k < e; ++k) s += (double)val[k] * (double)x[col[k]];
-    return s;
-}
 
 
 // rows of an NT-thread kernel, grid stride
 #define CG_ROWS(i) for (uint32_t i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT)
+
+// rows of an NT-thread kernel taken by groups of GS lanes (GS divides the
+// wavefront, so a group never straddles one); g = lane within the group
+#define CG_GROUP_ROWS(i, g, GS)                                                                   \
+    for (uint32_t _t = blockIdx.x * NT + threadIdx.x, g = _t % (GS), i = _t / (GS); i < n;       \
+         i += gridDim.x * (NT / (GS)))
+
+// FP64 (A x)_i of a CSR row by the GS lanes of a group: lane g adds entries
+// g, g + GS, ... in order, an xor butterfly combines the lanes (every lane
+// gets the same bits).  The lanes read consecutive entries, so a row's
+// columns and values arrive in one or two coalesced loads instead of one
+// load per entry from a lone lane.
+template <uint32_t GS>
+__device__ __forceinline__ double group_row_dot(const uint32_t* __restrict__ row, const uint32_t* __restrict__ col,
+                                                const double* __restrict__ val, const double* __restrict__ x,
+                                                uint32_t i, uint32_t g) {
+    double s = 0.0;
+    const uint32_t e = row[i + 1];
+    for (uint32_t k = row[i] + g; k < e; k += GS) s += val[k] * x[col[k]];
+#pragma unroll
+    for (uint32_t o = GS / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, GS);
+    return s;
+}
 
 template <class V> struct V2T;
 template <> struct V2T<float> {
@@ -577,15 +592,17 @@ __device__ __forceinline__ void sell_rows_product(const SellArgs<V>& M, const Sl
 //         below the true one, this degrades to a restart instead of scaling
 //         the old direction by r_true.z / r_rec.z.
 // ---------------------------------------------------------------------------
-template <class V, int PREC, int NT>
+template <class V, int PREC, int NT, uint32_t GS>
 __global__ __launch_bounds__(NT) void cg_residual_kernel(
     uint32_t n, const uint32_t* __restrict__ row, const uint32_t* __restrict__ col, const double* __restrict__ val,
     const double* __restrict__ b, const double* __restrict__ x, V* __restrict__ r, uint32_t rs,
     V2<V>* __restrict__ P, const V* __restrict__ idiag, CgScalars* sc, double* partials, uint32_t* counter, int mode,
     double rtol2) {
     double v[3] = {0.0, 0.0, 0.0};
-    CG_ROWS(i) {
-        const double ri = b[i] - row_dot(row, col, val, x, i);
+    CG_GROUP_ROWS(i, g, GS) {
+        const double ax = group_row_dot<GS>(row, col, val, x, i, g);
+        if (g != 0) continue;
+        const double ri = b[i] - ax;
         const V rv = (V)ri;
         r[(size_t)i * rs] = rv;
         if constexpr (PREC != kPrecLLt) {
@@ -906,7 +923,7 @@ __global__ __launch_bounds__(NT) void llt_lo_kernel(SellArgs<V> M, const V* __re
     }
 }
 
-template <int NT>
+template <int NT, uint32_t GS>
 __global__ __launch_bounds__(NT) void true_residual_kernel(uint32_t n, const uint32_t* __restrict__ row,
                                                                    const uint32_t* __restrict__ col,
                                                                    const double* __restrict__ val,
@@ -914,8 +931,10 @@ __global__ __launch_bounds__(NT) void true_residual_kernel(uint32_t n, const uin
                                                                    const double* __restrict__ x, double* partials,
                                                                    uint32_t* counter, double* out2) {
     double v[2] = {0.0, 0.0};
-    CG_ROWS(i) {
-        const double ri = b[i] - row_dot(row, col, val, x, i);
+    CG_GROUP_ROWS(i, g, GS) {
+        const double ax = group_row_dot<GS>(row, col, val, x, i, g);
+        if (g != 0) continue;
+        const double ri = b[i] - ax;
         v[0] += ri * ri;
         v[1] += b[i] * b[i];
     }
@@ -1022,20 +1041,28 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
     // RQ[it & 1], and leaves d and r for it + 1 in the other halves; a
     // replacement after iteration it - 1 rebuilds D[it & 1] from the
     // direction D[(it + 1) & 1] it was made from.
+    // FP64 residual rows by groups of 8 lanes (2D, ~7 entries per row) or
+    // 16 (3D, ~15)
+    const bool wide_rows = (uint64_t)A64.nnz > 8ull * n;
+    auto residual_kernel = [&](V* rp, uint32_t rs, V2<V>* cur, int mode) {
+        if (wide_rows)
+            hipLaunchKernelGGL((cg_residual_kernel<V, PREC, NT, 16>), grid, blk, 0, s, n, A64.row.p, A64.col.p,
+                               A64.val.p, b, x, rp, rs, cur, idiag, w.sc.p, w.partials.p, w.counter.p, mode, rtol2);
+        else
+            hipLaunchKernelGGL((cg_residual_kernel<V, PREC, NT, 8>), grid, blk, 0, s, n, A64.row.p, A64.col.p,
+                               A64.val.p, b, x, rp, rs, cur, idiag, w.sc.p, w.partials.p, w.counter.p, mode, rtol2);
+    };
     auto residual = [&](int mode, uint32_t it) {
         V2<V>* cur = vv.P[it & 1];
         if constexpr (PREC == kPrecLLt) {
             V2<V>* rqc = vv.RQ[it & 1];
-            hipLaunchKernelGGL((cg_residual_kernel<V, PREC, NT>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p,
-                               b, x, reinterpret_cast<V*>(rqc), 2u, cur, idiag, w.sc.p, w.partials.p, w.counter.p,
-                               mode, rtol2);
+            residual_kernel(reinterpret_cast<V*>(rqc), 2u, cur, mode);
             hipLaunchKernelGGL((llt_update_kernel<V, XV, CH, R, NT>), grid, blk, 0, s, MU, xacc, vv.D[it & 1], rqc, rqc,
                                vv.w, w.sc.p, w.partials.p, mode == 0 ? 1 : 2);
             hipLaunchKernelGGL((llt_lo_kernel<V, CH, R, NT>), grid, blk, 0, s, ML, vv.w, vv.D[(it + 1) & 1], vv.D[it & 1],
                                w.sc.p, w.partials.p, mode == 0 ? 1 : 2);
         } else {
-            hipLaunchKernelGGL((cg_residual_kernel<V, PREC, NT>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p,
-                               b, x, vv.r, 1u, cur, idiag, w.sc.p, w.partials.p, w.counter.p, mode, rtol2);
+            residual_kernel(vv.r, 1u, cur, mode);
         }
     };
     // one batch of o.batch iterations starting at iteration it0 (kernels
@@ -1111,8 +1138,12 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
         }
     }
     double tr[2];
-    hipLaunchKernelGGL(true_residual_kernel<NT>, grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p, b, x, w.partials.p,
-                       w.counter.p, w.out2.p);
+    if (wide_rows)
+        hipLaunchKernelGGL((true_residual_kernel<NT, 16>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p, b, x,
+                           w.partials.p, w.counter.p, w.out2.p);
+    else
+        hipLaunchKernelGGL((true_residual_kernel<NT, 8>), grid, blk, 0, s, n, A64.row.p, A64.col.p, A64.val.p, b, x,
+                           w.partials.p, w.counter.p, w.out2.p);
     d2h(tr, w.out2.p, 2);
     SA_HIP(hipEventRecord(e1, s));
     SA_HIP(hipEventSynchronize(e1));
@@ -1134,8 +1165,12 @@ void cg_drive(const DevCsr<double>& A64, const Sell<V>& S, const V* idiag, const
 void true_residual(const DevCsr<double>& A, const double* b, const double* x, double* out2, CgWork& w,
                    hipStream_t s) {
     w.resize(A.n);
-    hipLaunchKernelGGL(true_residual_kernel<kCgThreads>, dim3(w.blocks), dim3(kCgThreads), 0, s, A.n, A.row.p, A.col.p, A.val.p,
-                       b, x, w.partials.p, w.counter.p, w.out2.p);
+    if ((uint64_t)A.nnz > 8ull * A.n)
+        hipLaunchKernelGGL((true_residual_kernel<kCgThreads, 16>), dim3(w.blocks), dim3(kCgThreads), 0, s, A.n,
+                           A.row.p, A.col.p, A.val.p, b, x, w.partials.p, w.counter.p, w.out2.p);
+    else
+        hipLaunchKernelGGL((true_residual_kernel<kCgThreads, 8>), dim3(w.blocks), dim3(kCgThreads), 0, s, A.n,
+                           A.row.p, A.col.p, A.val.p, b, x, w.partials.p, w.counter.p, w.out2.p);
     d2h(out2, w.out2.p, 2);
 }
 
