@@ -58,11 +58,11 @@ __device__ __forceinline__ void sort4(uint32_t* v) {
 
 // Sort node i's incident elements and build its sorted unique column list
 // (i and the vertices of its elements) in tmp[3*toff[i] + i ...].
-__global__ __launch_bounds__(256) void tet_cols_kernel(uint32_t n, const uint32_t* __restrict__ elem,
-                                                       const uint32_t* __restrict__ toff, uint32_t* __restrict__ inc,
-                                                       uint32_t* __restrict__ tmp, uint32_t* __restrict__ ncol) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// Column set of row i (serial form, one thread): the incident tetrahedra in
+// ascending id order, then every vertex of them inserted into a sorted,
+// duplicate-free list that starts with i.
+__device__ void tet_cols_serial(uint32_t i, const uint32_t* __restrict__ elem, const uint32_t* __restrict__ toff,
+                                uint32_t* __restrict__ inc, uint32_t* __restrict__ tmp, uint32_t* __restrict__ ncol) {
     const uint32_t b0 = toff[i], b1 = toff[i + 1];
     for (uint32_t k = b0 + 1; k < b1; ++k) {
         const uint32_t x = inc[k];
@@ -97,6 +97,83 @@ __global__ __launch_bounds__(256) void tet_cols_kernel(uint32_t n, const uint32_
         }
     }
     ncol[i] = cnt;
+}
+
+// The same column sets, one wavefront per row: the (at most 64) incident
+// tetrahedra sit one per lane and are placed by their rank among the
+// row's ids; the 4 m vertices and i itself (at most 257 keys, 5 slots per
+// lane) are reduced to their distinct values by two passes over all keys
+// broadcast lane by lane: a key is kept if no earlier key is equal, and a
+// kept key goes to the position given by the number of kept keys below
+// it.  The output (sorted tetrahedra, sorted distinct columns, count)
+// equals the serial form's; rows with more than 64 incident tetrahedra
+// take the serial form on one lane.
+constexpr uint32_t kColSlots = 5;
+__global__ __launch_bounds__(256) void tet_cols_kernel(uint32_t n, const uint32_t* __restrict__ elem,
+                                                       const uint32_t* __restrict__ toff, uint32_t* __restrict__ inc,
+                                                       uint32_t* __restrict__ tmp, uint32_t* __restrict__ ncol) {
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t i0 = blockIdx.x * 4 + (threadIdx.x >> 6); i0 < n; i0 += gridDim.x * 4) {
+        const uint32_t i = __builtin_amdgcn_readfirstlane(i0);
+        const uint32_t b0 = toff[i], b1 = toff[i + 1];
+        const uint32_t m = b1 - b0;
+        if (m > 64) {
+            if (lane == 0) tet_cols_serial(i, elem, toff, inc, tmp, ncol);
+            continue;
+        }
+        const uint32_t t = lane < m ? inc[b0 + lane] : ~0u;
+        // rank of this lane's tetrahedron among the row's (ids are distinct)
+        uint32_t rt = 0;
+        for (uint32_t l = 0; l < m; ++l) rt += (uint32_t)__shfl((int)t, (int)l, 64) < t ? 1u : 0u;
+        // keys: position p = slot * 64 + lane; p < 4 m: vertex p % 4 of the
+        // tetrahedron on lane p / 4; p = 4 m: i itself
+        const uint32_t nk = 4 * m + 1;
+        uint32_t key[kColSlots];
+        bool first[kColSlots];
+#pragma unroll
+        for (uint32_t sl = 0; sl < kColSlots; ++sl) {
+            const uint32_t p = sl * 64 + lane;
+            const uint32_t tj = (uint32_t)__shfl((int)t, (int)((p >> 2) & 63u), 64);
+            key[sl] = p < 4 * m ? elem[4ull * tj + (p & 3u)] : (p == 4 * m ? i : ~0u);
+            first[sl] = p < nk;
+        }
+        if (lane < m) inc[b0 + rt] = t;  // every lane has read its tetrahedron
+        // pass 1: a key is kept when no earlier position holds the same value
+#pragma unroll
+        for (uint32_t S = 0; S < kColSlots; ++S) {
+            if (S * 64 >= nk) break;
+            const uint32_t lim = nk - S * 64 < 64 ? nk - S * 64 : 64;
+            for (uint32_t L = 0; L < lim; ++L) {
+                const uint32_t kP = (uint32_t)__shfl((int)key[S], (int)L, 64);
+                const uint32_t P = S * 64 + L;
+#pragma unroll
+                for (uint32_t sl = 0; sl < kColSlots; ++sl)
+                    if (kP == key[sl] && P < sl * 64 + lane) first[sl] = false;
+            }
+        }
+        // pass 2: a kept key's position = number of kept keys below it
+        uint32_t rank[kColSlots];
+#pragma unroll
+        for (uint32_t sl = 0; sl < kColSlots; ++sl) rank[sl] = 0;
+        uint32_t cnt = 0;
+#pragma unroll
+        for (uint32_t S = 0; S < kColSlots; ++S) {
+            if (S * 64 >= nk) break;
+            cnt += (uint32_t)__popcll(__ballot(first[S]));
+            const uint32_t lim = nk - S * 64 < 64 ? nk - S * 64 : 64;
+            for (uint32_t L = 0; L < lim; ++L) {
+                const uint32_t kP = (uint32_t)__shfl((int)key[S], (int)L, 64);
+                const bool fP = __shfl((int)first[S], (int)L, 64) != 0;
+#pragma unroll
+                for (uint32_t sl = 0; sl < kColSlots; ++sl) rank[sl] += (fP && kP < key[sl]) ? 1u : 0u;
+            }
+        }
+        uint32_t* c = tmp + 3ull * b0 + i;
+#pragma unroll
+        for (uint32_t sl = 0; sl < kColSlots; ++sl)
+            if (first[sl]) c[rank[sl]] = key[sl];
+        if (lane == 0) ncol[i] = cnt;
+    }
 }
 
 struct P1TetArgs {
@@ -315,7 +392,7 @@ struct Step3D {
         if (ne)
             hipLaunchKernelGGL(tet_fill_kernel, dim3(grid_for(ne, 256)), dim3(256), 0, s, ne, mesh.elemglob.p, toff.p,
                                tcur.p, inc.p);
-        hipLaunchKernelGGL(tet_cols_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, mesh.elemglob.p, toff.p,
+        hipLaunchKernelGGL(tet_cols_kernel, dim3(grid_for(n, 4)), dim3(256), 0, s, n, mesh.elemglob.p, toff.p,
                            inc.p, tmp.p, ncol.p);
         exclusive_scan_u32(ncol.p, A.row.p, n, scan_ws.p, s);
         A.nnz = read_scalar(A.row.p + n);
