@@ -115,10 +115,11 @@ def sell_update_bytes(n, nnz):
     return n * 42 + (nnz - n) // 2 * 6 + nslices * 6
 
 
-def tank3d_leg(steps=2):
+def tank3d_leg(steps=2, weights3d=None):
     """The 3D step on the sloshing-tank cloud (1,996,920 FP64 points,
     alpha = 1.3h): LBVH + alpha-shape tetrahedra + P1 + CG to rtol 1e-8,
-    mixed and FP64 Jacobi-PCG; steps/s and stage times of the last step."""
+    mixed and FP64 Jacobi-PCG (and the learned mixed PCG when 3D weights
+    are given); steps/s and stage times of the last step."""
     import numpy as np
 
     import stream_amd
@@ -129,7 +130,13 @@ def tank3d_leg(steps=2):
     f = np.sin(3.0 * pts[:, 0]) * np.cos(2.0 * pts[:, 1]) + pts[:, 2]
     st = PfemStep3D(pts, al, f)
     out = {"points": int(len(pts))}
-    for solver in ("mixed", "fp64"):
+    solvers = ("mixed", "fp64")
+    if weights3d:
+        from stream_amd.learned import PrecLearned
+
+        st.set_learned(PrecLearned(PrecLearned.load_params(weights3d)))
+        solvers += ("learned",)
+    for solver in solvers:
         st.run(solver, rtol=1e-8)
         stream_amd.synchronize()
         t0 = time.perf_counter()
@@ -139,8 +146,8 @@ def tank3d_leg(steps=2):
         ms = (time.perf_counter() - t0) * 1000.0 / steps
         out[solver] = {"steps_per_s": round(1000.0 / ms, 4), "ms_per_step": round(ms, 2),
                        "mesh_ms": round(r["mesh_ms"], 2), "assembly_ms": round(r["assemble_ms"], 2),
-                       "solve_ms": round(r["solve_ms"], 2), "iters": r["iters"], "true_rel": r["true_rel"],
-                       "tets": r["n_elems"], "nnz": r["nnz"]}
+                       "prep_ms": round(r["prep_ms"], 2), "solve_ms": round(r["solve_ms"], 2),
+                       "iters": r["iters"], "true_rel": r["true_rel"], "tets": r["n_elems"], "nnz": r["nnz"]}
         print(f"[bench] tank3d {solver}: {ms:.1f} ms/step {out[solver]}", file=sys.stderr, flush=True)
     del st
     return out
@@ -282,6 +289,8 @@ def main():
                          "advected in the seeded 3D divergence-free field; prints one JSON line")
     ap.add_argument("--seq-steps", type=int, default=50)
     ap.add_argument("--weights3d", default=WEIGHTS3D, help="weight prefix of the 3D learned solver")
+    ap.add_argument("--only-tank3d", action="store_true",
+                    help="time only the 2M 3D tank step (mixed, FP64 Jacobi and learned PCG) and print its JSON")
     ap.add_argument("--components", action="store_true",
                     help="time the path's components at 1M points (LBVH step, TRBVH, Mesh2D phases, P1, CG "
                          "kernels) and print them as one JSON line instead of the headline")
@@ -305,6 +314,9 @@ def main():
         sys.exit(1)
     if args.sequence3d:
         print(json.dumps(sequence3d(args.seq_steps, args.weights3d)), flush=True)
+        return
+    if args.only_tank3d:
+        print(json.dumps(tank3d_leg(args.steps, args.weights3d)), flush=True)
         return
     pts, al = clouds.dam_break()
     f = clouds.divergence_source(pts, clouds.fourier_modes(11))
