@@ -101,14 +101,14 @@ __device__ void tet_cols_serial(uint32_t i, const uint32_t* __restrict__ elem, c
 
 // The same column sets, one wavefront per row: the (at most 64) incident
 // tetrahedra sit one per lane and are placed by their rank among the
-// row's ids; the 4 m vertices and i itself (at most 257 keys, 5 slots per
-// lane) are reduced to their distinct values by two passes over all keys
-// broadcast lane by lane: a key is kept if no earlier key is equal, and a
+// row's ids; the 3 m vertices other than i and i itself (at most 193 keys,
+// 4 slots per lane) are reduced to their distinct values by two passes over
+// all keys broadcast lane by lane: a key is kept if no earlier key is equal, and a
 // kept key goes to the position given by the number of kept keys below
 // it.  The output (sorted tetrahedra, sorted distinct columns, count)
 // equals the serial form's; rows with more than 64 incident tetrahedra
 // take the serial form on one lane.
-constexpr uint32_t kColSlots = 5;
+constexpr uint32_t kColSlots = 4;
 __global__ __launch_bounds__(256) void tet_cols_kernel(uint32_t n, const uint32_t* __restrict__ elem,
                                                        const uint32_t* __restrict__ toff, uint32_t* __restrict__ inc,
                                                        uint32_t* __restrict__ tmp, uint32_t* __restrict__ ncol) {
@@ -125,16 +125,31 @@ __global__ __launch_bounds__(256) void tet_cols_kernel(uint32_t n, const uint32_
         // rank of this lane's tetrahedron among the row's (ids are distinct)
         uint32_t rt = 0;
         for (uint32_t l = 0; l < m; ++l) rt += (uint32_t)__shfl((int)t, (int)l, 64) < t ? 1u : 0u;
-        // keys: position p = slot * 64 + lane; p < 4 m: vertex p % 4 of the
-        // tetrahedron on lane p / 4; p = 4 m: i itself
-        const uint32_t nk = 4 * m + 1;
+        // keys: position p = slot * 64 + lane; p < 3 m: the (p % 3)-th vertex
+        // other than i of the tetrahedron on lane p / 3; p = 3 m: i itself
+        const uint32_t nk = 3 * m + 1;
         uint32_t key[kColSlots];
         bool first[kColSlots];
 #pragma unroll
         for (uint32_t sl = 0; sl < kColSlots; ++sl) {
             const uint32_t p = sl * 64 + lane;
-            const uint32_t tj = (uint32_t)__shfl((int)t, (int)((p >> 2) & 63u), 64);
-            key[sl] = p < 4 * m ? elem[4ull * tj + (p & 3u)] : (p == 4 * m ? i : ~0u);
+            const uint32_t j = p / 3, q = p - 3 * j;
+            const uint32_t tj = (uint32_t)__shfl((int)t, (int)(j & 63u), 64);
+            uint32_t kv = ~0u;
+            if (p < 3 * m) {
+                uint32_t c = 0;
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                    const uint32_t v = elem[4ull * tj + u];
+                    if (v != i) {
+                        if (c == q) kv = v;
+                        ++c;
+                    }
+                }
+            } else if (p == 3 * m) {
+                kv = i;
+            }
+            key[sl] = kv;
             first[sl] = p < nk;
         }
         if (lane < m) inc[b0 + rt] = t;  // every lane has read its tetrahedron
