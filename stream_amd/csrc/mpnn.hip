@@ -126,11 +126,13 @@ __global__ __launch_bounds__(256) void prep_rows_kernel(uint32_t n, const uint32
     if (threadIdx.x < kPrepVals) part[(size_t)blockIdx.x * kPrepVals + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// One wave: lane l folds blocks l, l + 64, ... in order, then an xor
+// butterfly combines the lanes (min/max exact, sums in a fixed order).
 __global__ __launch_bounds__(64) void prep_fold_kernel(uint32_t nblk, const double* __restrict__ part,
                                                        double* __restrict__ scal) {
-    if (threadIdx.x != 0) return;
+    const uint32_t l = threadIdx.x;
     double b[kPrepVals] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY, 0.0, 0.0};
-    for (uint32_t k = 0; k < nblk; ++k) {
+    for (uint32_t k = l; k < nblk; k += 64) {
         const double* p = part + (size_t)k * kPrepVals;
         for (int c = 0; c < 6; c += 2) {
             b[c] = fmin(b[c], p[c]);
@@ -139,7 +141,17 @@ __global__ __launch_bounds__(64) void prep_fold_kernel(uint32_t nblk, const doub
         b[6] += p[6];
         b[7] += p[7];
     }
-    for (int k = 0; k < kPrepVals; ++k) scal[k] = b[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        for (int c = 0; c < 6; c += 2) {
+            b[c] = fmin(b[c], __shfl_xor(b[c], o, 64));
+            b[c + 1] = fmax(b[c + 1], __shfl_xor(b[c + 1], o, 64));
+        }
+        b[6] += __shfl_xor(b[6], o, 64);
+        b[7] += __shfl_xor(b[7], o, 64);
+    }
+    if (l == 0)
+        for (int k = 0; k < kPrepVals; ++k) scal[k] = b[k];
 }
 
 // Node features, edge features (bf16-rounded) and the transpose map.
