@@ -636,14 +636,21 @@ __global__ __launch_bounds__(256) void loss_kernel(uint32_t n, const uint32_t* _
     }
 }
 
+// One wave: lane q sums blocks q, q + 64, ... in order, then an xor
+// butterfly combines the lanes (a fixed order: deterministic).
 __global__ __launch_bounds__(64) void loss_fold_kernel(uint32_t nblk, uint32_t n, const double* __restrict__ part,
                                                        double* __restrict__ lsc) {
-    if (threadIdx.x != 0) return;
     double t = 0.0, l = 0.0;
-    for (uint32_t b = 0; b < nblk; ++b) {
+    for (uint32_t b = threadIdx.x; b < nblk; b += 64) {
         t += part[(size_t)b * 2];
         l += part[(size_t)b * 2 + 1];
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        t += __shfl_xor(t, o, 64);
+        l += __shfl_xor(l, o, 64);
+    }
+    if (threadIdx.x != 0) return;
     lsc[0] = t;
     lsc[1] = l;
     lsc[2] = log(t / (double)n) - 2.0 / (double)n * l;
