@@ -38,6 +38,47 @@ template <> struct Consts<double> {
 };
 
 // --------------------------------------------------------------------------
+// Contracted evaluation of the reference's inexact arithmetic.  The reference
+// is built by nvcc, whose default (--fmad=true) may fuse a product that
+// feeds an addition or subtraction into one fma (one rounding instead of
+// two).  Its certified predicate signs do not depend on that, but three
+// inexact decisions do: the candidate test |c_i - c_j|^2 < (2 delta a)^2
+// (DirectAlphaShape2D.cpp:212-213, DirectAlphaShape3D.cpp:204-205), the
+// box-distance test of the BVH walk (2D.cpp:201-206, 3D.cpp:192-199) and the
+// circumradius filters (2D.cpp:554-563, 3D.cpp:611-645).  `mode` bits choose
+// which of these sites are evaluated fused; kFirstPair / kSubRight choose
+// which product of a two-product sum or difference is the fused one.  Mode 0
+// is the unfused statement-order evaluation.
+// --------------------------------------------------------------------------
+namespace fmac {
+enum : int { kLeaf = 1, kBox = 2, kFilter = 4, kFirstPair = 8, kSubRight = 16 };
+
+// x0*x0 + x1*x1 + ... + x(k-1)*x(k-1), summed left to right
+template <class S> inline S sumsq(int mode, bool site, const S* x, int k) {
+    if (!site) {
+        S s = x[0] * x[0];
+        for (int i = 1; i < k; ++i) s = s + x[i] * x[i];
+        return s;
+    }
+    S s = (mode & kFirstPair) ? std::fma(x[0], x[0], x[1] * x[1]) : std::fma(x[1], x[1], x[0] * x[0]);
+    for (int i = 2; i < k; ++i) s = std::fma(x[i], x[i], s);
+    return s;
+}
+// a*b - c*d
+template <class S> inline S diff(int mode, bool site, S a, S b, S c, S d) {
+    if (!site) return a * b - c * d;
+    return (mode & kSubRight) ? std::fma(-c, d, a * b) : std::fma(a, b, -(c * d));
+}
+// a*b + c*d
+template <class S> inline S dot2(int mode, bool site, S a, S b, S c, S d) {
+    if (!site) return a * b + c * d;
+    return (mode & kFirstPair) ? std::fma(a, b, c * d) : std::fma(c, d, a * b);
+}
+// s + a*b
+template <class S> inline S acc(bool site, S s, S a, S b) { return site ? std::fma(a, b, s) : s + a * b; }
+}  // namespace fmac
+
+// --------------------------------------------------------------------------
 // Morton codes (include/geometry/morton.hpp:27-92).  Written here as a plain
 // bit loop: bit k of axis a lands at bit k*dim + a.  Pinned against the
 // reference's magic-bit encoders compiled in oracle/_ref.
@@ -777,6 +818,7 @@ template <class S> struct AlphaShape2D {
     static constexpr uint32_t max_fill = 250;  // DirectAlphaShape2D.cpp:351
     uint32_t n = 0;
     int morton_bits = 32;
+    int contract = 0;  // fmac mode of the inexact tests
     Lbvh<S, 2> lbvh;
     std::vector<Sphere<S, 2>> coords_m;     // Morton order + 3 infinity nodes
     std::vector<uint32_t> count;            // count pass (cpp:178-225)
@@ -812,14 +854,15 @@ template <class S> struct AlphaShape2D {
                     const S dmaxy = std::fmax(si.c[1] - bb.hi[1], (S)0);
                     const S dminx = std::fmax(bb.lo[0] - si.c[0], (S)0);
                     const S dminy = std::fmax(bb.lo[1] - si.c[1], (S)0);
-                    const bool hit = dmaxx * dmaxx + dminx * dminx + dmaxy * dmaxy + dminy * dminy < rs * rs;
+                    const S dd[4] = {dmaxx, dminx, dmaxy, dminy};
+                    const bool hit = fmac::sumsq(contract, contract & fmac::kBox, dd, 4) < rs * rs;
                     stack[ss] = (int)(sepv + ch);
                     ss += hit;
                 } else if (tid != sepv + ch) {
                     const Sphere<S, 2>& sj = coords_m[sepv + ch];
                     const S ma = std::fmin(si.r, sj.r);
-                    const S dx = si.c[0] - sj.c[0], dy = si.c[1] - sj.c[1];
-                    const S sq = dx * dx + dy * dy;
+                    const S dxy[2] = {si.c[0] - sj.c[0], si.c[1] - sj.c[1]};
+                    const S sq = fmac::sumsq(contract, contract & fmac::kLeaf, dxy, 2);
                     if (sq < (two_delta * ma) * (two_delta * ma)) {
                         visit(sepv + ch);
                         ++nloc;
@@ -966,14 +1009,15 @@ template <class S> struct AlphaShape2D {
             const Sphere<S, 2>& p3 = coords_m[i3];
             const S ux = p3.c[0] - p2.c[0], uy = p3.c[1] - p2.c[1];
             const S vx = p1.c[0] - p2.c[0], vy = p1.c[1] - p2.c[1];
-            S det = ux * vy - uy * vx;
+            const bool fz = contract & fmac::kFilter;
+            S det = fmac::diff(contract, fz, ux, vy, uy, vx);
             if (swap) det = -det;
-            const S d12x = p1.c[0] - p2.c[0], d12y = p1.c[1] - p2.c[1];
-            const S d23x = p2.c[0] - p3.c[0], d23y = p2.c[1] - p3.c[1];
-            const S d31x = p3.c[0] - p1.c[0], d31y = p3.c[1] - p1.c[1];
-            const S n12 = d12x * d12x + d12y * d12y;
-            const S n23 = d23x * d23x + d23y * d23y;
-            const S n31 = d31x * d31x + d31y * d31y;
+            const S d12[2] = {p1.c[0] - p2.c[0], p1.c[1] - p2.c[1]};
+            const S d23[2] = {p2.c[0] - p3.c[0], p2.c[1] - p3.c[1]};
+            const S d31[2] = {p3.c[0] - p1.c[0], p3.c[1] - p1.c[1]};
+            const S n12 = fmac::sumsq(contract, fz, d12, 2);
+            const S n23 = fmac::sumsq(contract, fz, d23, 2);
+            const S n31 = fmac::sumsq(contract, fz, d31, 2);
             const S rsq = ((n12 * n23) * n31) / (((S)4 * det) * det);
             const S aeff = std::fmin(p3.r, std::fmin(p1.r, p2.r));
             if (rsq <= aeff * aeff) {

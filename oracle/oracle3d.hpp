@@ -2,12 +2,16 @@
 // 3D alpha-shape (src/mesh/DirectAlphaShape3D.cpp:121-743).
 //
 // TEST INFRASTRUCTURE ONLY (see oracle.hpp).  Pinned by the reference's 3D
-// golden vector (testcases/IMR26/README.md: 100k glibc-rand points in the
-// unit cube, alpha 0.022, FP32 -> 526,040 tetrahedra), the reference's
-// predicates compiled in oracle/_ref, exact rationals, and Delaunay
-// invariants checked against Qhull.
+// workload (testcases/IMR26/README.md:94-99: 100k glibc-rand points in the
+// unit cube, alpha 0.022, FP32; the README reports 526,040 tetrahedra, the
+// exact answer -- Qhull Delaunay + the circumradius filter -- is 526,042 and
+// is what this restatement gives, tetrahedron for tetrahedron), the
+// reference's predicates compiled in oracle/_ref (including the 425 exact
+// insphere calls of that workload), exact rationals, and Delaunay invariants
+// checked against Qhull.
 #pragma once
 #include <atomic>
+#include <mutex>
 
 #include "oracle.hpp"
 
@@ -141,9 +145,10 @@ struct Tri8 {
 template <class S> struct AlphaShape3D {
     static constexpr uint32_t n_inf = 4;
     static constexpr uint32_t max_cand = 128;  // cpp:369-370 (including the infinity nodes)
-    static constexpr int max_cavity_iter = 100;  // cpp:519
+    int max_cavity_iter = 100;  // cpp:519 (a parameter here only to show the cap's effect)
     uint32_t n = 0;
     int morton_bits = 21;
+    int contract = 0;  // fmac mode of the inexact tests
     Lbvh<S, 3> lbvh;
     std::vector<Sphere<S, 3>> coords_m;        // Morton order + 4 infinity nodes
     std::vector<uint32_t> count;               // count pass (cpp:178-231)
@@ -159,6 +164,12 @@ template <class S> struct AlphaShape3D {
     uint32_t n_elems = 0, n_edges = 0;
     uint64_t exact_calls = 0, over_cap = 0;
     std::atomic<uint64_t> broken_cycles{0};
+    std::atomic<uint64_t> capped_walks{0};  // cavity walks stopped by the 100-step cap
+    std::vector<uint8_t> node_capped;       // 1 where a walk of this node hit the cap
+    std::vector<uint32_t> node_tmax;        // largest local tet count during insertion
+    bool record_exact = false;              // keep the points of every exact insphere call
+    std::vector<S> exact_points;            // 15 per call, in the call's (SoS-sorted) order
+    std::mutex exact_mu;
 
     template <class Visit> void traverse(uint32_t tid, Visit visit) const {
         int stack[64];
@@ -180,16 +191,15 @@ template <class S> struct AlphaShape3D {
                     const S dminx = std::fmax(bb.lo[0] - si.c[0], (S)0);
                     const S dminy = std::fmax(bb.lo[1] - si.c[1], (S)0);
                     const S dminz = std::fmax(bb.lo[2] - si.c[2], (S)0);
-                    const bool hit = dmaxx * dmaxx + dminx * dminx + dmaxy * dmaxy + dminy * dminy + dmaxz * dmaxz +
-                                         dminz * dminz <
-                                     rs * rs;
+                    const S dd[6] = {dmaxx, dminx, dmaxy, dminy, dmaxz, dminz};
+                    const bool hit = fmac::sumsq(contract, contract & fmac::kBox, dd, 6) < rs * rs;
                     stack[ss] = (int)(sepv + ch);
                     ss += hit;
                 } else if (tid != sepv + ch) {
                     const Sphere<S, 3>& sj = coords_m[sepv + ch];
                     const S ma = std::fmin(si.r, sj.r);
-                    const S dx = si.c[0] - sj.c[0], dy = si.c[1] - sj.c[1], dz = si.c[2] - sj.c[2];
-                    const S sq = dx * dx + dy * dy + dz * dz;
+                    const S dxyz[3] = {si.c[0] - sj.c[0], si.c[1] - sj.c[1], si.c[2] - sj.c[2]};
+                    const S sq = fmac::sumsq(contract, contract & fmac::kLeaf, dxyz, 3);
                     if (sq < (two_delta * ma) * (two_delta * ma)) visit(sepv + ch);
                 }
             }
@@ -275,6 +285,10 @@ template <class S> struct AlphaShape3D {
                     cs(i2, i3, pb, pc);
                     cs(i4, i5, pd, pe);
                     det = (S)-exact::insphere_sign(pa, pb, pc, pd, pe);
+                    if (record_exact) {
+                        std::lock_guard<std::mutex> lk(exact_mu);
+                        for (const S* q : {pa, pb, pc, pd, pe}) exact_points.insert(exact_points.end(), q, q + 3);
+                    }
                     int depth = 0;
                     while (det == (S)0) {
                         ++depth;
@@ -336,6 +350,10 @@ template <class S> struct AlphaShape3D {
                     --Rsize;
                     cur = Tlast;
                 }
+                if (Rsize) {
+                    ++capped_walks;
+                    node_capped[tid] = 1;
+                }
                 // one new tet per boundary edge; a walk that leaves the
                 // cycle (only after the iteration cap) stops at 255 steps
                 uint32_t c = first;
@@ -348,6 +366,7 @@ template <class S> struct AlphaShape3D {
                     if (Tlast >= T.size()) T.resize(2 * T.size());
                     T[Tlast] = {(uint8_t)pi, (uint8_t)c, cycle[c]};
                     ++Tlast;
+                    if (Tlast > node_tmax[tid]) node_tmax[tid] = Tlast;
                     c = cycle[c];
                 } while (c != first);
             }
@@ -378,25 +397,28 @@ template <class S> struct AlphaShape3D {
             const S t[3] = {pa.c[0] - pd.c[0], pa.c[1] - pd.c[1], pa.c[2] - pd.c[2]};
             const S u[3] = {pb.c[0] - pd.c[0], pb.c[1] - pd.c[1], pb.c[2] - pd.c[2]};
             const S v[3] = {pc.c[0] - pd.c[0], pc.c[1] - pd.c[1], pc.c[2] - pd.c[2]};
-            const S d2t = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
-            const S d2u = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-            const S d2v = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-            const S vol = t[2] * (u[0] * v[1] - v[0] * u[1]) - u[2] * (t[0] * v[1] - v[0] * t[1]) +
-                          v[2] * (t[0] * u[1] - u[0] * t[1]);
-            auto cross = [](const S* p, const S* q, S* o) {
-                o[0] = p[1] * q[2] - p[2] * q[1];
-                o[1] = p[2] * q[0] - p[0] * q[2];
-                o[2] = p[0] * q[1] - p[1] * q[0];
+            const int m = contract;
+            const bool fz = m & fmac::kFilter;
+            const S d2t = fmac::sumsq(m, fz, t, 3);
+            const S d2u = fmac::sumsq(m, fz, u, 3);
+            const S d2v = fmac::sumsq(m, fz, v, 3);
+            const S vA = fmac::diff(m, fz, u[0], v[1], v[0], u[1]);
+            const S vB = fmac::diff(m, fz, t[0], v[1], v[0], t[1]);
+            const S vC = fmac::diff(m, fz, t[0], u[1], u[0], t[1]);
+            const S vol = fmac::acc(fz, fmac::diff(m, fz, t[2], vA, u[2], vB), v[2], vC);
+            auto cross = [&](const S* p, const S* q, S* o) {
+                o[0] = fmac::diff(m, fz, p[1], q[2], p[2], q[1]);
+                o[1] = fmac::diff(m, fz, p[2], q[0], p[0], q[2]);
+                o[2] = fmac::diff(m, fz, p[0], q[1], p[1], q[0]);
             };
             S uv_[3], vt_[3], tu_[3];
             cross(u, v, uv_);
             cross(v, t, vt_);
             cross(t, u, tu_);
             S w[3];
-            for (int a = 0; a < 3; ++a) w[a] = d2t * uv_[a];
-            for (int a = 0; a < 3; ++a) w[a] += d2u * vt_[a];
-            for (int a = 0; a < 3; ++a) w[a] += d2v * tu_[a];
-            const S wn = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+            for (int a = 0; a < 3; ++a) w[a] = fmac::dot2(m, fz, d2t, uv_[a], d2u, vt_[a]);
+            for (int a = 0; a < 3; ++a) w[a] = fmac::acc(fz, w[a], d2v, tu_[a]);
+            const S wn = fmac::sumsq(m, fz, w, 3);
             const S rc = wn / ((S)4 * vol * vol);
             const S r1 = pa.r, r2 = pb.r, r3 = pc.r, r4 = pd.r;
             if (rc <= r1 * r1 && rc <= r2 * r2 && rc <= r3 * r3 && rc <= r4 * r4) {
@@ -451,6 +473,8 @@ template <class S> struct AlphaShape3D {
         tet.assign(n, {});
         active.assign(n, {});
         node_is_bnd.assign(n, 0);
+        node_capped.assign(n, 0);
+        node_tmax.assign(n, 4);
         node_nelem.assign(n, 0);
         node_nelem_out.assign(n, 0);
         node_nfneig.assign(n, 0);

@@ -102,9 +102,10 @@ uint64_t lbvh_collisions(uint32_t n, const S* xyr, int bits) {
 }
 
 template <class S>
-void* ashape2d_run(uint32_t n, const S* xyr, int bits) {
+void* ashape2d_run(uint32_t n, const S* xyr, int bits, int contract = 0) {
     auto* a = new AlphaShape2D<S>();
     a->morton_bits = bits;
+    a->contract = contract;
     a->init(pack<S, 2>(n, xyr));
     a->compute();
     return a;
@@ -129,9 +130,10 @@ void get2d(AlphaShape2D<S>* a, uint32_t* elem, uint32_t* row, uint32_t* neig, ui
         for (uint32_t i = 0; i < n; ++i) nineig[i] = a->nineig[i];
 }
 template <class S>
-void* ashape3d_run(uint32_t n, const S* xyzr, int bits) {
+void* ashape3d_run(uint32_t n, const S* xyzr, int bits, int contract = 0) {
     auto* a = new AlphaShape3D<S>();
     a->morton_bits = bits;
+    a->contract = contract;
     a->init(pack<S, 3>(n, xyzr));
     a->compute();
     return a;
@@ -354,16 +356,41 @@ int64_t orc_assemble_p1_tet(uint32_t n, const double* xyz, uint32_t ne, const ui
 // ---- alpha-shape 3D ----
 void* orc_ashape3d_run_f32(uint32_t n, const float* xyzr, int bits) { return ashape3d_run<float>(n, xyzr, bits); }
 void* orc_ashape3d_run_f64(uint32_t n, const double* xyzr, int bits) { return ashape3d_run<double>(n, xyzr, bits); }
+// the same with the inexact tests evaluated in fmac mode `contract`
+void* orc_ashape3d_run_contract_f32(uint32_t n, const float* xyzr, int bits, int contract) {
+    return ashape3d_run<float>(n, xyzr, bits, contract);
+}
+// the README-style FP32 run with a cavity-walk cap of `cap` steps, keeping the
+// points of every exact insphere call (readable with orc_ashape3d_exact_points_f32)
+void* orc_ashape3d_run_probe_f32(uint32_t n, const float* xyzr, int bits, int cap, int record) {
+    auto* a = new AlphaShape3D<float>();
+    a->morton_bits = bits;
+    a->max_cavity_iter = cap;
+    a->record_exact = record != 0;
+    a->init(pack<float, 3>(n, xyzr));
+    a->compute();
+    return a;
+}
+uint64_t orc_ashape3d_exact_points_f32(void* h, float* out) {
+    auto* a = static_cast<AlphaShape3D<float>*>(h);
+    if (out) std::memcpy(out, a->exact_points.data(), sizeof(float) * a->exact_points.size());
+    return a->exact_points.size() / 15;
+}
+// per node: 1 where a cavity walk stopped at the 100-step cap
+void orc_ashape3d_capped_f32(void* h, uint8_t* out) {
+    auto* a = static_cast<AlphaShape3D<float>*>(h);
+    std::memcpy(out, a->node_capped.data(), a->n);
+}
 // sizes: [n, n_elems, n_edges, exact_calls, over_cap, broken_cycles]
 void orc_ashape3d_sizes(void* h, int is_f64, uint64_t* out) {
     if (is_f64) {
         auto* a = (AlphaShape3D<double>*)h;
         out[0] = a->n; out[1] = a->n_elems; out[2] = a->n_edges; out[3] = a->exact_calls;
-        out[4] = a->over_cap; out[5] = a->broken_cycles.load();
+        out[4] = a->over_cap; out[5] = a->broken_cycles.load(); out[6] = a->capped_walks.load();
     } else {
         auto* a = (AlphaShape3D<float>*)h;
         out[0] = a->n; out[1] = a->n_elems; out[2] = a->n_edges; out[3] = a->exact_calls;
-        out[4] = a->over_cap; out[5] = a->broken_cycles.load();
+        out[4] = a->over_cap; out[5] = a->broken_cycles.load(); out[6] = a->capped_walks.load();
     }
 }
 void orc_ashape3d_get_f32(void* h, uint32_t* elem, uint32_t* row, uint32_t* neig, uint8_t* eb, uint8_t* nb,
@@ -427,6 +454,9 @@ int orc_incircle_exact_f64(const double* a, const double* b, const double* c, co
 // ---- alpha-shape 2D: run, query sizes, copy outputs, free ----
 void* orc_ashape2d_run_f32(uint32_t n, const float* xyr, int bits) { return ashape2d_run<float>(n, xyr, bits); }
 void* orc_ashape2d_run_f64(uint32_t n, const double* xyr, int bits) { return ashape2d_run<double>(n, xyr, bits); }
+void* orc_ashape2d_run_contract_f32(uint32_t n, const float* xyr, int bits, int contract) {
+    return ashape2d_run<float>(n, xyr, bits, contract);
+}
 
 // sizes: [n, n_elems, n_edges, exact_calls]
 void orc_ashape2d_sizes(void* h, int is_f64, uint64_t* out) {

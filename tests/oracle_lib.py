@@ -28,6 +28,39 @@ def lib():
     return _lib
 
 
+def _bind_ref(lib):
+    P = ctypes.POINTER
+    f = ctypes.c_float
+    for name in ("ref_orient2d", "ref_orient2dexact"):
+        getattr(lib, name).restype = f
+        getattr(lib, name).argtypes = [P(f)] * 3
+    lib.ref_incircle.restype = f
+    lib.ref_incircle.argtypes = [P(f)] * 5
+    lib.ref_incircleexact.restype = f
+    lib.ref_incircleexact.argtypes = [P(f)] * 4
+    for name in ("ref_orient3d", "ref_orient3dexact"):
+        getattr(lib, name).restype = f
+        getattr(lib, name).argtypes = [P(f)] * 4
+    lib.ref_insphere.restype = f
+    lib.ref_insphere.argtypes = [P(f)] * 6
+    lib.ref_insphereexact.restype = f
+    lib.ref_insphereexact.argtypes = [P(f)] * 5
+    return lib
+
+
+_ref_fma = None
+
+
+def ref_fma():
+    """The same reference headers compiled with fma contraction (-mfma
+    -ffp-contract=fast), or None if absent."""
+    global _ref_fma
+    so = REF_SO.replace("libref_geom.so", "libref_geom_fma.so")
+    if _ref_fma is None and os.path.exists(so):
+        _ref_fma = _bind_ref(ctypes.CDLL(so))
+    return _ref_fma
+
+
 def ref():
     """The reference's own predicates/Morton headers compiled (None if absent)."""
     global _ref
@@ -97,6 +130,15 @@ def _declare(L):
         getattr(L, f"orc_ashape3d_run_{tag}").restype = vp
         getattr(L, f"orc_ashape3d_run_{tag}").argtypes = [u32, vp, i32]
         getattr(L, f"orc_ashape3d_get_{tag}").argtypes = [vp] * 9
+    L.orc_ashape3d_run_contract_f32.restype = vp
+    L.orc_ashape3d_run_contract_f32.argtypes = [u32, vp, i32, i32]
+    L.orc_ashape3d_capped_f32.argtypes = [vp, vp]
+    L.orc_ashape3d_run_probe_f32.restype = vp
+    L.orc_ashape3d_run_probe_f32.argtypes = [u32, vp, i32, i32, i32]
+    L.orc_ashape3d_exact_points_f32.restype = u64
+    L.orc_ashape3d_exact_points_f32.argtypes = [vp, vp]
+    L.orc_ashape2d_run_contract_f32.restype = vp
+    L.orc_ashape2d_run_contract_f32.argtypes = [u32, vp, i32, i32]
     L.orc_ashape3d_sizes.argtypes = [vp, i32, vp]
     L.orc_lbvha2d_build_f64.restype = i32
     L.orc_lbvha2d_build_f64.argtypes = [u32, vp, i32, i32] + [vp] * 7
@@ -165,13 +207,18 @@ def lbvh2d_collisions(coords, radii, bits=32, dtype=np.float64):
     return int(fn(xyr.shape[0], _p(xyr), bits))
 
 
-def ashape2d(coords, alpha, bits=32, dtype=np.float64):
-    """Run the oracle alpha-shape; returns a dict of outputs."""
+def ashape2d(coords, alpha, bits=32, dtype=np.float64, contract=0):
+    """Run the oracle alpha-shape; returns a dict of outputs.  `contract`
+    (FP32 only): fmac mode bits of oracle.hpp for the inexact tests."""
     L = lib()
     xyr = spheres(coords, alpha, dtype)
     n = xyr.shape[0]
     is64 = dtype == np.float64
-    h = (L.orc_ashape2d_run_f64 if is64 else L.orc_ashape2d_run_f32)(n, _p(xyr), bits)
+    if contract:
+        assert not is64
+        h = L.orc_ashape2d_run_contract_f32(n, _p(xyr), bits, contract)
+    else:
+        h = (L.orc_ashape2d_run_f64 if is64 else L.orc_ashape2d_run_f32)(n, _p(xyr), bits)
     sz = np.zeros(4, np.uint64)
     L.orc_ashape2d_sizes(h, int(is64), _p(sz))
     ne, nedge = int(sz[1]), int(sz[2])
@@ -245,21 +292,39 @@ def treelet_dp(boxes, cost, size):
     return copt, popt, sz
 
 
-def ashape3d(coords, alpha, bits=21, dtype=np.float64):
-    """Run the 3D oracle alpha-shape; returns a dict of outputs."""
+def ashape3d(coords, alpha, bits=21, dtype=np.float64, contract=0, probe=None):
+    """Run the 3D oracle alpha-shape; returns a dict of outputs.  `contract`
+    (FP32 only) evaluates the inexact search and filter tests fused (fmac
+    mode bits of oracle.hpp).  `probe` (FP32 only): {"cap": cavity-walk step
+    cap, "record": 1 to return the points of every exact insphere call as
+    r["exact_points"], float32 [m, 5, 3]}."""
     L = lib()
     xyzr = spheres(coords, alpha, dtype)
     n = xyzr.shape[0]
     is64 = dtype == np.float64
     tag = "f64" if is64 else "f32"
-    h = getattr(L, f"orc_ashape3d_run_{tag}")(n, _p(xyzr), bits)
-    sz = np.zeros(6, np.uint64)
+    if probe is not None:
+        assert not is64
+        h = L.orc_ashape3d_run_probe_f32(n, _p(xyzr), bits, int(probe.get("cap", 100)), int(probe.get("record", 0)))
+    elif contract:
+        assert not is64
+        h = L.orc_ashape3d_run_contract_f32(n, _p(xyzr), bits, contract)
+    else:
+        h = getattr(L, f"orc_ashape3d_run_{tag}")(n, _p(xyzr), bits)
+    sz = np.zeros(8, np.uint64)
     L.orc_ashape3d_sizes(h, int(is64), _p(sz))
     ne, nedge = int(sz[1]), int(sz[2])
     r = dict(elem=np.empty((ne, 4), np.uint32), row=np.empty(n + 1, np.uint32), neig=np.empty(nedge, np.uint32),
              edge_is_bnd=np.empty(nedge, np.uint8), node_is_bnd=np.empty(n, np.uint8), perm=np.empty(n, np.uint32),
              ordered=np.empty((n, 3), dtype), nineig=np.empty(n, np.uint32), n_exact_calls=int(sz[3]),
-             over_cap=int(sz[4]), broken_cycles=int(sz[5]))
+             over_cap=int(sz[4]), broken_cycles=int(sz[5]), capped_walks=int(sz[6]))
+    if not is64:
+        r["node_capped"] = np.empty(n, np.uint8)
+        L.orc_ashape3d_capped_f32(ctypes.c_void_p(h), _p(r["node_capped"]))
+        m = L.orc_ashape3d_exact_points_f32(ctypes.c_void_p(h), None)
+        r["exact_points"] = np.empty((m, 5, 3), np.float32)
+        if m:
+            L.orc_ashape3d_exact_points_f32(ctypes.c_void_p(h), _p(r["exact_points"]))
     getattr(L, f"orc_ashape3d_get_{tag}")(
         ctypes.c_void_p(h), _p(r["elem"]), _p(r["row"]), _p(r["neig"]), _p(r["edge_is_bnd"]), _p(r["node_is_bnd"]),
         _p(r["perm"]), _p(r["ordered"]), _p(r["nineig"]))
