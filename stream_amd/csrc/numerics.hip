@@ -445,6 +445,7 @@ void build_sell(uint32_t n, const uint32_t* row, const uint32_t* col, const doub
                        (int)part, S.len.p, S.width.p, S.tmp.p);
     exclusive_scan_u32(S.tmp.p, S.off.p, S.nslices, S.scan_ws.p, s);
     S.entries = read_scalar(S.off.p + S.nslices);
+    check_error_flag();  // a scan look-back that ran out of spins
     S.col.resize(S.entries ? S.entries : 1);
     S.col16.resize(S.entries ? S.entries : 1);
     S.val.resize(S.entries ? S.entries : 1);

@@ -44,14 +44,14 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(const Tin* __restric
         const uint32_t lane = lane_id();
         uint32_t prefix = 0;
         if (tile == 0) {
-            if (lane == 0) st_release_u64(status, kInc | total);
+            if (lane == 0) st_relaxed_u64(status, kInc | total);
         } else {
-            if (lane == 0) st_release_u64(status + tile, kAgg | total);
+            if (lane == 0) st_relaxed_u64(status + tile, kAgg | total);
             int64_t j0 = (int64_t)tile - 1;
             uint32_t spins = 0;
             while (true) {
                 const int64_t j = j0 - (int64_t)lane;
-                const uint64_t sv = (j >= 0) ? ld_acquire_u64(status + j) : kInc;
+                const uint64_t sv = (j >= 0) ? ld_relaxed_u64(status + j) : kInc;
                 const uint32_t fl = (uint32_t)(sv >> 32);
                 if (__ballot(fl == 0)) {
                     if (++spins > kSpinLimit) {
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(const Tin* __restric
                 prefix += wave_sum(val);
                 j0 -= kWave;
             }
-            if (lane == 0) st_release_u64(status + tile, kInc | (uint64_t)(prefix + total));
+            if (lane == 0) st_relaxed_u64(status + tile, kInc | (uint64_t)(prefix + total));
         }
         if (lane == 0) s_prefix = prefix;
     }
@@ -89,8 +89,11 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(const Tin* __restric
     }
 }
 
+// ws: scan_workspace_words(n) words, cleared here by a fill kernel (no
+// memset, so the sequence may be captured); flags: the error word a look-back
+// that runs out of spins raises.
 template <class Tin>
-static void exclusive_scan(const Tin* in, uint32_t* out, uint32_t n, uint64_t* ws, hipStream_t s) {
+static void exclusive_scan(const Tin* in, uint32_t* out, uint32_t n, uint64_t* ws, uint32_t* flags, hipStream_t s) {
     if (n == 0) {
         fill<uint32_t>(out, 1, 0u, s);
         return;
@@ -98,15 +101,16 @@ static void exclusive_scan(const Tin* in, uint32_t* out, uint32_t n, uint64_t* w
     const size_t words = scan_workspace_words(n);
     fill<uint64_t>(ws, words, 0ull, s);
     const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(scan_kernel<Tin>, dim3(tiles), dim3(kScanThreads), 0, s, in, out, n, ws, ctx().d_flags);
+    hipLaunchKernelGGL(scan_kernel<Tin>, dim3(tiles), dim3(kScanThreads), 0, s, in, out, n, ws, flags);
 }
 
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* ws, hipStream_t s) {
-    exclusive_scan<uint32_t>(in, out, n, ws, s);
+    exclusive_scan<uint32_t>(in, out, n, ws, ctx().d_flags, s);
 }
 void exclusive_scan_u8(const uint8_t* in, uint32_t* out, uint32_t n, uint64_t* ws, hipStream_t s) {
-    exclusive_scan<uint8_t>(in, out, n, ws, s);
+    exclusive_scan<uint8_t>(in, out, n, ws, ctx().d_flags, s);
 }
+
 
 // ---------------------------------------------------------------------------
 // Onesweep radix sort.
@@ -206,7 +210,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_pass_kernel(
         cnt += x;
     }
     uint32_t* my_status = status + (size_t)tile * 256 + t;
-    st_release_u32(my_status, ((tile == 0 ? 2u : 1u) << 30) | cnt);
+    st_relaxed_u32(my_status, ((tile == 0 ? 2u : 1u) << 30) | cnt);
     uint32_t tot;
     const uint32_t ex = block_excl_scan<kSortThreads>(cnt, s_tmp, &tot);
     const uint32_t gex = block_excl_scan<kSortThreads>(hist[t], s_tmp, &tot);
@@ -216,7 +220,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_pass_kernel(
         int32_t j = (int32_t)tile - 1;
         uint32_t spins = 0;
         while (j >= 0) {
-            const uint32_t sv = ld_acquire_u32(status + (size_t)j * 256 + t);
+            const uint32_t sv = ld_relaxed_u32(status + (size_t)j * 256 + t);
             const uint32_t fl = sv >> 30;
             if (fl == 0) {
                 if (++spins > kSpinLimit) {
@@ -230,7 +234,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_pass_kernel(
             if (fl == 2) break;
             --j;
         }
-        st_release_u32(my_status, (2u << 30) | (prefix + cnt));
+        st_relaxed_u32(my_status, (2u << 30) | (prefix + cnt));
     }
     s_dbase[t] = gex + prefix - ex;
     __syncthreads();
@@ -350,6 +354,7 @@ int32_t Stream_exclusive_scan_u32(uint32_t n, const uint32_t* in, uint32_t* out)
         h2d(din.p, in, n);
         exclusive_scan_u32(din.p, dout.p, n, ws.p, s);
         d2h(out, dout.p, n + 1);
+        check_error_flag();
     });
 }
 
@@ -366,6 +371,7 @@ int32_t Stream_exclusive_scan_u8(uint32_t n, const uint8_t* in, uint32_t* out) {
         h2d(din.p, in, n);
         exclusive_scan_u8(din.p, dout.p, n, ws.p, s);
         d2h(out, dout.p, n + 1);
+        check_error_flag();
     });
 }
 

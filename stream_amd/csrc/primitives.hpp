@@ -59,16 +59,20 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp,
     return off + incl - v;
 }
 
-__device__ __forceinline__ uint32_t ld_acquire_u32(const uint32_t* p) {
+// Relaxed agent-scope loads and stores of look-back status words.  They
+// carry no ordering: each status word packs its flag and its value into
+// one word, so a reader that sees the flag sees the value with it.  Data
+// published in any other word needs real release/acquire ordering.
+__device__ __forceinline__ uint32_t ld_relaxed_u32(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_release_u32(uint32_t* p, uint32_t v) {
+__device__ __forceinline__ void st_relaxed_u32(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ uint64_t ld_acquire_u64(const uint64_t* p) {
+__device__ __forceinline__ uint64_t ld_relaxed_u64(const uint64_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_release_u64(uint64_t* p, uint64_t v) {
+__device__ __forceinline__ void st_relaxed_u64(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -125,6 +129,16 @@ template <class T> __global__ void fill_kernel(T* p, size_t n, T v) {
 template <class T> void fill(T* p, size_t n, T v, hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(fill_kernel<T>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, p, n, v);
+}
+
+// Host side: read the context's error word (synchronises the stream); when
+// set, clear it and throw.
+inline void check_error_flag() {
+    const uint32_t f = read_scalar(ctx().d_flags + kFlagError);
+    if (f != 0) {
+        fill<uint32_t>(ctx().d_flags, 1, 0u, ctx().stream);
+        throw std::runtime_error("device error flag raised (code " + std::to_string(f) + ")");
+    }
 }
 
 }  // namespace stream_amd
