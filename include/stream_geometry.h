@@ -97,6 +97,14 @@ int32_t Stream_copy_calibration(uint64_t nbytes, uint32_t reps, double* ms);
 /* exclusive scans, out [n+1] with out[n] = total */
 int32_t Stream_exclusive_scan_u32(uint32_t n, const uint32_t* in, uint32_t* out);
 int32_t Stream_exclusive_scan_u8(uint32_t n, const uint8_t* in, uint32_t* out);
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (serving loops, harnesses that forbid hipMalloc): the scan
+ * of Stream_exclusive_scan_u32 on caller-owned DEVICE buffers, in [n] and
+ * out [n+1], with its look-back workspace in a static per-device scratch
+ * array; n <= 2^28 (larger n is refused, no launch).  Runs on `stream` (a
+ * hipStream_t, NULL = the null stream) and waits for it.  Makes no device
+ * allocation; calls that overlap on one device share the scratch and race. */
+int32_t Stream_exclusive_scan_u32_noalloc(uint32_t n, const uint32_t* d_in, uint32_t* d_out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,4 +1,5 @@
 // Device-wide scan and onesweep radix sort (see primitives.hpp).
+#include <mutex>
 #include <vector>
 
 #include "primitives.hpp"
@@ -111,6 +112,38 @@ void exclusive_scan_u8(const uint8_t* in, uint32_t* out, uint32_t n, uint64_t* w
     exclusive_scan<uint8_t>(in, out, n, ws, ctx().d_flags, s);
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free scan entry point: the look-back workspace
+// of up to kNoAllocScanMax items (ticket + one status word per 2048-item
+// tile) and, in the last word, the error flag its kernels raise.  One copy
+// per device, held until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kNoAllocScanMax = 1u << 28;
+constexpr size_t kNoAllocScanWords = 1 + (size_t)(kNoAllocScanMax + kScanTile - 1) / kScanTile + 1;
+static __device__ __align__(256) uint64_t scratch[kNoAllocScanWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint64_t* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the
+// first call for that device.
+uint64_t* scan_scratch() {
+    int dev = 0;
+    SA_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* ptr = nullptr;
+        SA_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw std::runtime_error("hipGetSymbolAddress returned no address for the scan scratch");
+        g_scratch_dev[dev] = (uint64_t*)ptr;
+    }
+    return g_scratch_dev[dev];
+}
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // Onesweep radix sort.
@@ -372,6 +405,33 @@ int32_t Stream_exclusive_scan_u8(uint32_t n, const uint8_t* in, uint32_t* out) {
         exclusive_scan_u8(din.p, dout.p, n, ws.p, s);
         d2h(out, dout.p, n + 1);
         check_error_flag();
+    });
+}
+
+// Allocation-free exclusive scan of u32 on device buffers (no counterpart
+// in the reference): out[0..n] with out[n] = total, in [n] and out [n + 1]
+// device pointers the caller owns, n <= 2^28.  The look-back workspace is
+// the file-scope `scratch` array (one per device; cleared by a fill kernel
+// on every call, so a call never depends on what the last one left), so
+// the call makes no device allocation.  Kernels run on `stream` (a
+// hipStream_t; NULL = the null stream) and the call waits for them, then
+// reads the error word.  Calls that overlap on one device share `scratch`
+// and race.
+int32_t Stream_exclusive_scan_u32_noalloc(uint32_t n, const uint32_t* d_in, uint32_t* d_out, void* stream) {
+    return guarded([&] {
+        if (n > kNoAllocScanMax)
+            throw std::invalid_argument("Stream_exclusive_scan_u32_noalloc: n exceeds the scratch limit of 2^28 items");
+        if (!d_out || (n > 0 && !d_in)) throw std::invalid_argument("Stream_exclusive_scan_u32_noalloc: null buffer");
+        hipStream_t s = (hipStream_t)stream;
+        uint64_t* ws = scan_scratch();
+        uint32_t* flag = (uint32_t*)(ws + kNoAllocScanWords - 1);
+        fill<uint32_t>(flag, 2, 0u, s);
+        exclusive_scan<uint32_t>(d_in, d_out, n, ws, flag, s);
+        SA_HIP(hipGetLastError());
+        uint32_t f = 0;
+        SA_HIP(hipMemcpyAsync(&f, flag, sizeof(f), hipMemcpyDeviceToHost, s));
+        SA_HIP(hipStreamSynchronize(s));
+        if (f != 0) throw std::runtime_error("Stream_exclusive_scan_u32_noalloc: device look-back ran out of spins");
     });
 }
 
