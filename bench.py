@@ -52,39 +52,39 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(pts, al, f, fp64_iters):
-    """Oracle step on the host: one mesh + assembly at full size and a bounded
-    number of FP64 Jacobi-PCG iterations, the CG time scaled to the
-    iteration count the GPU FP64 solve needed."""
-    import ctypes
-
+def cpu_baseline(pts, al, f):
+    """Oracle step on the host cores (the C++ restatement with OpenMP, one
+    parallel loop per device launch of the reference): one full-size mesh,
+    P1 assembly and FP64 Jacobi-PCG from x = 0 to a true relative residual
+    of 1e-8, all timed; then, untimed, the same PCG continued to 1e-11,
+    whose solution is the FP64 reference the GPU solutions are compared
+    with.  Threads: OMP_NUM_THREADS (the GPU box allots 16 host cores per
+    GPU and sets it to 16), bound close."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     import oracle_lib as O
 
-    threads = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    L = O.lib()
-    n = pts.shape[0]
-    xyr = O.spheres(pts, al, np.float64)
-    fa = np.ascontiguousarray(f, np.float64)
-    times = np.zeros(3)
-    sizes = np.zeros(2, np.uint64)
-    sample_iters = 100
-    it = L.orc_step_f64(n, xyr.ctypes.data_as(ctypes.c_void_p), fa.ctypes.data_as(ctypes.c_void_p), sample_iters,
-                        times.ctypes.data_as(ctypes.c_void_p), sizes.ctypes.data_as(ctypes.c_void_p))
-    per_iter = times[2] / max(1, it)
-    step_ms = times[0] + times[1] + per_iter * fp64_iters
-    return {
-        "value": 1000.0 / step_ms,
+    r = O.step_solve(pts, al, f, rtol=1e-8, ref_rtol=1e-11)
+    mesh_ms, asm_ms, cg_ms = r["times_ms"]
+    step_ms = mesh_ms + asm_ms + cg_ms
+    base = {
+        "value": round(1000.0 / step_ms, 4),
         "unit": "steps/s",
-        "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
+        "cores": threads,
         "kind": "port",
-        "sample": (f"one full-size mesh ({times[0]:.0f} ms) + P1 assembly ({times[1]:.0f} ms) + {it} FP64 "
-                   f"Jacobi-PCG iterations ({per_iter:.2f} ms/iter) on the C++ oracle (OpenMP), CG extrapolated "
-                   f"to the {fp64_iters} iterations of the FP64 solve; host CPU: {_cpu_model()}"),
+        "sample": (f"one full step on the C++ oracle (OpenMP, {threads} threads, OMP_PROC_BIND=close): mesh "
+                   f"{mesh_ms:.0f} ms + P1 assembly {asm_ms:.0f} ms + FP64 Jacobi-PCG to a true rtol of 1e-8 "
+                   f"({r['iters']} iterations, {cg_ms:.0f} ms, rel {r['rel']:.2e}); host CPU: {_cpu_model()}, "
+                   f"{os.cpu_count()} logical CPUs visible"),
+        "ms_per_step": round(step_ms, 1),
+        "cpu_model": _cpu_model(),
     }
+    ref = {"x": r["x_ref"], "rel": r["ref_rel"], "iters": r["ref_iters"]}
+    return base, ref
 
 
 def sell_dir_bytes(n, nnz, value_bytes, llt=False):
@@ -113,6 +113,45 @@ def sell_update_bytes(n, nnz):
     is counted once (neighbour re-reads are cache traffic)."""
     nslices = (n + 63) // 64
     return n * 42 + (nnz - n) // 2 * 6 + nslices * 6
+
+
+def section_bytes(n, nnz, tris, edges):
+    """Algorithmic HBM bytes of the mesh-and-assembly stages on n FP64
+    points (32-byte padded records), per stage, each array counted once:
+      morton: box partials read the records (32 n), the Morton pass reads
+              them again and writes codes + indices (32 n + 12 n);
+      sort: 8 digit passes, each reading and writing (u64 key, u32 index)
+            pairs (8 x 24 n);
+      gather: index 4 n, record read 32 n and written 32 n;
+      karras: codes 8 n, node words and parent links 17 n;
+      refit: record 32 n, parent links and flags 12 n, child boxes read
+             and box written 2 x 32 n;
+      fill: record 32 n, candidate ids written 4 x 17 n (K = 17 in the
+            2 delta alpha disc at alpha = 1.2 h);
+      delaunay: candidate ids 4 K n + record 32 n + local triangles
+                (2 B) and counts written, ~128 n in all;
+      scans: u8 and u32 counts read, two offset arrays written (13 n);
+      compress: per-candidate flags and ids read (5 K n), edges written
+                (5 B each), triangles written (12 B each);
+      p1: per node its local triangles (2 B each, ~6), the candidate ids
+          they name (4 B, ~6), counts and flags (2 B), edge offsets (4 B),
+          its record (32 B) and source (8 B); per edge its id (4 B); per
+          non-zero column and value written (12 B); per row the row offset,
+          FP64 inverse diagonal, FP32 inverse diagonal and load written
+          (24 B)."""
+    K = 17
+    return {
+        "morton": 76 * n,
+        "sort": 192 * n,
+        "gather": 68 * n,
+        "karras": 25 * n,
+        "refit": 108 * n,
+        "fill": (32 + 4 * K) * n,
+        "delaunay": 128 * n,
+        "scans": 13 * n,
+        "compress": 5 * K * n + 5 * edges + 12 * tris,
+        "p1": (12 + 24 + 2 + 4 + 32 + 8 + 24) * n + 4 * edges + 12 * nnz,
+    }
 
 
 def tank3d_leg(steps=2, weights3d=None):
@@ -374,6 +413,26 @@ def main():
         fp64 = dict(ms_per_step=ms64, iters=inf64[-1]["iters"], true_rel=inf64[-1]["true_rel"],
                     solve_ms=inf64[-1]["solve_ms"])
         print(f"[bench] fp64: {ms64:.2f} ms/step iters {fp64['iters']}", file=sys.stderr, flush=True)
+    fsai = None
+    if not args.no_fp64 and args.solver == "learned":
+        # the classical FSAI factor on the same pattern (the minimiser of the
+        # network's training loss), timed the same way
+        msf, inff = timed("fsai", max(1, args.steps // 2), 1)
+        fsai = dict(ms_per_step=msf, steps_per_s=1000.0 / msf, iters=inff[-1]["iters"],
+                    true_rel=inff[-1]["true_rel"], solve_ms=inff[-1]["solve_ms"], prep_ms=inff[-1]["prep_ms"])
+        print(f"[bench] fsai: {msf:.2f} ms/step iters {fsai['iters']}", file=sys.stderr, flush=True)
+
+    # PCIe-inclusive rate: the points, alpha and source copied from host
+    # memory into the step before every mesh-and-solve
+    k_pcie = max(2, args.steps // 4)
+    stream_amd.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k_pcie):
+        step.set_points(pts, al, f)
+        step.run(args.solver, rtol=1e-8)
+    stream_amd.synchronize()
+    ms_pcie = (time.perf_counter() - t0) * 1000.0 / k_pcie
+    print(f"[bench] {args.solver} with host-to-device points: {ms_pcie:.2f} ms/step", file=sys.stderr, flush=True)
 
     # dominant kernel: on the L L^T path the update kernel (update + L^T
     # apply), reported beside the direction kernel; on the Jacobi paths the
@@ -386,8 +445,11 @@ def main():
     def roof(kname, kms, alg_bytes, traffic):
         achieved = alg_bytes / (kms * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
-                "kernel_ms": round(kms, 5), "alg_bytes_per_launch": int(alg_bytes)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": None if traffic is None else
+                "committed counter summary (profiles/, rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE of "
+                "bench.py --profile-kernels), not measured in this run",
+                "kernel": kname, "kernel_ms": round(kms, 5), "alg_bytes_per_launch": int(alg_bytes)}
 
     kms = step.time_dir_kernel(fp32=fp32, reps=200, llt=llt)
     dname = "llt_dir_kernel" if llt else "cg_dir_kernel"
@@ -418,14 +480,38 @@ def main():
                 "bytes_per_launch": int(gbytes), "achieved_gbs": round(gbs, 1),
                 "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
+    # mesh-and-assembly stages (HIP events between the stages' launches)
+    step.run(args.solver, rtol=1e-8)
+    sec = step.time_sections(reps=5)
+    sb = section_bytes(n, nnz, int(last["n_elems"]), int(nnz - n))
+    stages = {}
+    for k, b in sb.items():
+        gbs = b / (sec[k] * 1e-3) / 1e9
+        stages[k] = {"ms": round(sec[k], 5), "alg_bytes": int(b), "achieved_gbs": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    stages["mesh_and_assembly_ms"] = round(sec["mesh_and_assembly"], 4)
+    stages["sort_per_pass_ms"] = round(sec["sort"] / 8, 5)
+
+    # the solutions of every solver, for the error against the reference
+    sols = {}
+    for solver in ("fp64", "mixed", "fsai") + (("learned",) if net is not None else ()):
+        step.run(solver, rtol=1e-8)
+        sols[solver] = step.solution(morton=True)
+
     tank = None
     if args.tank3d:
         tank = tank3d_leg()
 
     cpu = None
+    err = None
     if not args.no_cpu:
-        iters64 = fp64["iters"] if fp64 else last["iters"]
-        cpu = cpu_baseline(pts, al, f, iters64)
+        cpu, ref = cpu_baseline(pts, al, f)
+        xr = ref["x"]
+        nr = float(np.linalg.norm(xr))
+        err = {"reference": (f"oracle FP64 Jacobi-PCG on the host, continued from its 1e-8 solution to a true "
+                             f"relative residual of {ref['rel']:.1e} (FP64 rounding floor ~3e-12; a sparse direct "
+                             f"solve does not finish in 15 min at 1M)"),
+               **{k: float(np.linalg.norm(v - xr) / nr) for k, v in sols.items()}}
 
     out = {
         "metric": METRIC,
@@ -456,6 +542,12 @@ def main():
         "fp64_jacobi": None if fp64 is None else {
             "steps_per_s": round(1000.0 / fp64["ms_per_step"], 4), **fp64},
         "speedup_vs_fp64_jacobi": None if fp64 is None else round(fp64["ms_per_step"] / ms, 3),
+        "fsai": fsai,
+        "faster_llt_preconditioner": None if fsai is None else ("learned" if ms <= fsai["ms_per_step"] else "fsai"),
+        "pcie_inclusive": {"steps_per_s": round(1000.0 / ms_pcie, 4), "ms_per_step": round(ms_pcie, 3),
+                           "steps": k_pcie, "note": "points, alpha and source copied host-to-device every step"},
+        "err_vs_fp64_reference": err,
+        "mesh_stages": stages,
         "roofline": roofline,
         "mfma": mfma,
         "tank3d": tank,

@@ -124,6 +124,11 @@ int32_t PfemStep2D_get_mesh(PfemStep2D* step, uint32_t* elem, uint8_t* node_is_b
  * kind 3 = the L L^T update kernel on the learned factor (FSAI without a
  * network). */
 int32_t PfemStep2D_time_dir_kernel(PfemStep2D* step, int32_t kind, uint32_t reps, double* ms);
+/* no reference counterpart: GPU ms of the mesh-and-assembly stages on the
+ * current points, averaged over reps runs; ms [11] = box + Morton codes,
+ * radix sort (all passes), gather, hierarchy, refit, candidate search, local
+ * Delaunay + filter, scans, compress, P1 assembly, whole mesh + assembly */
+int32_t PfemStep2D_time_sections(PfemStep2D* step, uint32_t reps, double* ms);
 
 #ifdef __cplusplus
 }

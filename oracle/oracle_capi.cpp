@@ -543,4 +543,47 @@ uint32_t orc_step_f64(uint32_t n, const double* xyr, const double* f, uint32_t c
     return it;
 }
 
+// ---- timed CPU step solved to the tolerance (baseline), then an untimed
+// reference solve: LBVH + alpha-shape, P1 assembly and FP64 Jacobi-PCG to
+// a true relative residual of rtol from x = 0 (timed: times_ms [mesh,
+// assembly, cg]); then the same PCG continued from that solution to
+// ref_rtol (untimed), written to x_ref (Morton order).  info: [cg
+// iterations, true rel after the timed solve, reference iterations, true
+// rel of the reference]; sizes: [triangles, nnz].
+void orc_step_solve_f64(uint32_t n, const double* xyr, const double* f, double rtol, double ref_rtol, uint32_t maxit,
+                        double* times_ms, uint64_t* sizes, double* x_ref, double* info) {
+    using clk = std::chrono::steady_clock;
+    auto t0 = clk::now();
+    AlphaShape2D<double> a;
+    a.morton_bits = 32;
+    a.init(pack<double, 2>(n, xyr));
+    a.compute();
+    auto t1 = clk::now();
+    std::vector<double> xy(2ull * n), fm(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        xy[2 * i] = a.coords_m[i].c[0];
+        xy[2 * i + 1] = a.coords_m[i].c[1];
+        fm[i] = f[a.lbvh.perm[i]];
+    }
+    CsrD A;
+    std::vector<double> bb;
+    assemble_p1(n, xy.data(), a.n_elems, a.elem.data(), a.node_is_bnd.data(), fm.data(), A, bb);
+    auto t2 = clk::now();
+    for (uint32_t i = 0; i < n; ++i) x_ref[i] = 0.0;
+    double rel = 0;
+    const uint32_t it = jacobi_pcg(A, bb.data(), x_ref, rtol, maxit, &rel);
+    auto t3 = clk::now();
+    times_ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    times_ms[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    times_ms[2] = std::chrono::duration<double, std::milli>(t3 - t2).count();
+    sizes[0] = a.n_elems;
+    sizes[1] = A.col.size();
+    double rel2 = 0;
+    const uint32_t it2 = jacobi_pcg(A, bb.data(), x_ref, ref_rtol, maxit, &rel2);
+    info[0] = it;
+    info[1] = rel;
+    info[2] = it2;
+    info[3] = rel2;
+}
+
 }  // extern "C"

@@ -42,3 +42,20 @@ def bind(lib, name, restype, argtypes):
     fn.restype = restype
     fn.argtypes = argtypes
     return fn
+
+
+def node_arrays(nodes, dim, dtype, *per_node):
+    """C-contiguous copies of an (n, dim) node array and of per-node arrays
+    of length n, each checked for its shape before any of them reaches the
+    native library (which reads n rows of each)."""
+    nodes = np.ascontiguousarray(nodes, dtype=dtype)
+    if nodes.ndim != 2 or nodes.shape[1] != dim:
+        raise ValueError(f"nodes must have shape (n, {dim}), got {nodes.shape}")
+    n = nodes.shape[0]
+    out = [nodes]
+    for a in per_node:
+        a = np.ascontiguousarray(a, dtype=dtype)
+        if a.shape != (n,):
+            raise ValueError(f"per-node array must have shape ({n},), got {a.shape}")
+        out.append(a)
+    return out

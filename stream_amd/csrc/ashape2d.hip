@@ -6,6 +6,12 @@
 
 namespace stream_amd {
 
+uint32_t& ashape2d_edge_capacity() {
+    static uint32_t k = 6;
+    return k;
+}
+
+
 template <class S> struct Consts2D;
 template <> struct Consts2D<float> {
     // 2*delta, delta = sqrt((1+4eps)/(1-3eps)) (DirectAlphaShape2D.cpp:186-188)
@@ -432,7 +438,7 @@ template <class S> void Ash2D<S>::init(hipStream_t s) {
     nelem_out.resize(n);
     row.resize(n + 1);
     elemrow.resize(n + 1);
-    cap_edges = 6u * n + 64u;
+    cap_edges = ashape2d_edge_capacity() * n + 64u;
     cap_elems = 2u * n + 64u;
     neig.resize(cap_edges);
     edge_is_bnd.resize(cap_edges);
@@ -446,15 +452,20 @@ template <class S> void Ash2D<S>::init(hipStream_t s) {
 template <class S> void Ash2D<S>::compute(hipStream_t s) {
     if (!inited) throw std::invalid_argument("AlphaShape2D_compute called before init");
     AshArgs<S> a = make_args(*this);
+    if (!exact_ws.p) exact_ws.resize((size_t)kExactLanes * kExactWsDoubles);
+    section_mark(s, kMarkCompute0);
     fill<uint32_t>(queue.p, 1, 0u, s);
     hipLaunchKernelGGL((fill_kernel<S>), dim3(grid_for(n, 256)), dim3(256), 0, s, a);
+    section_mark(s, kMarkFill);
     hipLaunchKernelGGL((delaunay_kernel<S>), dim3(grid_for(n, 256)), dim3(256), 0, s, a);
-    if (!exact_ws.p) exact_ws.resize((size_t)kExactLanes * kExactWsDoubles);
     hipLaunchKernelGGL((delaunay_exact_kernel<S>), dim3(kExactLanes / 64), dim3(64), 0, s, a, exact_ws.p);
+    section_mark(s, kMarkDelaunay);
     exclusive_scan_u8(nelem_out.p, elemrow.p, n, scan_ws.p, s);
     exclusive_scan_u32(nfneig.p, row.p, n, scan_ws.p, s);
+    section_mark(s, kMarkScans);
     hipLaunchKernelGGL((compress_kernel<S>), dim3(grid_for(n, 256)), dim3(256), 0, s, a, row.p, elemrow.p, neig.p,
                        edge_is_bnd.p, elemglob.p, cap_edges, cap_elems);
+    section_mark(s, kMarkCompress);
     computed = true;
 }
 
@@ -592,6 +603,8 @@ void AlphaShape2D_get_elem(const AlphaShape2D* hc, uint32_t* elems) {
     run_void(h, [&] {
         const uint32_t ne = h->f64 ? h->a64.n_elems() : h->a32.n_elems();
         check_flags();
+        const uint32_t cap = h->f64 ? h->a64.cap_elems : h->a32.cap_elems;
+        if (ne > cap) throw std::runtime_error("triangle count exceeds the output capacity");
         d2h(elems, h->f64 ? h->a64.elemglob.p : h->a32.elemglob.p, 3ull * ne);
     });
 }
@@ -639,6 +652,7 @@ int32_t AlphaShape2D_get_edges(const AlphaShape2D* hc, uint32_t* row, uint32_t* 
         auto go = [&](auto& A) {
             const uint32_t ne = A.n_edges();
             check_flags();
+            if (ne > A.cap_edges) throw std::runtime_error("edge count exceeds the output capacity");
             if (row) d2h(row, A.row.p, A.n + 1);
             if (neig) d2h(neig, A.neig.p, ne);
             if (edge_is_bnd) d2h(edge_is_bnd, A.edge_is_bnd.p, ne);

@@ -39,6 +39,11 @@ __host__ __device__ inline size_t slab_index(uint32_t t, uint32_t j) {
 }
 inline size_t slab_words(uint32_t n) { return (size_t)((n + 63) / 64) * kSlabCap * 64; }
 
+// Directed edges reserved per node for the compressed output (the planar
+// bound is 6; Stream_tune("edge_capacity", k) lowers it, which makes the
+// capacity checks observable on small inputs).
+uint32_t& ashape2d_edge_capacity();
+
 template <class S> struct Ash2D {
     uint32_t n = 0;
     int bits = 32;

@@ -268,22 +268,28 @@ void LbvhGpu<S, D>::build(const Sph4<S>* d_obj, uint32_t n_, int bits_, uint32_t
     RadixWork w = radix_work_from(sortws.p, n);
     radix_reset(w, passes, s);
     const unsigned nb = grid_for(n, 256, kBoxBlocks);
+    section_mark(s, kMarkBuild0);
     hipLaunchKernelGGL((box_partials_kernel<S, D>), dim3(nb), dim3(256), 0, s, d_obj, n, partials.p);
     hipLaunchKernelGGL((morton_hist_kernel<S, D>), dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, d_obj, n,
                        partials.p, nb, gbox.p, bits, k0.p, v0.p, w.hist, passes);
+    section_mark(s, kMarkMorton);
     const bool in1 = radix_sort_pairs(k0.p, v0.p, k1.p, v1.p, n, passes * 8, w, ctx().d_flags, s);
+    section_mark(s, kMarkSort);
     codes_sorted = in1 ? k1.p : k0.p;
     perm = in1 ? v1.p : v0.p;
     hipLaunchKernelGGL((gather_kernel<S>), dim3(grid_for(n, 256)), dim3(256), 0, s, d_obj, perm, n, obj_m.p);
+    section_mark(s, kMarkGather);
     if (n < 2) return;
     hipLaunchKernelGGL(karras_kernel, dim3(grid_for(n - 1, 256)), dim3(256), 0, s, codes_sorted, (int)n, node.p,
                        leaf_parent.p, internal_parent.p, touched.p);
+    section_mark(s, kMarkKarras);
     if (leaf_mode == kLeafCentre)
         hipLaunchKernelGGL((refit_kernel<S, D, kLeafCentre>), dim3(grid_for(n, 256)), dim3(256), 0, s, obj_m.p, n,
                            node.p, leaf_parent.p, internal_parent.p, touched.p, ibox.p, ctx().d_flags);
     else
         hipLaunchKernelGGL((refit_kernel<S, D, kLeafSphere>), dim3(grid_for(n, 256)), dim3(256), 0, s, obj_m.p, n,
                            node.p, leaf_parent.p, internal_parent.p, touched.p, ibox.p, ctx().d_flags);
+    section_mark(s, kMarkRefit);
 }
 
 template struct LbvhGpu<float, 2>;

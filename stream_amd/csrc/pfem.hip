@@ -51,7 +51,8 @@ struct P1Args {
     double* idiag;
     float* idiag32;
     double* b;
-    uint32_t cap;  // capacity of acol / aval
+    uint32_t cap;        // capacity of acol / aval
+    uint32_t cap_edges;  // capacity of eneig (the mesh's edge list)
     uint32_t* flags;
 };
 
@@ -64,8 +65,10 @@ __global__ __launch_bounds__(256) void p1_assemble_kernel(P1Args a) {
     const uint32_t base = e0 + i;
     a.arow[i] = base;
     if (i == n - 1) a.arow[n] = a.erow[n] + n;
-    if (base + nf + 1 > a.cap) {
-        // only reachable after an edge-capacity overflow of the mesh
+    if (e0 + nf > a.cap_edges || base + nf + 1 > a.cap) {
+        // only reachable after an edge-capacity overflow of the mesh: the
+        // row's edges (or its matrix entries) would lie past the end of
+        // their arrays
         raise_flag(a.flags, kErrCapacity);
         return;
     }
@@ -344,9 +347,23 @@ struct Step2D {
         a.idiag32 = idiag32.p;
         a.b = b.p;
         a.cap = cap;
+        a.cap_edges = mesh.cap_edges;
         a.flags = ctx().d_flags;
+        section_mark(s, kMarkAsm0);
         hipLaunchKernelGGL(p1_assemble_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, a);
+        section_mark(s, kMarkAsm);
         A.nnz = read_scalar(A.row.p + n);
+        // a capacity overflow of the mesh (edges or triangles) or of the
+        // matrix leaves entries unwritten or past the end of A.col / A.val:
+        // stop before anything reads them
+        const uint32_t fl = read_scalar(ctx().d_flags + kFlagError);
+        if (fl || A.nnz > cap) {
+            fill<uint32_t>(ctx().d_flags, 1, 0u, s);
+            A.nnz = 0;
+            throw std::runtime_error(fl == kErrCapacity || fl == 0
+                                         ? "mesh or matrix capacity overflow (edges > 6n + 64 or triangles > 2n + 64)"
+                                         : "device error flag raised during meshing or assembly");
+        }
     }
 
     // SELL copy for the chosen solver (FP64 standard or FP32 row-sum form)
@@ -774,6 +791,10 @@ int32_t Stream_tune(const char* key, double value) {
             tuning().llt_threads = (uint32_t)value;
         } else if (k == "fold_group") {
             set_fold_group((uint32_t)value);
+        } else if (k == "edge_capacity") {
+            if (!(value >= 1.0 && value <= 6.0 && value == (double)(uint32_t)value))
+                throw std::invalid_argument("edge_capacity must be an integer in [1, 6]");
+            ashape2d_edge_capacity() = (uint32_t)value;
         } else if (k == "xcd_map") {
             if (!(value == 0.0 || value == 1.0)) throw std::invalid_argument("xcd_map must be 0 or 1");
             tuning().xcd_map = (uint32_t)value;
@@ -950,6 +971,34 @@ int32_t PfemStep2D_time_dir_kernel(PfemStep2D* h, int32_t kind, uint32_t reps, d
     });
 }
 
+// GPU time of the mesh-and-assembly stages on the current points, averaged
+// over `reps` runs (HIP events on the library stream between the stages'
+// launches).  ms [11]: global box + Morton codes + digit histograms, radix
+// sort (all passes), Morton-order gather, hierarchy, refit, candidate
+// search, local Delaunay + filter (+ exact redo), element/edge scans,
+// compress, P1 assembly kernel, whole mesh + assembly.
+int32_t PfemStep2D_time_sections(PfemStep2D* h, uint32_t reps, double* ms) {
+    return g([&] {
+        if (!h || !ms || !reps) throw std::invalid_argument("PfemStep2D_time_sections: bad arguments");
+        static const int a[11] = {kMarkBuild0, kMarkMorton, kMarkSort, kMarkGather, kMarkKarras, kMarkCompute0,
+                                  kMarkFill, kMarkDelaunay, kMarkScans, kMarkAsm0, kMarkBuild0};
+        static const int b[11] = {kMarkMorton, kMarkSort, kMarkGather, kMarkKarras, kMarkRefit, kMarkFill,
+                                  kMarkDelaunay, kMarkScans, kMarkCompress, kMarkAsm, kMarkAsm};
+        for (int k = 0; k < 11; ++k) ms[k] = 0.0;
+        struct Disarm {
+            ~Disarm() { sections_arm(false); }
+        } disarm;
+        for (uint32_t r = 0; r < reps; ++r) {
+            sections_arm(true);
+            h->s.mesh_and_assemble(h->bits, -1);
+            for (int k = 0; k < 11; ++k) ms[k] += sections_elapsed(a[k], b[k]);
+        }
+        sections_arm(false);
+        for (int k = 0; k < 11; ++k) ms[k] /= reps;
+        check_error_flag();
+    });
+}
+
 // Solution in input order [N].
 int32_t PfemStep2D_get_solution(PfemStep2D* h, double* p) {
     return g([&] { h->s.solution_input_order(p); });
@@ -991,7 +1040,11 @@ int32_t PfemStep2D_get_mesh(PfemStep2D* h, uint32_t* elem, uint8_t* node_is_bnd,
     return g([&] {
         auto& m = h->s.mesh;
         const uint32_t n = m.n;
-        if (elem) d2h(elem, m.elemglob.p, 3ull * m.n_elems());
+        if (elem) {
+            const uint32_t ne = m.n_elems();
+            if (ne > m.cap_elems) throw std::runtime_error("triangle count exceeds the mesh's capacity");
+            d2h(elem, m.elemglob.p, 3ull * ne);
+        }
         if (node_is_bnd) d2h(node_is_bnd, m.node_is_bnd.p, n);
         if (perm) d2h(perm, m.lbvh.perm, n);
         if (xy) {

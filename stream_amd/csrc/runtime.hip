@@ -46,6 +46,46 @@ Context& ctx() {
     return c;
 }
 
+namespace {
+struct Sections {
+    bool armed = false;
+    bool created = false;
+    hipEvent_t ev[kMaxMarks];
+    bool recorded[kMaxMarks] = {};
+};
+Sections& sections() {
+    static Sections t;
+    return t;
+}
+}  // namespace
+
+void sections_arm(bool on) {
+    Sections& t = sections();
+    if (on && !t.created) {
+        for (auto& e : t.ev) SA_HIP(hipEventCreate(&e));
+        t.created = true;
+    }
+    for (bool& r : t.recorded) r = false;
+    t.armed = on;
+}
+
+void section_mark(hipStream_t s, int id) {
+    Sections& t = sections();
+    if (!t.armed || id < 0 || id >= kMaxMarks) return;
+    SA_HIP(hipEventRecord(t.ev[id], s));
+    t.recorded[id] = true;
+}
+
+float sections_elapsed(int a, int b) {
+    Sections& t = sections();
+    if (a < 0 || b < 0 || a >= kMaxMarks || b >= kMaxMarks || !t.created || !t.recorded[a] || !t.recorded[b])
+        throw std::logic_error("section marks not recorded");
+    SA_HIP(hipEventSynchronize(t.ev[b]));
+    float ms = 0.f;
+    SA_HIP(hipEventElapsedTime(&ms, t.ev[a], t.ev[b]));
+    return ms;
+}
+
 }  // namespace stream_amd
 
 extern "C" {

@@ -118,6 +118,31 @@ template <class T> T read_scalar(const T* d) {
     return *(const T*)c.pinned;
 }
 
+// Section timer for per-kernel figures: while armed, section_mark(s, id)
+// records event `id` (0 .. kMaxMarks - 1) on stream s; unarmed it does
+// nothing.  sections_elapsed(a, b) is the time in ms from mark a to mark b
+// of the last armed run (both marks must have been recorded; synchronises).
+constexpr int kMaxMarks = 16;
+// Marks of the mesh-and-assemble step, in launch order.
+enum SectionMark : int {
+    kMarkBuild0 = 0,    // LBVH build starts
+    kMarkMorton = 1,    // global box + Morton codes + digit histograms done
+    kMarkSort = 2,      // radix sort passes done
+    kMarkGather = 3,    // Morton-order gather done
+    kMarkKarras = 4,    // hierarchy done
+    kMarkRefit = 5,     // refit done
+    kMarkCompute0 = 6,  // alpha-shape compute starts
+    kMarkFill = 7,      // candidate search done
+    kMarkDelaunay = 8,  // local Delaunay + filter (+ exact redo) done
+    kMarkScans = 9,     // element / edge scans done
+    kMarkCompress = 10, // compress done
+    kMarkAsm0 = 11,     // P1 assembly starts
+    kMarkAsm = 12,      // P1 assembly kernel done
+};
+void sections_arm(bool on);
+void section_mark(hipStream_t s, int id);
+float sections_elapsed(int a, int b);
+
 inline unsigned grid_for(size_t n, unsigned block, unsigned max_blocks = 65535u * 4u) {
     size_t g = (n + block - 1) / block;
     if (g == 0) g = 1;

@@ -17,7 +17,7 @@ import numpy as np
 import numpy.typing as npt
 
 from .. import StreamError, check, lib, libmesh
-from .._ct import arr, bind, i32, opt, u32, vp
+from .._ct import arr, bind, i32, node_arrays, opt, u32, vp
 
 AlphaShape2D_ptr = ctypes.c_void_p
 
@@ -68,8 +68,7 @@ class AlphaShape2D:
         check(AlphaShape2D_last_status(self.mesh))
 
     def set_nodes(self, nodes: npt.NDArray, alpha: npt.NDArray) -> None:
-        nodes = np.ascontiguousarray(nodes, dtype=self.dtype)
-        alpha = np.ascontiguousarray(alpha, dtype=self.dtype)
+        nodes, alpha = node_arrays(nodes, 2, self.dtype, alpha)
         self.nnodes = nodes.shape[0]
         if self.dtype == np.float32:
             AlphaShape2D_set_nodes(self.mesh, self.nnodes, nodes, alpha)

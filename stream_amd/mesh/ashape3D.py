@@ -17,7 +17,7 @@ import numpy as np
 import numpy.typing as npt
 
 from .. import StreamError, check, lib, libmesh
-from .._ct import arr, bind, i32, opt, u32, vp
+from .._ct import arr, bind, i32, node_arrays, opt, u32, vp
 
 AlphaShape3D_ptr = ctypes.c_void_p
 
@@ -69,8 +69,7 @@ class AlphaShape3D:
         check(AlphaShape3D_last_status(self.mesh))
 
     def set_nodes(self, nodes: npt.NDArray, alpha: npt.NDArray) -> None:
-        nodes = np.ascontiguousarray(nodes, dtype=self.dtype)
-        alpha = np.ascontiguousarray(alpha, dtype=self.dtype)
+        nodes, alpha = node_arrays(nodes, 3, self.dtype, alpha)
         self.nnodes = nodes.shape[0]
         if self.dtype == np.float32:
             AlphaShape3D_set_nodes(self.mesh, self.nnodes, nodes, alpha)

@@ -18,7 +18,7 @@ import numpy as np
 import numpy.typing as npt
 
 from .. import StreamError, check, lib, libmesh
-from .._ct import arr, bind, i32, opt, u32
+from .._ct import arr, bind, i32, node_arrays, opt, u32
 
 mesh2D_ptr = ctypes.c_void_p
 
@@ -63,7 +63,7 @@ class Mesh2D:
         self._status()
 
     def set_nodes(self, nodes: npt.NDArray[np.float32]) -> None:
-        nodes = np.ascontiguousarray(nodes, dtype=np.float32)
+        (nodes,) = node_arrays(nodes, 2, np.float32)
         self.nnodes = nodes.shape[0]
         mesh2D_set_nodes(self.mesh, self.nnodes, nodes)
         self._status()

@@ -8,7 +8,7 @@ import ctypes
 import numpy as np
 
 from . import StreamError, check, lib
-from ._ct import arr, bind, f64, i32, opt, u32
+from ._ct import arr, bind, f64, i32, node_arrays, opt, u32
 
 _p = ctypes.c_void_p
 PfemStep2D_create = bind(lib, "PfemStep2D_create", _p, [])
@@ -34,6 +34,11 @@ PfemStep2D_set_learned = bind(lib, "PfemStep2D_set_learned", i32, [_p, _p])
 
 PfemStep2D_time_dir_kernel = bind(lib, "PfemStep2D_time_dir_kernel", i32,
                                   [_p, i32, u32, ctypes.POINTER(ctypes.c_double)])
+PfemStep2D_time_sections = bind(lib, "PfemStep2D_time_sections", i32, [_p, u32, arr(np.float64, 1, True)])
+
+# stage names of PfemStep2D.time_sections, in launch order
+SECTION_NAMES = ("morton", "sort", "gather", "karras", "refit", "fill", "delaunay", "scans", "compress", "p1",
+                 "mesh_and_assembly")
 
 SOLVER_FP64_JACOBI = 0
 SOLVER_MIXED_JACOBI = 1
@@ -48,11 +53,10 @@ class PfemStep2D:
         self.h = PfemStep2D_create()
         if not self.h:
             raise StreamError(lib.Stream_last_error().decode())
-        xy = np.ascontiguousarray(xy, np.float64)
+        xy, alpha, f = node_arrays(xy, 2, np.float64, alpha, f)
         self.n = xy.shape[0]
         check(PfemStep2D_set_morton_bits(self.h, morton_bits))
-        check(PfemStep2D_set_points(self.h, self.n, xy, np.ascontiguousarray(alpha, np.float64),
-                                    np.ascontiguousarray(f, np.float64)))
+        check(PfemStep2D_set_points(self.h, self.n, xy, alpha, f))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -69,11 +73,10 @@ class PfemStep2D:
 
     def set_points(self, xy, alpha, f):
         """Replace the points (same count or a new one), alpha and host source."""
-        xy = np.ascontiguousarray(xy, np.float64)
+        xy, alpha, f = node_arrays(xy, 2, np.float64, alpha, f)
         self.n = xy.shape[0]
         check(PfemStep2D_set_source_modes(self.h, 0, None))
-        check(PfemStep2D_set_points(self.h, self.n, xy, np.ascontiguousarray(alpha, np.float64),
-                                    np.ascontiguousarray(f, np.float64)))
+        check(PfemStep2D_set_points(self.h, self.n, xy, alpha, f))
 
     def assemble(self):
         """Mesh and assemble without solving; returns (mesh ms, assembly ms)."""
@@ -120,6 +123,13 @@ class PfemStep2D:
         ms = ctypes.c_double(0.0)
         check(PfemStep2D_time_dir_kernel(self.h, 3, reps, ctypes.byref(ms)))
         return ms.value
+
+    def time_sections(self, reps: int = 5) -> dict:
+        """GPU ms of the mesh-and-assembly stages on the current points
+        (averaged over reps runs), keyed by SECTION_NAMES."""
+        ms = np.zeros(len(SECTION_NAMES))
+        check(PfemStep2D_time_sections(self.h, reps, ms))
+        return dict(zip(SECTION_NAMES, ms.tolist()))
 
     def solution(self, morton=False):
         p = np.empty(self.n)
@@ -171,11 +181,10 @@ class PfemStep3D:
         self.h = PfemStep3D_create()
         if not self.h:
             raise StreamError(lib.Stream_last_error().decode())
-        xyz = np.ascontiguousarray(xyz, np.float64)
+        xyz, alpha, f = node_arrays(xyz, 3, np.float64, alpha, f)
         self.n = xyz.shape[0]
         check(PfemStep3D_set_morton_bits(self.h, morton_bits))
-        check(PfemStep3D_set_points(self.h, self.n, xyz, np.ascontiguousarray(alpha, np.float64),
-                                    np.ascontiguousarray(f, np.float64)))
+        check(PfemStep3D_set_points(self.h, self.n, xyz, alpha, f))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -204,10 +213,9 @@ class PfemStep3D:
         return float(st[0]), float(st[1])
 
     def set_points(self, xyz, alpha, f):
-        xyz = np.ascontiguousarray(xyz, np.float64)
+        xyz, alpha, f = node_arrays(xyz, 3, np.float64, alpha, f)
         self.n = xyz.shape[0]
-        check(PfemStep3D_set_points(self.h, self.n, xyz, np.ascontiguousarray(alpha, np.float64),
-                                    np.ascontiguousarray(f, np.float64)))
+        check(PfemStep3D_set_points(self.h, self.n, xyz, alpha, f))
 
     def advect(self, modes, dt: float, steps: int = 1):
         """Advect the points on the device in the divergence-free field of

@@ -116,6 +116,8 @@ def _declare(L):
     L.orc_ashape2d_free.argtypes = [vp, i32]
     L.orc_assemble_p1.restype = ctypes.c_int64
     L.orc_assemble_p1.argtypes = [u32, vp, u32, vp, vp, vp, u64, vp, vp, vp, vp]
+    L.orc_step_solve_f64.restype = None
+    L.orc_step_solve_f64.argtypes = [u32, vp, vp, f64, f64, u32, vp, vp, vp, vp]
     L.orc_jacobi_pcg.restype = u32
     L.orc_jacobi_pcg.argtypes = [u32, vp, vp, vp, vp, vp, f64, u32, vp]
     for tag, ft in (("f32", ctypes.c_float), ("f64", f64)):
@@ -459,3 +461,21 @@ def super_nodes(xym):
     a1 = f(0.5) * (hi[1] + lo[1])
     mr = f(max(hi[0] - lo[0], hi[1] - lo[1])) + f(10)
     return np.array([[a0 - mr, a1 - mr], [a0 + mr, a1 - mr], [a0, a1 + mr]], np.float32)
+
+
+def step_solve(coords, alpha, f, rtol=1e-8, ref_rtol=1e-11, maxit=200000):
+    """The oracle's timed step (mesh + P1 + FP64 Jacobi-PCG to rtol from 0)
+    and an untimed reference solve continued from it to ref_rtol.  Returns
+    dict(times_ms=[mesh, assembly, cg], iters, rel, ref_iters, ref_rel,
+    triangles, nnz, x_ref (Morton order))."""
+    L = lib()
+    n = coords.shape[0]
+    xyr = spheres(coords, alpha, np.float64)
+    fa = np.ascontiguousarray(f, np.float64)
+    times = np.zeros(3)
+    sizes = np.zeros(2, np.uint64)
+    x = np.zeros(n)
+    info = np.zeros(4)
+    L.orc_step_solve_f64(n, _p(xyr), _p(fa), rtol, ref_rtol, maxit, _p(times), _p(sizes), _p(x), _p(info))
+    return dict(times_ms=times.tolist(), iters=int(info[0]), rel=float(info[1]), ref_iters=int(info[2]),
+                ref_rel=float(info[3]), triangles=int(sizes[0]), nnz=int(sizes[1]), x_ref=x)

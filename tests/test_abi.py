@@ -56,3 +56,34 @@ def test_python_package_imports_and_reports_no_device_here():
     assert stream_amd.device_count() >= 0
     from stream_amd.mesh import AlphaShape2D  # noqa: F401
     from stream_amd.numerics import DeviceCSR, HostCSR, LinSys, PrecJacobi, SolverCG  # noqa: F401
+
+
+def test_wrappers_refuse_mismatched_node_arrays():
+    """Node arrays of the wrong width and per-node arrays of the wrong length
+    are refused in Python, before the native library reads n rows of them."""
+    import numpy as np
+
+    from stream_amd._ct import node_arrays
+
+    xy = np.zeros((10, 2))
+    a, b = node_arrays(xy, 2, np.float64, np.ones(10))
+    assert a.shape == (10, 2) and b.shape == (10,)
+    for bad in (np.zeros((10, 3)), np.zeros(20)):
+        with pytest.raises(ValueError):
+            node_arrays(bad, 2, np.float64)
+    with pytest.raises(ValueError):
+        node_arrays(xy, 2, np.float64, np.ones(9))
+    with pytest.raises(ValueError):
+        node_arrays(np.zeros((10, 2)), 3, np.float64)
+
+
+def test_noalloc_scan_refuses_oversize_without_a_device():
+    """The allocation-free scan checks its size limit on the host before any
+    HIP call, so the refusal is observable without a GPU."""
+    import stream_amd
+
+    lib = stream_amd.lib
+    lib.Stream_exclusive_scan_u32_noalloc.restype = ctypes.c_int32
+    lib.Stream_exclusive_scan_u32_noalloc.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                                      ctypes.c_void_p]
+    assert lib.Stream_exclusive_scan_u32_noalloc((1 << 28) + 1, None, None, None) == 2

@@ -373,3 +373,51 @@ def test_step3d_learned_beats_jacobi():
     assert not rl["fallback"]
     assert rl["true_rel"] <= 1e-8 and rj["true_rel"] <= 1e-8
     assert rl["iters"] < rj["iters"]
+
+
+def _adam_f32(w, m, v, g, t, lr, b1, b2, eps):
+    """One Adam step in FP32 in the kernel's operation order (no fused
+    multiply-add): m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g g,
+    w -= lr (m / c1) / (sqrt(v / c2) + eps), c1 = 1 - b1^t, c2 = 1 - b2^t."""
+    f = np.float32
+    b1, b2, lr, eps = f(b1), f(b2), f(lr), f(eps)
+    c1 = f(1.0) - f(np.power(b1, f(t), dtype=np.float32))
+    c2 = f(1.0) - f(np.power(b2, f(t), dtype=np.float32))
+    m = (b1 * m + (f(1.0) - b1) * g).astype(np.float32)
+    v = (b2 * v + (f(1.0) - b2) * g * g).astype(np.float32)
+    w = (w - lr * (m / c1) / (np.sqrt(v / c2) + eps)).astype(np.float32)
+    return w, m, v
+
+
+def test_adam_steps_match_oracle(small):
+    """k training steps on the GPU (forward, loss, backward, adam_kernel)
+    against Adam applied on the host to the GPU's own gradient at each step:
+    an FP32 restatement in the kernel's operation order (a few ulps: sqrt
+    and the bias-correction powers may round differently) and the oracle's
+    FP64 Adam (oracle/mpnn.py Adam, pinned by hand-worked values in
+    test_mpnn_oracle.py) within 1e-4 lr.  The gradient itself is compared
+    with the oracle's backward pass in test_gradient_matches_oracle."""
+    from stream_amd.learned import PrecLearned
+
+    row, col, val, b, xy, bnd = small
+    lr, b1, b2, eps = 2e-3, 0.9, 0.999, 1e-8
+    net = PrecLearned()
+    net.set_params(_params())
+    net.set_adam(b1, b2, eps)
+    net.set_system(row, col, val, xy, bnd)
+    w = net.params().copy()
+    m = np.zeros_like(w)
+    v = np.zeros_like(w)
+    ref = mpnn.Adam(len(w), lr=lr, b1=b1, b2=b2, eps=eps)
+    w64 = w.astype(np.float64)
+    for t in range(1, 4):
+        _, g = net.gradient()
+        net.train_step(lr, read_loss=False)
+        got = net.params()
+        w, m, v = _adam_f32(w, m, v, g, t, lr, b1, b2, eps)
+        ulp = np.spacing(np.maximum(np.abs(w), np.abs(got)).astype(np.float32))
+        assert np.all(np.abs(got - w) <= 4 * ulp), (t, np.max(np.abs(got - w) / ulp))
+        w64 = ref.step(w64, g.astype(np.float64))
+        assert np.max(np.abs(got.astype(np.float64) - w64)) <= 1e-4 * lr + 4 * float(np.max(ulp)), t
+    # every step moved the parameters with non-zero gradient by about lr
+    assert np.median(np.abs(got - _params())[np.abs(g) > 1e-3]) > 0.5 * lr

@@ -194,3 +194,30 @@ def test_training_lowers_loss(graph):
         p = opt.step(p, mpnn.flatten(grad))
     assert abs(losses[0]) < 1e-12
     assert losses[-1] < -0.02
+
+
+def test_adam_hand_worked_steps():
+    """The oracle's Adam against values worked by hand (Kingma & Ba,
+    Algorithm 1, with bias correction).
+    * Constant gradient g: m_t = (1 - b1^t) g and v_t = (1 - b2^t) g^2, so
+      the corrected moments are g and g^2 at every t and each step moves w
+      by lr g / (|g| + eps): after k steps w_k = w_0 - k lr g / (|g| + eps).
+    * Gradients +1 then -1 (b1 = 0.9, b2 = 0.999): step 1 moves by
+      -lr / (1 + eps); at t = 2, m = 0.9 * 0.1 - 0.1 = -0.01 and
+      1 - b1^2 = 0.19, so m_hat = -1/19; v = 0.999 * 0.001 + 0.001 =
+      0.001999 = 1 - b2^2, so v_hat = 1: step 2 moves by +lr (1/19)/(1 + eps).
+      Without the bias correction step 2 would move by
+      lr * 0.01 / (sqrt(0.001999) + eps) = 0.2237 lr instead of 0.0526 lr."""
+    lr, eps = 1e-2, 1e-8
+    g = np.array([0.5, -2.0, 1e-3, 3.0])
+    opt = mpnn.Adam(4, lr=lr, eps=eps)
+    w = np.zeros(4)
+    for _ in range(5):
+        w = opt.step(w, g)
+    assert np.allclose(w, -5 * lr * g / (np.abs(g) + eps), rtol=1e-12, atol=0)
+
+    opt = mpnn.Adam(1, lr=lr, b1=0.9, b2=0.999, eps=eps)
+    w1 = opt.step(np.zeros(1), np.ones(1))
+    assert np.allclose(w1, -lr / (1 + eps), rtol=1e-14, atol=0)
+    w2 = opt.step(w1, -np.ones(1))
+    assert np.allclose(w2 - w1, lr * (1.0 / 19.0) / (1 + eps), rtol=1e-12, atol=0)
