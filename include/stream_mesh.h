@@ -52,6 +52,17 @@ int32_t AlphaShape2D_get_edges(const AlphaShape2D* ashape, uint32_t* row, uint32
                                uint8_t* node_is_bnd);
 /* perm[i] = input index of Morton slot i; getPermutation, DirectAlphaShape2D.cpp:75-80 */
 int32_t AlphaShape2D_get_permutation(const AlphaShape2D* ashape, uint32_t* perm);
+/* Nodal fields between input order and Morton order: permuteForward /
+ * permuteBackward (DirectAlphaShape2D.hpp:220-242), forward out[i] =
+ * in[perm[i]], backward out[perm[i]] = in[i]; nnodes elements of
+ * elem_bytes each.  Host arrays, or device buffers (_device: on the
+ * library stream, not synchronised; in and out must not overlap). */
+int32_t AlphaShape2D_permute_forward(const AlphaShape2D* ashape, const void* in, void* out, uint32_t elem_bytes);
+int32_t AlphaShape2D_permute_backward(const AlphaShape2D* ashape, const void* in, void* out, uint32_t elem_bytes);
+int32_t AlphaShape2D_permute_forward_device(const AlphaShape2D* ashape, const void* d_in, void* d_out,
+                                            uint32_t elem_bytes);
+int32_t AlphaShape2D_permute_backward_device(const AlphaShape2D* ashape, const void* d_in, void* d_out,
+                                             uint32_t elem_bytes);
 /* nodes whose local triangulation needed the exact predicate stage */
 uint32_t AlphaShape2D_get_nexact(const AlphaShape2D* ashape);
 /* status of the last call on the handle (0 = ok) */
@@ -91,6 +102,13 @@ uint32_t AlphaShape3D_get_nedge(const AlphaShape3D* ashape);
 int32_t AlphaShape3D_get_edges(const AlphaShape3D* ashape, uint32_t* row, uint32_t* neig, uint8_t* edge_is_bnd,
                                uint8_t* node_is_bnd);
 int32_t AlphaShape3D_get_permutation(const AlphaShape3D* ashape, uint32_t* perm);
+/* permuteForward / permuteBackward of DirectAlphaShape3D.hpp (as the 2D ones) */
+int32_t AlphaShape3D_permute_forward(const AlphaShape3D* ashape, const void* in, void* out, uint32_t elem_bytes);
+int32_t AlphaShape3D_permute_backward(const AlphaShape3D* ashape, const void* in, void* out, uint32_t elem_bytes);
+int32_t AlphaShape3D_permute_forward_device(const AlphaShape3D* ashape, const void* d_in, void* d_out,
+                                            uint32_t elem_bytes);
+int32_t AlphaShape3D_permute_backward_device(const AlphaShape3D* ashape, const void* d_in, void* d_out,
+                                             uint32_t elem_bytes);
 /* nodes that needed the exact stage; nodes whose candidate list was clamped
  * to 124 (the reference's 128 entries with the 4 infinity nodes, cpp:369) */
 uint32_t AlphaShape3D_get_nexact(const AlphaShape3D* ashape);

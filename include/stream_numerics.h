@@ -115,6 +115,13 @@ int32_t PfemStep3D_get_system(PfemStep3D* step, uint32_t* row, uint32_t* col, do
 int32_t PfemStep3D_get_mesh(PfemStep3D* step, uint32_t* elem, uint8_t* node_is_bnd, double* xyz, uint32_t* perm);
 int32_t PfemStep2D_get_solution(PfemStep2D* step, double* p);
 int32_t PfemStep2D_get_solution_morton(PfemStep2D* step, double* p);
+/* nodal fields between input order and the Morton order of the step's last
+ * mesh (permuteForward / permuteBackward, DirectAlphaShape2D.hpp:220-242);
+ * host arrays of n elements of elem_bytes each */
+int32_t PfemStep2D_permute_forward(PfemStep2D* step, const void* in, void* out, uint32_t elem_bytes);
+int32_t PfemStep2D_permute_backward(PfemStep2D* step, const void* in, void* out, uint32_t elem_bytes);
+int32_t PfemStep3D_permute_forward(PfemStep3D* step, const void* in, void* out, uint32_t elem_bytes);
+int32_t PfemStep3D_permute_backward(PfemStep3D* step, const void* in, void* out, uint32_t elem_bytes);
 uint32_t PfemStep2D_nnz(PfemStep2D* step);
 int32_t PfemStep2D_get_system(PfemStep2D* step, uint32_t* row, uint32_t* col, double* val, double* b);
 int32_t PfemStep2D_get_mesh(PfemStep2D* step, uint32_t* elem, uint8_t* node_is_bnd, double* xy, uint32_t* perm);

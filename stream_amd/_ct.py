@@ -59,3 +59,20 @@ def node_arrays(nodes, dim, dtype, *per_node):
             raise ValueError(f"per-node array must have shape ({n},), got {a.shape}")
         out.append(a)
     return out
+
+
+def permute_field(fn, handle, field, n):
+    """Call a native permute_forward / permute_backward entry point on a
+    field of n nodes (shape (n, ...), any dtype) and return the permuted
+    copy; each node's element is field.itemsize * prod(shape[1:]) bytes."""
+    f = np.ascontiguousarray(field)
+    if f.ndim < 1 or f.shape[0] != n:
+        raise ValueError(f"field must have {n} rows, got shape {f.shape}")
+    out = np.empty_like(f)
+    eb = f.itemsize * int(np.prod(f.shape[1:], dtype=np.int64))
+    if n and eb:
+        rc = fn(handle, f.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p), eb)
+        from . import check
+
+        check(rc)
+    return out

@@ -689,4 +689,49 @@ uint32_t AlphaShape2D_get_nexact(const AlphaShape2D* hc) {
 // Stream_last_error().
 int32_t AlphaShape2D_last_status(const AlphaShape2D* h) { return h ? h->status : kErrArgument; }
 
+
+// Nodal fields between input order and Morton order (permuteForward /
+// permuteBackward, include/mesh/DirectAlphaShape2D.hpp): forward out[i] =
+// in[perm[i]] gives the field in the Morton order of the outputs, backward
+// out[perm[i]] = in[i] returns it to input order.  n elements of
+// elem_bytes each; host arrays (copied) or device buffers (_device, on the
+// library stream, not synchronised).  Needs init (the permutation).
+}  // extern "C"
+
+namespace {
+template <class F> int32_t permute_call(const AlphaShape2D* hc, F&& f) {
+    AlphaShape2D* h = const_cast<AlphaShape2D*>(hc);
+    return run_status(h, [&] {
+        const bool ok = h->f64 ? h->a64.inited : h->a32.inited;
+        if (!ok) throw std::invalid_argument("AlphaShape2D: permutation needs init (set_nodes)");
+        const uint32_t* perm = h->f64 ? h->a64.lbvh.perm : h->a32.lbvh.perm;
+        const uint32_t n = h->f64 ? h->a64.n : h->a32.n;
+        f(perm, n);
+    });
+}
+}  // namespace
+
+extern "C" {
+
+int32_t AlphaShape2D_permute_forward(const AlphaShape2D* h, const void* in, void* out, uint32_t elem_bytes) {
+    return permute_call(h, [&](const uint32_t* perm, uint32_t n) {
+        permute_host_elements(in, out, perm, n, elem_bytes, true);
+    });
+}
+int32_t AlphaShape2D_permute_backward(const AlphaShape2D* h, const void* in, void* out, uint32_t elem_bytes) {
+    return permute_call(h, [&](const uint32_t* perm, uint32_t n) {
+        permute_host_elements(in, out, perm, n, elem_bytes, false);
+    });
+}
+int32_t AlphaShape2D_permute_forward_device(const AlphaShape2D* h, const void* d_in, void* d_out, uint32_t elem_bytes) {
+    return permute_call(h, [&](const uint32_t* perm, uint32_t n) {
+        permute_elements(d_in, d_out, perm, n, elem_bytes, true, ctx().stream);
+    });
+}
+int32_t AlphaShape2D_permute_backward_device(const AlphaShape2D* h, const void* d_in, void* d_out, uint32_t elem_bytes) {
+    return permute_call(h, [&](const uint32_t* perm, uint32_t n) {
+        permute_elements(d_in, d_out, perm, n, elem_bytes, false, ctx().stream);
+    });
+}
+
 }  // extern "C"

@@ -717,4 +717,49 @@ uint32_t AlphaShape3D_get_nclamped(const AlphaShape3D* hc) {
 
 int32_t AlphaShape3D_last_status(const AlphaShape3D* h) { return h ? h->status : kErrArgument; }
 
+
+// Nodal fields between input order and Morton order (permuteForward /
+// permuteBackward, include/mesh/DirectAlphaShape3D.hpp): forward out[i] =
+// in[perm[i]] gives the field in the Morton order of the outputs, backward
+// out[perm[i]] = in[i] returns it to input order.  n elements of
+// elem_bytes each; host arrays (copied) or device buffers (_device, on the
+// library stream, not synchronised).  Needs init (the permutation).
+}  // extern "C"
+
+namespace {
+template <class F> int32_t permute_call(const AlphaShape3D* hc, F&& f) {
+    AlphaShape3D* h = const_cast<AlphaShape3D*>(hc);
+    return run_status3(h, [&] {
+        const bool ok = h->f64 ? h->a64.inited : h->a32.inited;
+        if (!ok) throw std::invalid_argument("AlphaShape3D: permutation needs init (set_nodes)");
+        const uint32_t* perm = h->f64 ? h->a64.lbvh.perm : h->a32.lbvh.perm;
+        const uint32_t n = h->f64 ? h->a64.n : h->a32.n;
+        f(perm, n);
+    });
+}
+}  // namespace
+
+extern "C" {
+
+int32_t AlphaShape3D_permute_forward(const AlphaShape3D* h, const void* in, void* out, uint32_t elem_bytes) {
+    return permute_call(h, [&](const uint32_t* perm, uint32_t n) {
+        permute_host_elements(in, out, perm, n, elem_bytes, true);
+    });
+}
+int32_t AlphaShape3D_permute_backward(const AlphaShape3D* h, const void* in, void* out, uint32_t elem_bytes) {
+    return permute_call(h, [&](const uint32_t* perm, uint32_t n) {
+        permute_host_elements(in, out, perm, n, elem_bytes, false);
+    });
+}
+int32_t AlphaShape3D_permute_forward_device(const AlphaShape3D* h, const void* d_in, void* d_out, uint32_t elem_bytes) {
+    return permute_call(h, [&](const uint32_t* perm, uint32_t n) {
+        permute_elements(d_in, d_out, perm, n, elem_bytes, true, ctx().stream);
+    });
+}
+int32_t AlphaShape3D_permute_backward_device(const AlphaShape3D* h, const void* d_in, void* d_out, uint32_t elem_bytes) {
+    return permute_call(h, [&](const uint32_t* perm, uint32_t n) {
+        permute_elements(d_in, d_out, perm, n, elem_bytes, false, ctx().stream);
+    });
+}
+
 }  // extern "C"

@@ -690,4 +690,23 @@ int32_t PfemStep3D_get_mesh(PfemStep3D* h, uint32_t* elem, uint8_t* node_is_bnd,
 
 uint32_t PfemStep3D_get_nelem(PfemStep3D* h) { return h ? h->s.mesh.n_elems() : 0u; }
 
+
+// Nodal fields of the current points between input order and the Morton
+// order of the step's mesh, matrix and solution (permuteForward /
+// permuteBackward of the alpha-shape): forward out[i] = in[perm[i]],
+// backward out[perm[i]] = in[i]; host arrays of n elements of elem_bytes.
+// Needs a meshed step.
+int32_t PfemStep3D_permute_forward(PfemStep3D* h, const void* in, void* out, uint32_t elem_bytes) {
+    return g3([&] {
+        if (!h || !h->s.mesh.inited || h->s.mesh.n != h->s.n) throw std::invalid_argument("PfemStep3D: no mesh for the current points");
+        permute_host_elements(in, out, h->s.mesh.lbvh.perm, h->s.n, elem_bytes, true);
+    });
+}
+int32_t PfemStep3D_permute_backward(PfemStep3D* h, const void* in, void* out, uint32_t elem_bytes) {
+    return g3([&] {
+        if (!h || !h->s.mesh.inited || h->s.mesh.n != h->s.n) throw std::invalid_argument("PfemStep3D: no mesh for the current points");
+        permute_host_elements(in, out, h->s.mesh.lbvh.perm, h->s.n, elem_bytes, false);
+    });
+}
+
 }  // extern "C"

@@ -309,6 +309,66 @@ bool radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, ui
     return (passes % 2) == 1;
 }
 
+// one thread per word of an element; W words of type T per element
+template <class T>
+__global__ __launch_bounds__(256) void permute_words_kernel(const T* __restrict__ in, T* __restrict__ out,
+                                                            const uint32_t* __restrict__ perm, uint32_t n, uint32_t W,
+                                                            int forward) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t total = (uint64_t)n * W;
+    if (t >= total) return;
+    const uint64_t i = t / W, k = t - i * W;
+    const uint64_t p = perm[i];
+    if (forward)
+        out[i * W + k] = in[p * W + k];
+    else
+        out[p * W + k] = in[i * W + k];
+}
+
+template <class T>
+static void permute_words(const void* in, void* out, const uint32_t* perm, uint32_t n, uint32_t W, bool forward,
+                          hipStream_t s) {
+    const uint64_t total = (uint64_t)n * W;
+    const uint64_t blocks = (total + 255) / 256;
+    if (blocks > 0x7fffffffull) throw std::invalid_argument("permutation too large");
+    hipLaunchKernelGGL((permute_words_kernel<T>), dim3((uint32_t)blocks), dim3(256), 0, s, (const T*)in, (T*)out,
+                       perm, n, W, forward ? 1 : 0);
+}
+
+void permute_elements(const void* in, void* out, const uint32_t* perm, uint32_t n, uint32_t elem_bytes, bool forward,
+                      hipStream_t s) {
+    if (n == 0) return;
+    if (elem_bytes == 0) throw std::invalid_argument("element size must be positive");
+    if (in == out) throw std::invalid_argument("permutation in place is not supported (in == out)");
+    // widest word that divides the element and both addresses
+    const uintptr_t al = (uintptr_t)in | (uintptr_t)out;
+    for (uint32_t w : {16u, 8u, 4u, 2u, 1u}) {
+        if (elem_bytes % w || al % w) continue;
+        const uint32_t W = elem_bytes / w;
+        switch (w) {
+            case 16: permute_words<uint4>(in, out, perm, n, W, forward, s); return;
+            case 8: permute_words<uint2>(in, out, perm, n, W, forward, s); return;
+            case 4: permute_words<uint32_t>(in, out, perm, n, W, forward, s); return;
+            case 2: permute_words<uint16_t>(in, out, perm, n, W, forward, s); return;
+            default: permute_words<uint8_t>(in, out, perm, n, W, forward, s); return;
+        }
+    }
+}
+
+void permute_host_elements(const void* in, void* out, const uint32_t* d_perm, uint32_t n, uint32_t elem_bytes,
+                           bool forward) {
+    if (n == 0) return;
+    if (!in || !out) throw std::invalid_argument("null field array");
+    if (elem_bytes == 0) throw std::invalid_argument("element size must be positive");
+    const size_t bytes = (size_t)n * elem_bytes;
+    DBuf<uint8_t> din, dout;
+    din.resize(bytes);
+    dout.resize(bytes);
+    h2d(din.p, (const uint8_t*)in, bytes);
+    permute_elements(din.p, dout.p, d_perm, n, elem_bytes, forward, ctx().stream);
+    d2h((uint8_t*)out, dout.p, bytes);
+}
+
 __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ a, uint4* __restrict__ b, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         b[i] = a[i];

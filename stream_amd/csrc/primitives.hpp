@@ -131,6 +131,17 @@ template <class T> void fill(T* p, size_t n, T v, hipStream_t s) {
     hipLaunchKernelGGL(fill_kernel<T>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, p, n, v);
 }
 
+// Field permutation between input order and Morton order (the reference's
+// permuteForward / permuteBackward, include/mesh/DirectAlphaShape2D.hpp:
+// 220-242): forward out[i] = in[perm[i]], backward out[perm[i]] = in[i],
+// perm[i] = input index of Morton slot i.  Elements are elem_bytes wide;
+// in and out are device buffers of n elements that do not overlap.
+void permute_elements(const void* in, void* out, const uint32_t* perm, uint32_t n, uint32_t elem_bytes, bool forward,
+                      hipStream_t s);
+// The same on host arrays (copied to the device and back; synchronises).
+void permute_host_elements(const void* in, void* out, const uint32_t* d_perm, uint32_t n, uint32_t elem_bytes,
+                           bool forward);
+
 // Host side: read the context's error word (synchronises the stream); when
 // set, clear it and throw.
 inline void check_error_flag() {

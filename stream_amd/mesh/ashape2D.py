@@ -17,7 +17,7 @@ import numpy as np
 import numpy.typing as npt
 
 from .. import StreamError, check, lib, libmesh
-from .._ct import arr, bind, i32, node_arrays, opt, u32, vp
+from .._ct import arr, bind, i32, node_arrays, opt, permute_field, u32, vp
 
 AlphaShape2D_ptr = ctypes.c_void_p
 
@@ -46,6 +46,8 @@ AlphaShape2D_get_permutation = bind(libmesh, "AlphaShape2D_get_permutation", i32
                                     [AlphaShape2D_ptr, arr(np.uint32, 1, writeable=True)])
 AlphaShape2D_get_nexact = bind(libmesh, "AlphaShape2D_get_nexact", u32, [AlphaShape2D_ptr])
 AlphaShape2D_last_status = bind(libmesh, "AlphaShape2D_last_status", i32, [AlphaShape2D_ptr])
+AlphaShape2D_permute_forward = bind(libmesh, "AlphaShape2D_permute_forward", i32, [AlphaShape2D_ptr, vp, vp, u32])
+AlphaShape2D_permute_backward = bind(libmesh, "AlphaShape2D_permute_backward", i32, [AlphaShape2D_ptr, vp, vp, u32])
 
 
 class AlphaShape2D:
@@ -117,6 +119,16 @@ class AlphaShape2D:
         nb = np.empty(self.nnodes, np.uint8)
         check(AlphaShape2D_get_edges(self.mesh, None, None, None, nb))
         return nb
+
+    def permute_forward(self, field: npt.NDArray) -> npt.NDArray:
+        """Nodal field (shape (n, ...)) from input order to the Morton order
+        of the outputs: out[i] = field[perm[i]] (permuteForward)."""
+        return permute_field(AlphaShape2D_permute_forward, self.mesh, field, self.nnodes)
+
+    def permute_backward(self, field: npt.NDArray) -> npt.NDArray:
+        """Nodal field from Morton order back to input order:
+        out[perm[i]] = field[i] (permuteBackward)."""
+        return permute_field(AlphaShape2D_permute_backward, self.mesh, field, self.nnodes)
 
     def get_permutation(self) -> npt.NDArray[np.uint32]:
         """perm[i] = input index of Morton slot i."""

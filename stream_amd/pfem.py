@@ -8,7 +8,7 @@ import ctypes
 import numpy as np
 
 from . import StreamError, check, lib
-from ._ct import arr, bind, f64, i32, node_arrays, opt, u32
+from ._ct import arr, bind, f64, i32, node_arrays, opt, permute_field, u32
 
 _p = ctypes.c_void_p
 PfemStep2D_create = bind(lib, "PfemStep2D_create", _p, [])
@@ -34,6 +34,10 @@ PfemStep2D_set_learned = bind(lib, "PfemStep2D_set_learned", i32, [_p, _p])
 
 PfemStep2D_time_dir_kernel = bind(lib, "PfemStep2D_time_dir_kernel", i32,
                                   [_p, i32, u32, ctypes.POINTER(ctypes.c_double)])
+PfemStep2D_permute_forward = bind(lib, "PfemStep2D_permute_forward", i32, [_p, _p, _p, u32])
+PfemStep2D_permute_backward = bind(lib, "PfemStep2D_permute_backward", i32, [_p, _p, _p, u32])
+PfemStep3D_permute_forward = bind(lib, "PfemStep3D_permute_forward", i32, [_p, _p, _p, u32])
+PfemStep3D_permute_backward = bind(lib, "PfemStep3D_permute_backward", i32, [_p, _p, _p, u32])
 PfemStep2D_time_sections = bind(lib, "PfemStep2D_time_sections", i32, [_p, u32, arr(np.float64, 1, True)])
 
 # stage names of PfemStep2D.time_sections, in launch order
@@ -131,6 +135,16 @@ class PfemStep2D:
         check(PfemStep2D_time_sections(self.h, reps, ms))
         return dict(zip(SECTION_NAMES, ms.tolist()))
 
+    def permute_forward(self, field):
+        """Nodal field (shape (n, ...)) from input order to the Morton order
+        of the last mesh, system and solution: out[i] = field[perm[i]]."""
+        return permute_field(PfemStep2D_permute_forward, self.h, field, self.n)
+
+    def permute_backward(self, field):
+        """Nodal field from the Morton order of the last mesh back to input
+        order: out[perm[i]] = field[i]."""
+        return permute_field(PfemStep2D_permute_backward, self.h, field, self.n)
+
     def solution(self, morton=False):
         p = np.empty(self.n)
         check((PfemStep2D_get_solution_morton if morton else PfemStep2D_get_solution)(self.h, p))
@@ -227,6 +241,16 @@ class PfemStep3D:
         xyz = np.empty((self.n, 3))
         check(PfemStep3D_get_points(self.h, xyz))
         return xyz
+
+    def permute_forward(self, field):
+        """Nodal field (shape (n, ...)) from input order to the Morton order
+        of the last mesh, system and solution: out[i] = field[perm[i]]."""
+        return permute_field(PfemStep3D_permute_forward, self.h, field, self.n)
+
+    def permute_backward(self, field):
+        """Nodal field from the Morton order of the last mesh back to input
+        order: out[perm[i]] = field[i]."""
+        return permute_field(PfemStep3D_permute_backward, self.h, field, self.n)
 
     def solution(self):
         p = np.empty(self.n)
