@@ -154,6 +154,32 @@ def section_bytes(n, nnz, tris, edges):
     }
 
 
+def imr26_leg(runs=5):
+    """The reference's published alpha-shape timings (testcases/IMR26/
+    README.md:49-54, 94-99: 2D 1M uniform points at alpha 0.0012 and 3D
+    100k at alpha 0.022, FP32, seed 42, ttot = tlbvh + tashape) measured the
+    same way here (stream_amd.imr26): the median over runs after the first,
+    GPU ms, with the host wall time of init + compute beside it."""
+    import numpy as np
+
+    from stream_amd import imr26
+
+    out = {}
+    for dim, n, alpha, ref_ms, ref_count in ((2, 1_000_000, 0.0012, 22.35, 1_877_465),
+                                              (3, 100_000, 0.022, 28.0, 526_040)):
+        rows = imr26.run_uniform(dim, n, alpha, runs)[1:]
+        med = {k: float(np.median([r[k] for r in rows])) for k in ("tlbvh", "tashape", "tcompress", "ttot", "wall")}
+        out[f"uniform{dim}D"] = {"npoints": n, "alpha": alpha, "elements": rows[-1]["ntri"],
+                                 "reference_elements": ref_count,
+                                 **{k: round(v, 4) for k, v in med.items()},
+                                 "reference_ttot_ms": ref_ms,
+                                 "reference_hardware": "RTX 4060 (testcases/IMR26/README.md:8)",
+                                 "speedup_ttot": round(ref_ms / med["ttot"], 2)}
+        print(f"[bench] imr26 uniform{dim}D n={n}: ttot {med['ttot']:.3f} ms (reference {ref_ms} ms), "
+              f"wall {med['wall']:.3f} ms, elements {rows[-1]['ntri']}", file=sys.stderr, flush=True)
+    return out
+
+
 def tank3d_leg(steps=2, weights3d=None):
     """The 3D step on the sloshing-tank cloud (1,996,920 FP64 points,
     alpha = 1.3h): LBVH + alpha-shape tetrahedra + P1 + CG to rtol 1e-8,
@@ -498,6 +524,8 @@ def main():
         step.run(solver, rtol=1e-8)
         sols[solver] = step.solution(morton=True)
 
+    imr = imr26_leg()
+
     tank = None
     if args.tank3d:
         tank = tank3d_leg()
@@ -548,6 +576,7 @@ def main():
                            "steps": k_pcie, "note": "points, alpha and source copied host-to-device every step"},
         "err_vs_fp64_reference": err,
         "mesh_stages": stages,
+        "imr26": imr,
         "roofline": roofline,
         "mfma": mfma,
         "tank3d": tank,

@@ -21,6 +21,10 @@ extern "C" {
 const char* Stream_last_error(void);
 int32_t Stream_device_count(void);
 int32_t Stream_synchronize(void);
+/* the reference benchmarks' point generator (testcases/IMR26/uniform2D.cpp:
+ * 130-141, uniform3D.cpp): srand(seed), (float)rand()/RAND_MAX per
+ * coordinate; out [n][dim], dim 2 or 3 (host only) */
+int32_t Stream_uniform_cloud_f32(uint32_t n, uint32_t dim, uint32_t seed, float* out);
 /* no reference counterpart: device bytes held by the library now / at the
  * high-water mark (reset_peak != 0 restarts it), device free / total memory */
 int32_t Stream_device_bytes(uint64_t* current, uint64_t* peak, int32_t reset_peak, uint64_t* dev_free,
@@ -87,6 +91,12 @@ int32_t TRBVH3D_collisions_f32(uint32_t n, const float* xyzr, int32_t rounds, ui
  * n items (32 slots), pop all into out_idx/out_cost; *pushed = accepted. */
 int32_t Stream_heap_order(uint32_t n, const uint32_t* idx, const float* cost, uint32_t* out_idx, float* out_cost,
                           int32_t* pushed);
+
+/* no reference counterpart (verification hook): the exact sign of orient2d
+ * (kind 0, 3 points x 2), incircle (1, 4 x 2), orient3d (2, 4 x 3) or
+ * insphere (3, 5 x 3) by the device exact stage compiled for the host; pts
+ * float (fp64 = 0) or double (predicates.hpp:342-1100 semantics) */
+int32_t Stream_exact_sign_host(int32_t kind, int32_t fp64, const void* pts, int32_t* sign);
 
 /* stable onesweep radix sort of (u64, u32) pairs over the low `bits` bits */
 int32_t Stream_sort_pairs_u64(uint32_t n, const uint64_t* keys, const uint32_t* vals, uint64_t* keys_out,

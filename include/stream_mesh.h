@@ -67,6 +67,13 @@ int32_t AlphaShape2D_permute_backward_device(const AlphaShape2D* ashape, const v
 uint32_t AlphaShape2D_get_nexact(const AlphaShape2D* ashape);
 /* status of the last call on the handle (0 = ok) */
 int32_t AlphaShape2D_last_status(const AlphaShape2D* ashape);
+/* no reference counterpart (the timed loop of testcases/IMR26/uniform2D.cpp:
+ * 159-183): init + compute with ms [8] = LBVH build, candidate search +
+ * local Delaunay + filter + scans, compaction (GPU times), host wall time of
+ * the two calls, GPU time from build start to compaction end; then, within
+ * the second, the candidate search, the local Delaunay of every node and the
+ * exact redo of undecided nodes */
+int32_t AlphaShape2D_timed_run(AlphaShape2D* ashape, double* ms);
 
 /* ======================== 3D alpha-shape ======================== */
 typedef struct AlphaShape3D AlphaShape3D;
@@ -114,6 +121,8 @@ int32_t AlphaShape3D_permute_backward_device(const AlphaShape3D* ashape, const v
 uint32_t AlphaShape3D_get_nexact(const AlphaShape3D* ashape);
 uint32_t AlphaShape3D_get_nclamped(const AlphaShape3D* ashape);
 int32_t AlphaShape3D_last_status(const AlphaShape3D* ashape);
+/* the same for testcases/IMR26/uniform3D.cpp */
+int32_t AlphaShape3D_timed_run(AlphaShape3D* ashape, double* ms);
 
 /* Full 2D Delaunay (include/mesh/mesh.hpp:23-62, src/mesh/mesh.cpp:800-862),
  * float nodes.  Triangles and node ids are in Morton order, as in the

@@ -92,6 +92,19 @@ int32_t PfemStep2D_set_source_modes(PfemStep2D* step, uint32_t n_modes, const do
 int32_t PfemStep2D_advect(PfemStep2D* step, uint32_t n_modes, const double* modes, double dt, uint32_t steps);
 /* current points [N][2], input order */
 int32_t PfemStep2D_get_points(PfemStep2D* step, double* xy);
+/* no reference counterpart (the PFEM velocity step named as a goal in the
+ * reference README.md:12-18): nodal velocity [N][2] in input order, one
+ * fractional (projection) step -- predictor u* = u + dt (g - nu M^-1 K u),
+ * pressure K p = -(rho/dt) D(u*) with p = 0 on free-surface nodes solved by
+ * `solver` (as PfemStep2D_run) to rtol, correction u = u* - (dt/rho) M^-1
+ * G(p), then (move != 0) the points move by dt u; g [2]; stats [11] =
+ * PfemStep2D_run's ten + ms of the velocity kernels -- and the last step's
+ * fields in Morton order (u*, new velocity, lumped mass, D(u_new)) */
+int32_t PfemStep2D_set_velocity(PfemStep2D* step, const double* uv);
+int32_t PfemStep2D_get_velocity(PfemStep2D* step, double* uv);
+int32_t PfemStep2D_projection_step(PfemStep2D* step, int32_t solver, double dt, double rho, double nu, const double* g,
+                                   double rtol, uint32_t maxit, int32_t move, double* stats);
+int32_t PfemStep2D_get_flow_fields(PfemStep2D* step, double* us, double* unew, double* mass, double* div);
 
 /* 3D mesh-and-solve step: LBVH + alpha-shape tetrahedra + P1 pressure
  * assembly on tetrahedra + CG (solvers 0, 1, 2 as above). */

@@ -98,6 +98,17 @@ def fourier_modes(seed: int = 11, n_modes: int = 8):
     return np.stack([kx, ky, phi, np.cos(th), np.sin(th)], axis=1)
 
 
+def fourier_velocity(xy: np.ndarray, modes: np.ndarray) -> np.ndarray:
+    """u = sum_m a_m sin(k_m . x + phi_m) at the points, shape (n, 2): the
+    velocity whose divergence divergence_source gives."""
+    u = np.zeros((xy.shape[0], 2))
+    for kx, ky, phi, ax, ay in modes:
+        s = np.sin(kx * xy[:, 0] + ky * xy[:, 1] + phi)
+        u[:, 0] += ax * s
+        u[:, 1] += ay * s
+    return u
+
+
 def divergence_source(xy: np.ndarray, modes: np.ndarray) -> np.ndarray:
     """f = div u = sum_m (a_m . k_m) cos(k_m . x + phi_m) at the points."""
     f = np.zeros(xy.shape[0])

@@ -1,4 +1,5 @@
 // 2D alpha-shape kernels and the AlphaShape2D C-ABI (see ashape2d.hpp).
+#include <chrono>
 #include <vector>
 
 #include "ashape2d.hpp"
@@ -164,6 +165,7 @@ template <class S, bool EXACT> __device__ bool local_delaunay(const AshArgs<S>& 
     const S pa[2] = {P[si].c[0], P[si].c[1]};
     uint64_t t0, t1, t2, t3;
     uint32_t tlast = 3;
+    bool over = false;  // an exact expansion outgrew the workspace
     for (uint32_t ni = kNInf2D; ni < nn; ++ni) {
         t0 = t1 = t2 = t3 = 0;
         const uint32_t sj = neig(ni);
@@ -184,7 +186,7 @@ template <class S, bool EXACT> __device__ bool local_delaunay(const AshArgs<S>& 
                 if constexpr (!EXACT)
                     return false;
                 else
-                    orientation = (S)orient2d_exact(pa, pb, pc, ws);
+                    orientation = (S)orient2d_exact(pa, pb, pc, ws, kExactWsDoubles, &over);
             }
             S err;
             S det = incircle_filter(pa, pb, pc, pd, &err);
@@ -212,18 +214,18 @@ template <class S, bool EXACT> __device__ bool local_delaunay(const AshArgs<S>& 
                     cs(i1, i2, qa, qb);
                     cs(i3, i4, qc, qd);
                     cs(i2, i3, qb, qc);
-                    det = (S)incircle_exact(qa, qb, qc, qd, ws);
+                    det = (S)incircle_exact(qa, qb, qc, qd, ws, kExactWsDoubles, &over);
                     int depth = 0;
                     while (det == (S)0) {
                         ++depth;
                         if (depth == 1) {
                             bool c1;
                             det = orient2d_filter(qb, qc, qd, &c1);
-                            if (!c1) det = (S)orient2d_exact(qb, qc, qd, ws);
+                            if (!c1) det = (S)orient2d_exact(qb, qc, qd, ws, kExactWsDoubles, &over);
                         } else if (depth == 2) {
                             bool c2;
                             det = orient2d_filter(qa, qc, qd, &c2);
-                            if (!c2) det = (S)orient2d_exact(qa, qc, qd, ws);
+                            if (!c2) det = (S)orient2d_exact(qa, qc, qd, ws, kExactWsDoubles, &over);
                             det = -det;
                         } else {
                             det = ((int)swap + (int)(orientation < (S)0)) == 1 ? (S)1 : (S)-1;
@@ -311,6 +313,7 @@ template <class S, bool EXACT> __device__ bool local_delaunay(const AshArgs<S>& 
     a.nelem_out[si] = (uint8_t)nout;
     a.nelem[si] = (uint8_t)nloc;
     a.nfneig[si] = (uint32_t)(__popcll(t0) + __popcll(t1) + __popcll(t2) + __popcll(t3));
+    if (over) raise_flag(a.flags, kErrCapacity);
     return true;
 }
 
@@ -323,11 +326,15 @@ template <class S> __global__ __launch_bounds__(256) void delaunay_kernel(AshArg
     }
 }
 
-template <class S> __global__ __launch_bounds__(64) void delaunay_exact_kernel(AshArgs<S> a, double* ws_base) {
-    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
-    double* ws = ws_base + (size_t)lane * kExactWsDoubles;
+// One queued node per wavefront (lane 0; the block is one wavefront): the
+// exact evaluations of different nodes happen at different points of their
+// insertions, so nodes sharing a wavefront would run their exact stages one
+// after another.  The expansion workspace lives in LDS.
+template <class S> __global__ __launch_bounds__(64) void delaunay_exact_kernel(AshArgs<S> a) {
+    __shared__ double ws[kExactWsDoubles];
+    if (threadIdx.x != 0) return;
     const uint32_t cnt = a.queue[0];
-    for (uint32_t k = lane; k < cnt; k += gridDim.x * blockDim.x) local_delaunay<S, true>(a, a.queue[1 + k], ws);
+    for (uint32_t k = blockIdx.x; k < cnt; k += gridDim.x) local_delaunay<S, true>(a, a.queue[1 + k], ws);
 }
 
 // Slab -> CSR (cpp:642-689) with capacity checks.
@@ -452,17 +459,18 @@ template <class S> void Ash2D<S>::init(hipStream_t s) {
 template <class S> void Ash2D<S>::compute(hipStream_t s) {
     if (!inited) throw std::invalid_argument("AlphaShape2D_compute called before init");
     AshArgs<S> a = make_args(*this);
-    if (!exact_ws.p) exact_ws.resize((size_t)kExactLanes * kExactWsDoubles);
     section_mark(s, kMarkCompute0);
     fill<uint32_t>(queue.p, 1, 0u, s);
     hipLaunchKernelGGL((fill_kernel<S>), dim3(grid_for(n, 256)), dim3(256), 0, s, a);
     section_mark(s, kMarkFill);
     hipLaunchKernelGGL((delaunay_kernel<S>), dim3(grid_for(n, 256)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL((delaunay_exact_kernel<S>), dim3(kExactLanes / 64), dim3(64), 0, s, a, exact_ws.p);
+    section_mark(s, kMarkDelaunayMain);
+    hipLaunchKernelGGL((delaunay_exact_kernel<S>), dim3(kExactWaves), dim3(64), 0, s, a);
     section_mark(s, kMarkDelaunay);
     exclusive_scan_u8(nelem_out.p, elemrow.p, n, scan_ws.p, s);
     exclusive_scan_u32(nfneig.p, row.p, n, scan_ws.p, s);
     section_mark(s, kMarkScans);
+    section_mark(s, kMarkCompress0);
     hipLaunchKernelGGL((compress_kernel<S>), dim3(grid_for(n, 256)), dim3(256), 0, s, a, row.p, elemrow.p, neig.p,
                        edge_is_bnd.p, elemglob.p, cap_edges, cap_elems);
     section_mark(s, kMarkCompress);
@@ -731,6 +739,45 @@ int32_t AlphaShape2D_permute_forward_device(const AlphaShape2D* h, const void* d
 int32_t AlphaShape2D_permute_backward_device(const AlphaShape2D* h, const void* d_in, void* d_out, uint32_t elem_bytes) {
     return permute_call(h, [&](const uint32_t* perm, uint32_t n) {
         permute_elements(d_in, d_out, perm, n, elem_bytes, false, ctx().stream);
+    });
+}
+
+
+// One init + compute timed as testcases/IMR26/uniform2d.cpp times them:
+// ms[0] tlbvh (LBVH build), ms[1] tashape (candidate search, local
+// Delaunay + filter, element/edge scans), ms[2] tcompress (compaction to
+// CSR), ms[3] the host wall time of both calls, ms[4] the GPU time from the
+// build's start to the compaction's end; within tashape, ms[5] the
+// candidate search, ms[6] the local Delaunay + filter of every node, ms[7]
+// the exact redo of the nodes whose filters could not decide a sign.  GPU times come from HIP events
+// on the library stream between the launches.
+int32_t AlphaShape2D_timed_run(AlphaShape2D* h, double* ms) {
+    return run_status(h, [&] {
+        if (!ms) throw std::invalid_argument("AlphaShape2D_timed_run: null ms");
+        struct Disarm {
+            ~Disarm() { sections_arm(false); }
+        } disarm;
+        sections_arm(true);
+        const auto t0 = std::chrono::steady_clock::now();
+        if (h->f64) {
+            h->a64.init(ctx().stream);
+            h->a64.compute(ctx().stream);
+        } else {
+            h->a32.init(ctx().stream);
+            h->a32.compute(ctx().stream);
+        }
+        SA_HIP(hipStreamSynchronize(ctx().stream));
+        const auto t1 = std::chrono::steady_clock::now();
+        ms[0] = sections_elapsed(kMarkBuild0, kMarkRefit);
+        ms[1] = sections_elapsed(kMarkCompute0, kMarkScans);
+        ms[2] = sections_elapsed(kMarkCompress0, kMarkCompress);
+        ms[3] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        ms[4] = sections_elapsed(kMarkBuild0, kMarkCompress);
+        ms[5] = sections_elapsed(kMarkCompute0, kMarkFill);
+        ms[6] = sections_elapsed(kMarkFill, kMarkDelaunayMain);
+        ms[7] = sections_elapsed(kMarkDelaunayMain, kMarkDelaunay);
+        sections_arm(false);
+        check_flags();
     });
 }
 

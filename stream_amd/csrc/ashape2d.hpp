@@ -32,7 +32,7 @@ namespace stream_amd {
 constexpr uint32_t kSlabCap = 256;
 constexpr uint32_t kNInf2D = 3;
 constexpr uint32_t kMaxFill2D = 250;  // cpp:351
-constexpr uint32_t kExactLanes = 4096;
+constexpr uint32_t kExactWaves = 4096;  // wavefronts of the exact redo (one queued node each at a time)
 
 __host__ __device__ inline size_t slab_index(uint32_t t, uint32_t j) {
     return ((size_t)(t >> 6) * kSlabCap + j) * 64 + (t & 63u);
@@ -57,7 +57,6 @@ template <class S> struct Ash2D {
     DBuf<uint32_t> neig, elemglob;
     DBuf<uint8_t> edge_is_bnd;
     DBuf<uint32_t> queue;     // [0] = count, then node ids
-    DBuf<double> exact_ws;    // kExactLanes * kExactWsDoubles
     DBuf<uint64_t> scan_ws;
     uint32_t cap_edges = 0, cap_elems = 0;
     bool inited = false, computed = false;

@@ -1,4 +1,5 @@
 // 3D alpha-shape kernels and the AlphaShape3D C-ABI (see ashape3d.hpp).
+#include <chrono>
 #include <vector>
 
 #include "ashape3d.hpp"
@@ -390,12 +391,16 @@ template <class S> __global__ __launch_bounds__(64) void delaunay3_kernel(Ash3Ar
     }
 }
 
-template <class S> __global__ __launch_bounds__(64) void delaunay3_exact_kernel(Ash3Args<S> a, double* ws_base) {
+// One queued node per wavefront (lane 0; the block is one wavefront): the
+// exact evaluations of different nodes happen at different points of their
+// insertions, so nodes sharing a wavefront would run their exact stages one
+// after another.  The expansion workspace lives in LDS.
+template <class S> __global__ __launch_bounds__(64) void delaunay3_exact_kernel(Ash3Args<S> a) {
     __shared__ uint8_t cyc[kCycle3D * 64];
-    const uint32_t lane = blockIdx.x * 64 + threadIdx.x;
-    double* ws = ws_base + (size_t)lane * kExactWs3D;
+    __shared__ double ws[kExactWs3D];
+    if (threadIdx.x != 0) return;
     const uint32_t cnt = a.queue[0];
-    for (uint32_t k = lane; k < cnt; k += gridDim.x * 64) local_delaunay3<S, true>(a, a.queue[2 + k], cyc, ws);
+    for (uint32_t k = blockIdx.x; k < cnt; k += gridDim.x) local_delaunay3<S, true>(a, a.queue[2 + k], cyc, ws);
 }
 
 // Slab -> CSR (cpp:692-735): edges with count <= 3 are boundary edges; the
@@ -494,21 +499,27 @@ template <class S> void Ash3D<S>::init(hipStream_t s) {
 template <class S> void Ash3D<S>::compute(hipStream_t s) {
     if (!inited) throw std::invalid_argument("AlphaShape3D_compute called before init");
     Ash3Args<S> a = make_args3(*this);
+    section_mark(s, kMarkCompute0);
     fill<uint32_t>(queue.p, 2, 0u, s);
     hipLaunchKernelGGL((fill3_kernel<S>), dim3(grid_for(n, 256)), dim3(256), 0, s, a);
+    section_mark(s, kMarkFill);
     hipLaunchKernelGGL((delaunay3_kernel<S>), dim3((n + 63) / 64), dim3(64), 0, s, a);
-    if (!exact_ws.p) exact_ws.resize((size_t)kExactLanes3D * kExactWs3D);
-    hipLaunchKernelGGL((delaunay3_exact_kernel<S>), dim3(kExactLanes3D / 64), dim3(64), 0, s, a, exact_ws.p);
+    section_mark(s, kMarkDelaunayMain);
+    hipLaunchKernelGGL((delaunay3_exact_kernel<S>), dim3(kExactWaves3D), dim3(64), 0, s, a);
+    section_mark(s, kMarkDelaunay);
     exclusive_scan_u8(nelem_out.p, elemrow.p, n, scan_ws.p, s);
     exclusive_scan_u32(nfneig.p, row.p, n, scan_ws.p, s);
+    section_mark(s, kMarkScans);
     n_elems_ = read_scalar(elemrow.p + n);
     n_edges_ = read_scalar(row.p + n);
     neig.resize(n_edges_ ? n_edges_ : 1);
     edge_is_bnd.resize(n_edges_ ? n_edges_ : 1);
     elemglob.resize(4ull * (n_elems_ ? n_elems_ : 1));
     a = make_args3(*this);
+    section_mark(s, kMarkCompress0);
     hipLaunchKernelGGL((compress3_kernel<S>), dim3(grid_for(n, 256)), dim3(256), 0, s, a, row.p, elemrow.p, neig.p,
                        edge_is_bnd.p, elemglob.p);
+    section_mark(s, kMarkCompress);
     computed = true;
 }
 
@@ -759,6 +770,45 @@ int32_t AlphaShape3D_permute_forward_device(const AlphaShape3D* h, const void* d
 int32_t AlphaShape3D_permute_backward_device(const AlphaShape3D* h, const void* d_in, void* d_out, uint32_t elem_bytes) {
     return permute_call(h, [&](const uint32_t* perm, uint32_t n) {
         permute_elements(d_in, d_out, perm, n, elem_bytes, false, ctx().stream);
+    });
+}
+
+
+// One init + compute timed as testcases/IMR26/uniform3d.cpp times them:
+// ms[0] tlbvh (LBVH build), ms[1] tashape (candidate search, local
+// Delaunay + filter, element/edge scans), ms[2] tcompress (compaction to
+// CSR), ms[3] the host wall time of both calls, ms[4] the GPU time from the
+// build's start to the compaction's end; within tashape, ms[5] the
+// candidate search, ms[6] the local Delaunay + filter of every node, ms[7]
+// the exact redo of the nodes whose filters could not decide a sign.  GPU times come from HIP events
+// on the library stream between the launches.
+int32_t AlphaShape3D_timed_run(AlphaShape3D* h, double* ms) {
+    return run_status3(h, [&] {
+        if (!ms) throw std::invalid_argument("AlphaShape3D_timed_run: null ms");
+        struct Disarm {
+            ~Disarm() { sections_arm(false); }
+        } disarm;
+        sections_arm(true);
+        const auto t0 = std::chrono::steady_clock::now();
+        if (h->f64) {
+            h->a64.init(ctx().stream);
+            h->a64.compute(ctx().stream);
+        } else {
+            h->a32.init(ctx().stream);
+            h->a32.compute(ctx().stream);
+        }
+        SA_HIP(hipStreamSynchronize(ctx().stream));
+        const auto t1 = std::chrono::steady_clock::now();
+        ms[0] = sections_elapsed(kMarkBuild0, kMarkRefit);
+        ms[1] = sections_elapsed(kMarkCompute0, kMarkScans);
+        ms[2] = sections_elapsed(kMarkCompress0, kMarkCompress);
+        ms[3] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        ms[4] = sections_elapsed(kMarkBuild0, kMarkCompress);
+        ms[5] = sections_elapsed(kMarkCompute0, kMarkFill);
+        ms[6] = sections_elapsed(kMarkFill, kMarkDelaunayMain);
+        ms[7] = sections_elapsed(kMarkDelaunayMain, kMarkDelaunay);
+        sections_arm(false);
+        check_flags3();
     });
 }
 

@@ -25,7 +25,7 @@ constexpr uint32_t kNInf3D = 4;
 constexpr uint32_t kCand3D = 124;    // real candidates per node
 constexpr uint32_t kTet3D = 256;     // local tetrahedra per node
 constexpr uint32_t kCycle3D = 128;   // local ids (4 infinity + 124)
-constexpr uint32_t kExactLanes3D = 4096;
+constexpr uint32_t kExactWaves3D = 4096;  // wavefronts of the exact redo (one queued node each at a time)
 constexpr uint32_t kExactWs3D = 512;  // doubles per lane (expansion components)
 
 __host__ __device__ inline size_t slab3(uint32_t t, uint32_t j, uint32_t cap) {
@@ -46,7 +46,6 @@ template <class S> struct Ash3D {
     DBuf<uint32_t> neig, elemglob;
     DBuf<uint8_t> edge_is_bnd;
     DBuf<uint32_t> queue;   // [0] = count, [1] = candidate overflows, then node ids
-    DBuf<double> exact_ws;  // kExactLanes3D * kExactWs3D
     DBuf<uint64_t> scan_ws;
     uint32_t n_elems_ = 0, n_edges_ = 0;
     bool inited = false, computed = false;

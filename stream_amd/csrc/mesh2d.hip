@@ -61,7 +61,7 @@ __device__ __forceinline__ M2Star m2_star(uint16_t* elem, uint32_t* neig, uint32
 // components, far below the 64-double workspace.
 __device__ __noinline__ float m2_orient_exact(const float* a, const float* b, const float* c) {
     double ws[64];
-    return (float)orient2d_exact(a, b, c, ws);
+    return (float)orient2d_exact(a, b, c, ws, 64);
 }
 __device__ __forceinline__ float m2_orient(const float* a, const float* b, const float* c) {
     bool certain;
@@ -97,7 +97,7 @@ __device__ __noinline__ float m2_incircle_sos_exact(uint32_t i1, uint32_t i2, ui
     cs(i1, i2, qa, qb);
     cs(i3, i4, qc, qd);
     cs(i2, i3, qb, qc);
-    float det = (float)incircle_exact(qa, qb, qc, qd, ws);
+    float det = (float)incircle_exact(qa, qb, qc, qd, ws, 64);
     int depth = 0;
     while (det == 0.0f) {
         ++depth;

@@ -220,6 +220,162 @@ __global__ __launch_bounds__(256) void source_kernel(uint32_t n, Modes md, const
     f[i] = s;
 }
 
+// ---------------------------------------------------------------------------
+// Fractional-step (projection) velocity update on the P1 alpha-shape mesh,
+// lumped mass.  With K_ij = sum_T |T| grad phi_i . grad phi_j (the assembled
+// pressure matrix before its Dirichlet rows), m_i = sum_T |T| / 3,
+// D_i(u) = sum_T (|T| / 3) div u_h|T and G_i(p) = sum_T (|T| / 3) grad p_h|T
+// over the triangles T of node i:
+//   predictor   u*_i = u_i + dt (g - nu (K u)_i / m_i),
+//   pressure    K p = -(rho / dt) D(u*), p = 0 on free-surface nodes,
+//   correction  u_i <- u*_i - (dt / rho) G_i(p) / m_i.
+// A node without triangles (m_i = 0) keeps u*_i = u_i + dt g.  Each node
+// sums its own triangles, the vertices of each in ascending id order, so
+// the results do not depend on scheduling.
+// ---------------------------------------------------------------------------
+struct FlowArgs {
+    uint32_t n;
+    const Sph4<double>* P;  // Morton-ordered records
+    const uint32_t* cand;
+    const uint16_t* tri;
+    const uint8_t* nelem;
+    const uint8_t* bnd;
+    const double2* u;   // velocity (Morton)
+    double2* us;        // u*
+    double* mass;
+    const double* p;    // pressure (Morton)
+    double2* u_new;
+    double* b;          // pressure right-hand side
+    double* div;        // D(u) of the field `u` (flow_div_kernel with div != null)
+    double dt, rho, nu, gx, gy;
+};
+
+// Vertices (ascending ids), area and basis gradients of triangle (i, ga, gb).
+struct P1Tri {
+    uint32_t v[3];
+    double area;
+    double gx[3], gy[3];
+};
+__device__ __forceinline__ P1Tri p1_tri(const Sph4<double>* P, uint32_t i, uint32_t ga, uint32_t gb) {
+    P1Tri t;
+    t.v[0] = i;
+    t.v[1] = ga;
+    t.v[2] = gb;
+    if (t.v[1] < t.v[0]) { const uint32_t x = t.v[0]; t.v[0] = t.v[1]; t.v[1] = x; }
+    if (t.v[2] < t.v[1]) { const uint32_t x = t.v[1]; t.v[1] = t.v[2]; t.v[2] = x; }
+    if (t.v[1] < t.v[0]) { const uint32_t x = t.v[0]; t.v[0] = t.v[1]; t.v[1] = x; }
+    double x[3], y[3];
+    for (int k = 0; k < 3; ++k) {
+        const Sph4<double> s = P[t.v[k]];
+        x[k] = s.c[0];
+        y[k] = s.c[1];
+    }
+    const double a2 = (x[1] - x[0]) * (y[2] - y[0]) - (y[1] - y[0]) * (x[2] - x[0]);  // signed 2|T|
+    t.area = 0.5 * fabs(a2);
+    const double inv = 1.0 / a2;
+    t.gx[0] = (y[1] - y[2]) * inv;
+    t.gy[0] = (x[2] - x[1]) * inv;
+    t.gx[1] = (y[2] - y[0]) * inv;
+    t.gy[1] = (x[0] - x[2]) * inv;
+    t.gx[2] = (y[0] - y[1]) * inv;
+    t.gy[2] = (x[1] - x[0]) * inv;
+    return t;
+}
+
+template <class F> __device__ __forceinline__ void for_node_triangles(const FlowArgs& a, uint32_t i, F&& f) {
+    auto gneig = [&](uint32_t j) -> uint32_t { return j < kNInf2D ? a.n + j : a.cand[slab_index(i, j - kNInf2D)]; };
+    const uint32_t nt = a.nelem[i];
+    for (uint32_t t = 0; t < nt; ++t) {
+        const uint16_t e = a.tri[slab_index(i, t)];
+        f(p1_tri(a.P, i, gneig(e & 255u), gneig(e >> 8)));
+    }
+}
+
+__device__ __forceinline__ int local_of(const P1Tri& t, uint32_t i) { return t.v[0] == i ? 0 : (t.v[1] == i ? 1 : 2); }
+
+__global__ __launch_bounds__(256) void flow_predict_kernel(FlowArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double2 ui = a.u[i];
+    double m = 0.0, kx = 0.0, ky = 0.0;
+    for_node_triangles(a, i, [&](const P1Tri& t) {
+        const int li = local_of(t, i);
+        m += t.area / 3.0;
+        for (int k = 0; k < 3; ++k) {
+            if (k == li) continue;
+            const double kij = t.area * (t.gx[li] * t.gx[k] + t.gy[li] * t.gy[k]);
+            const double2 uk = a.u[t.v[k]];
+            kx += kij * (uk.x - ui.x);  // rows of K sum to zero: (K u)_i = sum_j K_ij (u_j - u_i)
+            ky += kij * (uk.y - ui.y);
+        }
+    });
+    double2 us;
+    us.x = ui.x + a.dt * a.gx;
+    us.y = ui.y + a.dt * a.gy;
+    if (m > 0.0) {
+        us.x -= a.dt * a.nu * kx / m;
+        us.y -= a.dt * a.nu * ky / m;
+    }
+    a.us[i] = us;
+    a.mass[i] = m;
+}
+
+// b_i = -(rho / dt) D_i(u*) on free nodes, 0 on free-surface nodes; with
+// a.div set, D_i(a.u) instead (diagnostics).
+__global__ __launch_bounds__(256) void flow_div_kernel(FlowArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double2* u = a.div ? a.u : a.us;
+    double d = 0.0;
+    for_node_triangles(a, i, [&](const P1Tri& t) {
+        double dv = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double2 uk = u[t.v[k]];
+            dv += uk.x * t.gx[k] + uk.y * t.gy[k];
+        }
+        d += t.area / 3.0 * dv;
+    });
+    if (a.div)
+        a.div[i] = d;
+    else
+        a.b[i] = a.bnd[i] ? 0.0 : -(a.rho / a.dt) * d;
+}
+
+__global__ __launch_bounds__(256) void flow_correct_kernel(FlowArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double gx = 0.0, gy = 0.0;
+    for_node_triangles(a, i, [&](const P1Tri& t) {
+        double px = 0.0, py = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double pk = a.p[t.v[k]];
+            px += pk * t.gx[k];
+            py += pk * t.gy[k];
+        }
+        gx += t.area / 3.0 * px;
+        gy += t.area / 3.0 * py;
+    });
+    double2 u = a.us[i];
+    const double m = a.mass[i];
+    if (m > 0.0) {
+        u.x -= (a.dt / a.rho) * gx / m;
+        u.y -= (a.dt / a.rho) * gy / m;
+    }
+    a.u_new[i] = u;
+}
+
+// Lagrangian move of the particles: x += dt u (input order).
+__global__ __launch_bounds__(256) void move_kernel(uint32_t n, double dt, const double2* __restrict__ u,
+                                                   Sph4<double>* __restrict__ pts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Sph4<double> p = pts[i];
+    const double2 v = u[i];
+    p.c[0] += dt * v.x;
+    p.c[1] += dt * v.y;
+    pts[i] = p;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -243,6 +399,11 @@ struct Step2D {
     PrecLearned* learned = nullptr;  // network of solver 3 (not owned)
     bool learned_fallback = false;   // the last learned factor was rejected
     Modes source;                    // device source f(x) (n = 0: host-given f)
+    // projection step: nodal velocity (input order), Morton copies, mass
+    DBuf<double2> u_in, u_m, us_m, unew_m;
+    DBuf<double> mass, div_m;
+    bool has_velocity = false;
+    double t_flow = 0;
 
     // Point the network's graph at this step's system (after assembly).
     void bind_graph(MpnnGraph& g, hipStream_t s) {
@@ -271,7 +432,104 @@ struct Step2D {
         hipLaunchKernelGGL(source_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, source, pts.p, f_in.p);
     }
 
+    FlowArgs flow_args() {
+        FlowArgs a;
+        a.n = n;
+        a.P = mesh.lbvh.obj_m.p;
+        a.cand = mesh.cand.p;
+        a.tri = mesh.tri.p;
+        a.nelem = mesh.nelem.p;
+        a.bnd = mesh.node_is_bnd.p;
+        a.u = u_m.p;
+        a.us = us_m.p;
+        a.mass = mass.p;
+        a.p = x.p;
+        a.u_new = unew_m.p;
+        a.b = b.p;
+        a.div = nullptr;
+        a.dt = a.rho = a.nu = a.gx = a.gy = 0.0;
+        return a;
+    }
+
+    void set_velocity(const double* h_uv) {
+        u_in.resize(n ? n : 1);
+        h2d(u_in.p, reinterpret_cast<const double2*>(h_uv), n);
+        has_velocity = true;
+    }
+
+    // One fractional step on the current points: mesh, assemble, predictor,
+    // pressure right-hand side, solve (solver as in solve()), correction;
+    // move = true then moves the points by dt u (the PFEM Lagrangian step).
+    void projection_step(int bits, int solver, double dt, double rho, double nu, double gx, double gy, double rtol,
+                         uint32_t maxit, bool move) {
+        if (!has_velocity) throw std::invalid_argument("projection step needs a velocity (PfemStep2D_set_velocity)");
+        if (!(dt > 0.0) || !(rho > 0.0) || !(nu >= 0.0)) throw std::invalid_argument("projection step: bad dt, rho or nu");
+        mesh_and_assemble(bits, solver);
+        hipStream_t s = ctx().stream;
+        hipEvent_t e0, e1;
+        SA_HIP(hipEventCreate(&e0));
+        SA_HIP(hipEventCreate(&e1));
+        SA_HIP(hipEventRecord(e0, s));
+        u_m.resize(n);
+        us_m.resize(n);
+        unew_m.resize(n);
+        mass.resize(n);
+        permute_elements(u_in.p, u_m.p, mesh.lbvh.perm, n, sizeof(double2), true, s);
+        FlowArgs a = flow_args();
+        a.dt = dt;
+        a.rho = rho;
+        a.nu = nu;
+        a.gx = gx;
+        a.gy = gy;
+        const dim3 grid(grid_for(n, 256)), blk(256);
+        hipLaunchKernelGGL(flow_predict_kernel, grid, blk, 0, s, a);
+        hipLaunchKernelGGL(flow_div_kernel, grid, blk, 0, s, a);
+        SA_HIP(hipEventRecord(e1, s));
+        SA_HIP(hipEventSynchronize(e1));
+        float t0 = 0;
+        SA_HIP(hipEventElapsedTime(&t0, e0, e1));
+        solve(solver, rtol, maxit);
+        SA_HIP(hipEventRecord(e0, s));
+        hipLaunchKernelGGL(flow_correct_kernel, grid, blk, 0, s, a);
+        permute_elements(unew_m.p, u_in.p, mesh.lbvh.perm, n, sizeof(double2), false, s);
+        if (move) {
+            hipLaunchKernelGGL(move_kernel, grid, blk, 0, s, n, dt, u_in.p, pts.p);
+            if (source.n) update_source(s);
+        }
+        SA_HIP(hipEventRecord(e1, s));
+        SA_HIP(hipEventSynchronize(e1));
+        float t1 = 0;
+        SA_HIP(hipEventElapsedTime(&t1, e0, e1));
+        t_flow = t0 + t1;
+        SA_HIP(hipEventDestroy(e0));
+        SA_HIP(hipEventDestroy(e1));
+        if (move) {
+            A.n = 0;  // the system and solution belong to the points before the move
+            A.nnz = 0;
+            solved = false;
+        }
+    }
+
+    // Diagnostics of the last projection step (Morton order): u*, the new
+    // velocity, the lumped mass and D(u_new).
+    void flow_fields(double* h_us, double* h_unew, double* h_mass, double* h_div) {
+        if (!unew_m.p || unew_m.n != n) throw std::logic_error("no projection step for the current points");
+        hipStream_t s = ctx().stream;
+        if (h_div) {
+            div_m.resize(n);
+            FlowArgs a = flow_args();
+            a.u = unew_m.p;
+            a.div = div_m.p;
+            hipLaunchKernelGGL(flow_div_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, a);
+            d2h(h_div, div_m.p, n);
+        }
+        if (h_us) d2h(reinterpret_cast<double2*>(h_us), us_m.p, n);
+        if (h_unew) d2h(reinterpret_cast<double2*>(h_unew), unew_m.p, n);
+        if (h_mass) d2h(h_mass, mass.p, n);
+    }
+
     void set_points(uint32_t n_, const double* h_xy, const double* h_alpha, const double* h_f) {
+        if (n_ != n) has_velocity = false;
         n = n_;
         A.n = 0;  // the assembled system and solution belong to the previous points
         A.nnz = 0;
@@ -867,6 +1125,57 @@ int32_t PfemStep2D_advect(PfemStep2D* h, uint32_t n_modes, const double* modes, 
         }
         h->s.advect(md, dt, steps);
     });
+}
+
+// Nodal velocity [N][2] (input order) of the projection step.
+int32_t PfemStep2D_set_velocity(PfemStep2D* h, const double* uv) {
+    return g([&] {
+        if (!h || !uv) throw std::invalid_argument("PfemStep2D_set_velocity: bad arguments");
+        h->s.set_velocity(uv);
+    });
+}
+int32_t PfemStep2D_get_velocity(PfemStep2D* h, double* uv) {
+    return g([&] {
+        if (!h || !uv || !h->s.has_velocity) throw std::invalid_argument("PfemStep2D_get_velocity: no velocity set");
+        d2h(reinterpret_cast<double2*>(uv), h->s.u_in.p, h->s.n);
+    });
+}
+
+// One fractional (projection) step of the PFEM velocity on the current
+// points: mesh + assembly, predictor u* = u + dt (g - nu M^-1 K u),
+// pressure K p = -(rho / dt) D(u*) (p = 0 on free-surface nodes) solved by
+// `solver` (as PfemStep2D_run) to rtol, correction u = u* - (dt / rho)
+// M^-1 G(p); move != 0 then moves the points by dt u.  g [2]: body force.
+// stats [11]: PfemStep2D_run's ten, then the ms of the velocity kernels.
+int32_t PfemStep2D_projection_step(PfemStep2D* h, int32_t solver, double dt, double rho, double nu, const double* gv,
+                                   double rtol, uint32_t maxit, int32_t move, double* stats) {
+    return g([&] {
+        if (!h || !gv) throw std::invalid_argument("PfemStep2D_projection_step: bad arguments");
+        if (solver < 0 || solver > 3) throw std::invalid_argument("unknown solver kind");
+        h->s.projection_step(h->bits, solver, dt, rho, nu, gv[0], gv[1], rtol, maxit, move != 0);
+        if (stats) {
+            stats[0] = h->s.t_mesh;
+            stats[1] = h->s.t_asm;
+            stats[2] = h->s.t_solve;
+            stats[3] = h->s.last.iters;
+            stats[4] = h->s.last.true_rel;
+            stats[5] = h->s.last.replacements;
+            stats[6] = h->s.mesh.n_edges() + h->s.n;
+            stats[7] = h->s.mesh.n_elems();
+            stats[8] = h->s.t_prep;
+            stats[9] = h->s.learned_fallback ? 1.0 : 0.0;
+            stats[10] = h->s.t_flow;
+        }
+        check_error_flag();
+        if (h->s.last.true_rel > rtol) throw std::runtime_error("CG did not reach the requested residual");
+    });
+}
+
+// Fields of the last projection step in Morton order (any pointer may be
+// null): u* [N][2], the new velocity [N][2], the lumped mass [N] and the
+// discrete divergence D(u_new) [N].
+int32_t PfemStep2D_get_flow_fields(PfemStep2D* h, double* us, double* unew, double* mass, double* div) {
+    return g([&] { h->s.flow_fields(us, unew, mass, div); });
 }
 
 // Current point coordinates [N][2] (input order).

@@ -9,7 +9,8 @@
 //     expanded into monomials, each monomial is formed exactly as a short
 //     sum of doubles through error-free fma products, and all components are
 //     accumulated into a nonoverlapping expansion with a zero-eliminating
-//     grow step.  The expansion lives in a caller-provided per-lane slice
+//     grow step, renormalised whenever it reaches kCompressAt components.
+//     The expansion lives in a caller-provided per-lane slice
 //     of global memory (kExactWsDoubles doubles), so the hot kernels keep a
 //     small private footprint; only the deferred exact kernel passes one.
 //
@@ -34,8 +35,9 @@ template <class S> struct PredBounds {
     static constexpr S iccA = (S)((10.0 + 96.0 * PredEps<S>::eps) * PredEps<S>::eps);
 };
 
-// Doubles per lane needed by the exact 2D stage (incircle: 48 monomials of
-// degree 4, at most 8 components each, plus the growing sum).
+// Expansion workspace of the exact 2D stage in doubles (incircle: 48
+// monomials of degree 4, at most 8 components each; renormalised at
+// kCompressAt components, so far fewer are ever held).
 constexpr int kExactWsDoubles = 400;
 
 // orient2d filter: returns det; *certain = false when the sign is not
@@ -83,14 +85,57 @@ __device__ __forceinline__ S incircle_filter(const S* pa, const S* pb, const S* 
 // ---------------------------------------------------------------------------
 // Exact stage.
 // ---------------------------------------------------------------------------
+// Expansion length at which the accumulator renormalises itself.  Growing
+// one component at a time keeps the expansion nonoverlapping but not short:
+// without renormalisation its length tracks the number of components added
+// (hundreds for an insphere determinant), and every further add walks all of
+// them.  Renormalised, it holds about as many components as its bit span
+// needs (a handful for FP32 inputs).
+constexpr int kCompressAt = 24;
+
 struct ExactAcc {
     double* h;  // per-lane workspace
     int n;
     int cap = 1 << 30;  // workspace length (components)
     bool over = false;  // a component would not fit: the sign is unreliable
 
-    __device__ __forceinline__ void add(double b) {
+    // Shewchuk's renormalisation (COMPRESS, "Adaptive Precision Floating-
+    // Point Arithmetic and Fast Robust Geometric Predicates", 1997), in
+    // place: a downward pass merges each component into a running sum and
+    // keeps the roundoff terms at the top of the array, an upward pass
+    // merges them back from the top, keeping the nonzero roundoffs at the
+    // bottom; the result is a nonoverlapping expansion of the same value,
+    // increasing magnitude, with no two adjacent components overlapping.
+    __host__ __device__ __forceinline__ void compress() {
+        if (n < 2) return;
+        int bottom = n - 1;
+        double Q = h[bottom];
+        for (int i = n - 2; i >= 0; --i) {
+            const double e = h[i];
+            const double x = Q + e;  // |Q| >= |e|: fast two-sum
+            const double y = e - (x - Q);
+            if (y != 0.0) {
+                h[bottom--] = x;
+                Q = y;
+            } else {
+                Q = x;
+            }
+        }
+        int top = 0;
+        for (int i = bottom + 1; i < n; ++i) {
+            const double hv = h[i];
+            const double x = hv + Q;  // |hv| >= |Q|: fast two-sum
+            const double y = Q - (x - hv);
+            if (y != 0.0) h[top++] = y;
+            Q = x;
+        }
+        h[top] = Q;
+        n = top + 1;
+    }
+
+    __host__ __device__ __forceinline__ void add(double b) {
         if (b == 0.0) return;
+        if (n >= kCompressAt) compress();
         if (n + 1 > cap) {
             over = true;
             return;
@@ -110,7 +155,7 @@ struct ExactAcc {
         n = k;
     }
     // sign * f0 * f1 * f2 (exact, up to four components)
-    __device__ __forceinline__ void add_prod3(double f0, double f1, double f2, double sign) {
+    __host__ __device__ __forceinline__ void add_prod3(double f0, double f1, double f2, double sign) {
         const double a = sign * f0;
         const double p0 = a * f1;
         const double p1 = fma(a, f1, -p0);
@@ -122,7 +167,7 @@ struct ExactAcc {
         add(q3);
     }
     // sign * f0 * f1 * f2 * f3 * f4 (exact, up to sixteen components)
-    __device__ __forceinline__ void add_prod5(double f0, double f1, double f2, double f3, double f4,
+    __host__ __device__ __forceinline__ void add_prod5(double f0, double f1, double f2, double f3, double f4,
                                               double sign) {
         double p[16], q[16];
         const double a = sign * f0;
@@ -147,7 +192,7 @@ struct ExactAcc {
         for (int k = 0; k < 16; ++k) add(q[k]);
     }
     // sign * f0 * f1 (exact, two components)
-    __device__ __forceinline__ void add_prod2(double f0, double f1, double sign) {
+    __host__ __device__ __forceinline__ void add_prod2(double f0, double f1, double sign) {
         const double a = sign * f0;
         const double x = a * f1;
         const double y = fma(a, f1, -x);
@@ -155,7 +200,7 @@ struct ExactAcc {
         add(x);
     }
     // sign * f0 * f1 * f2 * f3 (exact, up to eight components)
-    __device__ __forceinline__ void add_prod4(double f0, double f1, double f2, double f3, double sign) {
+    __host__ __device__ __forceinline__ void add_prod4(double f0, double f1, double f2, double f3, double sign) {
         double p[8];
         const double a = sign * f0;
         p[0] = a * f1;
@@ -174,7 +219,7 @@ struct ExactAcc {
 #pragma unroll
         for (int k = 0; k < 8; ++k) add(p[k]);
     }
-    __device__ __forceinline__ int sign() const {
+    __host__ __device__ __forceinline__ int sign() const {
         for (int i = n - 1; i >= 0; --i)
             if (h[i] != 0.0) return h[i] > 0.0 ? 1 : -1;
         return 0;
@@ -182,8 +227,10 @@ struct ExactAcc {
 };
 
 // Exact sign of det | ax ay 1 ; bx by 1 ; cx cy 1 |.
-template <class S> __device__ __noinline__ int orient2d_exact(const S* a, const S* b, const S* c, double* ws) {
-    ExactAcc acc{ws, 0};
+template <class S>
+__host__ __device__ __noinline__ int orient2d_exact(const S* a, const S* b, const S* c, double* ws, int cap = 1 << 30,
+                                                    bool* over = nullptr) {
+    ExactAcc acc{ws, 0, cap};
     const double ax = a[0], ay = a[1], bx = b[0], by = b[1], cx = c[0], cy = c[1];
     acc.add_prod2(ax, by, 1.0);
     acc.add_prod2(bx, cy, 1.0);
@@ -191,6 +238,7 @@ template <class S> __device__ __noinline__ int orient2d_exact(const S* a, const 
     acc.add_prod2(ax, cy, -1.0);
     acc.add_prod2(bx, ay, -1.0);
     acc.add_prod2(cx, by, -1.0);
+    if (over) *over |= acc.over;
     return acc.sign();
 }
 
@@ -198,12 +246,13 @@ template <class S> __device__ __noinline__ int orient2d_exact(const S* a, const 
 // the lift column: sum_k (-1)^k lift_k M_k, M_k the (x, y, 1) minor of the
 // three other rows.
 template <class S>
-__device__ __noinline__ int incircle_exact(const S* a, const S* b, const S* c, const S* d, double* ws) {
+__host__ __device__ __noinline__ int incircle_exact(const S* a, const S* b, const S* c, const S* d, double* ws,
+                                                    int cap = 1 << 30, bool* over = nullptr) {
     const double P[4][2] = {{(double)a[0], (double)a[1]},
                             {(double)b[0], (double)b[1]},
                             {(double)c[0], (double)c[1]},
                             {(double)d[0], (double)d[1]}};
-    ExactAcc acc{ws, 0};
+    ExactAcc acc{ws, 0, cap};
     for (int k = 0; k < 4; ++k) {
         int r0, r1, r2;
         if (k == 0) { r0 = 1; r1 = 2; r2 = 3; }
@@ -218,6 +267,7 @@ __device__ __noinline__ int incircle_exact(const S* a, const S* b, const S* c, c
             for (int comp = 0; comp < 2; ++comp)
                 acc.add_prod4(P[k][comp], P[k][comp], m[t][0], m[t][1], cof * m[t][2]);
     }
+    if (over) *over |= acc.over;
     return acc.sign();
 }
 
@@ -293,7 +343,7 @@ constexpr int kPerm3[6][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {0, 2, 1}, {1, 0,
 // Laplace on the ones column, 24 monomials of degree 3.  *over: workspace
 // exhausted.
 template <class S>
-__device__ __noinline__ int orient3d_exact(const S* a, const S* b, const S* c, const S* d, double* ws, int cap,
+__host__ __device__ __noinline__ int orient3d_exact(const S* a, const S* b, const S* c, const S* d, double* ws, int cap,
                                            bool* over) {
     const double P[4][3] = {{(double)a[0], (double)a[1], (double)a[2]}, {(double)b[0], (double)b[1], (double)b[2]},
                             {(double)c[0], (double)c[1], (double)c[2]}, {(double)d[0], (double)d[1], (double)d[2]}};
@@ -315,7 +365,7 @@ __device__ __noinline__ int orient3d_exact(const S* a, const S* b, const S* c, c
 // insphere): Laplace on the ones column, then on the lift column, the lift
 // split into its three squares; 360 monomials of degree 5.
 template <class S>
-__device__ __noinline__ int insphere_exact(const S* a, const S* b, const S* c, const S* d, const S* e, double* ws,
+__host__ __device__ __noinline__ int insphere_exact(const S* a, const S* b, const S* c, const S* d, const S* e, double* ws,
                                            int cap, bool* over) {
     const double P[5][3] = {{(double)a[0], (double)a[1], (double)a[2]}, {(double)b[0], (double)b[1], (double)b[2]},
                             {(double)c[0], (double)c[1], (double)c[2]}, {(double)d[0], (double)d[1], (double)d[2]},

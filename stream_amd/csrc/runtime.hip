@@ -119,6 +119,18 @@ int32_t Stream_device_bytes(uint64_t* current, uint64_t* peak, int32_t reset_pea
     });
 }
 
+// Points as testcases/IMR26/uniform2D.cpp:130-141 and uniform3D.cpp draw
+// them: srand(seed), then (float)rand() / RAND_MAX per coordinate, point by
+// point; out [n][dim].  Host only (the C library's generator).
+int32_t Stream_uniform_cloud_f32(uint32_t n, uint32_t dim, uint32_t seed, float* out) {
+    return stream_amd::guarded([&] {
+        if (!out || dim < 1 || dim > 3) throw std::invalid_argument("Stream_uniform_cloud_f32: bad arguments");
+        srand(seed);
+        const uint64_t total = (uint64_t)n * dim;
+        for (uint64_t k = 0; k < total; ++k) out[k] = (float)rand() / RAND_MAX;
+    });
+}
+
 // Block until all queued device work is done.
 int32_t Stream_synchronize(void) {
     return stream_amd::guarded([] { SA_HIP(hipStreamSynchronize(stream_amd::ctx().stream)); });

@@ -138,6 +138,8 @@ enum SectionMark : int {
     kMarkCompress = 10, // compress done
     kMarkAsm0 = 11,     // P1 assembly starts
     kMarkAsm = 12,      // P1 assembly kernel done
+    kMarkCompress0 = 13, // compress starts (after any host read of the counts)
+    kMarkDelaunayMain = 14,  // local Delaunay of every node done (exact redo next)
 };
 void sections_arm(bool on);
 void section_mark(hipStream_t s, int id);

@@ -47,6 +47,7 @@ AlphaShape3D_get_permutation = bind(libmesh, "AlphaShape3D_get_permutation", i32
 AlphaShape3D_get_nexact = bind(libmesh, "AlphaShape3D_get_nexact", u32, [AlphaShape3D_ptr])
 AlphaShape3D_get_nclamped = bind(libmesh, "AlphaShape3D_get_nclamped", u32, [AlphaShape3D_ptr])
 AlphaShape3D_last_status = bind(libmesh, "AlphaShape3D_last_status", i32, [AlphaShape3D_ptr])
+AlphaShape3D_timed_run = bind(libmesh, "AlphaShape3D_timed_run", i32, [AlphaShape3D_ptr, arr(np.float64, 1, writeable=True)])
 AlphaShape3D_permute_forward = bind(libmesh, "AlphaShape3D_permute_forward", i32, [AlphaShape3D_ptr, vp, vp, u32])
 AlphaShape3D_permute_backward = bind(libmesh, "AlphaShape3D_permute_backward", i32, [AlphaShape3D_ptr, vp, vp, u32])
 
@@ -120,6 +121,19 @@ class AlphaShape3D:
         nb = np.empty(self.nnodes, np.uint8)
         check(AlphaShape3D_get_edges(self.mesh, None, None, None, nb))
         return nb
+
+    def timed_run(self) -> dict:
+        """init + compute, timed as the reference's IMR26 benchmark loop:
+        GPU ms of the LBVH build (tlbvh), the candidate search, local
+        Delaunay, filter and scans (tashape) and the compaction (tcompress),
+        the host wall ms of both calls and the GPU ms from the build's start
+        to the compaction's end; within tashape the candidate search (fill),
+        the local Delaunay of every node and the exact redo of the nodes
+        whose filters could not decide a sign."""
+        ms = np.zeros(8)
+        check(AlphaShape3D_timed_run(self.mesh, ms))
+        return dict(tlbvh=ms[0], tashape=ms[1], tcompress=ms[2], wall=ms[3], gpu_total=ms[4], fill=ms[5],
+                    delaunay=ms[6], exact_redo=ms[7])
 
     def permute_forward(self, field: npt.NDArray) -> npt.NDArray:
         """Nodal field (shape (n, ...)) from input order to the Morton order

@@ -38,6 +38,13 @@ PfemStep2D_permute_forward = bind(lib, "PfemStep2D_permute_forward", i32, [_p, _
 PfemStep2D_permute_backward = bind(lib, "PfemStep2D_permute_backward", i32, [_p, _p, _p, u32])
 PfemStep3D_permute_forward = bind(lib, "PfemStep3D_permute_forward", i32, [_p, _p, _p, u32])
 PfemStep3D_permute_backward = bind(lib, "PfemStep3D_permute_backward", i32, [_p, _p, _p, u32])
+PfemStep2D_set_velocity = bind(lib, "PfemStep2D_set_velocity", i32, [_p, arr(np.float64, 2)])
+PfemStep2D_get_velocity = bind(lib, "PfemStep2D_get_velocity", i32, [_p, arr(np.float64, 2, True)])
+PfemStep2D_projection_step = bind(lib, "PfemStep2D_projection_step", i32,
+                                  [_p, i32, f64, f64, f64, arr(np.float64, 1), f64, u32, i32,
+                                   arr(np.float64, 1, True)])
+PfemStep2D_get_flow_fields = bind(lib, "PfemStep2D_get_flow_fields", i32,
+                                  [_p, opt(np.float64), opt(np.float64), opt(np.float64), opt(np.float64)])
 PfemStep2D_time_sections = bind(lib, "PfemStep2D_time_sections", i32, [_p, u32, arr(np.float64, 1, True)])
 
 # stage names of PfemStep2D.time_sections, in launch order
@@ -127,6 +134,41 @@ class PfemStep2D:
         ms = ctypes.c_double(0.0)
         check(PfemStep2D_time_dir_kernel(self.h, 3, reps, ctypes.byref(ms)))
         return ms.value
+
+    def set_velocity(self, uv):
+        """Nodal velocity (n, 2) in input order for projection_step."""
+        uv = np.ascontiguousarray(uv, np.float64)
+        if uv.shape != (self.n, 2):
+            raise ValueError(f"velocity must have shape ({self.n}, 2), got {uv.shape}")
+        check(PfemStep2D_set_velocity(self.h, uv))
+
+    def velocity(self):
+        uv = np.empty((self.n, 2))
+        check(PfemStep2D_get_velocity(self.h, uv))
+        return uv
+
+    def projection_step(self, solver="fp64", dt=1e-3, rho=1.0, nu=0.0, g=(0.0, 0.0), rtol=1e-8, maxit=200000,
+                        move=True):
+        """One fractional step of the PFEM velocity (predictor, pressure
+        solve with p = 0 on the free surface, correction; then, with move,
+        the points move by dt u).  Returns run()'s fields plus flow_ms."""
+        st = np.zeros(11)
+        kind = SOLVER_NAMES[solver] if isinstance(solver, str) else int(solver)
+        check(PfemStep2D_projection_step(self.h, kind, dt, rho, nu, np.asarray(g, np.float64), rtol, maxit,
+                                         int(bool(move)), st))
+        return dict(mesh_ms=st[0], assemble_ms=st[1], solve_ms=st[2], iters=int(st[3]), true_rel=st[4],
+                    replacements=int(st[5]), nnz=int(st[6]), n_elems=int(st[7]), prep_ms=st[8],
+                    fallback=bool(st[9]), flow_ms=st[10])
+
+    def flow_fields(self):
+        """Last projection step, Morton order: (u* (n, 2), new velocity (n, 2),
+        lumped mass (n,), D(new velocity) (n,))."""
+        us = np.empty((self.n, 2))
+        un = np.empty((self.n, 2))
+        m = np.empty(self.n)
+        d = np.empty(self.n)
+        check(PfemStep2D_get_flow_fields(self.h, us, un, m, d))
+        return us, un, m, d
 
     def time_sections(self, reps: int = 5) -> dict:
         """GPU ms of the mesh-and-assembly stages on the current points
