@@ -219,6 +219,12 @@ __device__ void store_partials(const double (&v)[K], double* __restrict__ p, uin
         }
 }
 
+// The solver's stop flag as a plain vector load (relaxed, agent scope), so
+// that a kernel can issue its first data loads before waiting for it.
+__device__ __forceinline__ uint32_t flag_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // The K x G partials a previous kernel stored with store_partials, read as
 // one index space e = k G + b: thread t adds entries t, t + NT, ...
 // of every k in ascending order, then the lanes' sums meet in an xor
@@ -693,7 +699,9 @@ template <class V, bool DIFF, int CH, int R, int NT>
 __global__ __launch_bounds__(NT) void llt_dir_kernel(SellArgs<V> M, const V* __restrict__ d,
                                                              V* __restrict__ q, uint32_t qs, const CgScalars* sc,
                                                              double* partials) {
-    if (sc->done) return;
+    // the stop flag is read with the first slice's loads in flight and
+    // tested before anything is stored (every block sees the same flag)
+    const uint32_t done = flag_load(&sc->done);
     const uint32_t lane = threadIdx.x & 63u;
     double v[1] = {0.0};
     SELL_FOR_SLICE_ROWS(R, rows) {
@@ -703,6 +711,7 @@ __global__ __launch_bounds__(NT) void llt_dir_kernel(SellArgs<V> M, const V* __r
             dn[r] = d[rows.ii[r]];
             acc[r] = M.diag[rows.ii[r]] * dn[r];
         }
+        if (done) return;
         sell_rows_product<V, R, CH, V>(
             M, rows, acc, [&](uint32_t c) { return d[c]; },
             [&](int r, V ac, V a, V g) {
@@ -718,6 +727,7 @@ __global__ __launch_bounds__(NT) void llt_dir_kernel(SellArgs<V> M, const V* __r
                 v[0] += (double)dn[r] * (double)acc[r];
             }
     }
+    if (done) return;
     store_partials<1, NT>(v, partials, kMaxCgBlocks);
 }
 
@@ -785,7 +795,9 @@ __global__ __launch_bounds__(NT) void llt_update_kernel(SellArgs<V> U, XV* __res
                                                                 const V2<V>* __restrict__ rq,
                                                                 V2<V>* __restrict__ rq_nxt, V* __restrict__ w,
                                                                 CgScalars* sc, double* partials, int mode) {
-    if (mode == 0 && sc->done) return;
+    // the stop flag is read with the first loads in flight and tested
+    // before the partials' sum (a barrier) and before anything is stored
+    const uint32_t done = mode == 0 ? flag_load(&sc->done) : 0u;
     const SellArgs<V>& M = U;
     const bool it = mode == 0;
     // the direction kernel's partials and the first slice group's own data
@@ -803,6 +815,7 @@ __global__ __launch_bounds__(NT) void llt_update_kernel(SellArgs<V> U, XV* __res
         diag0[r] = U.diag[first.ii[r]];
     }
     double alpha = 0.0;
+    if (done) return;
     if (it) {
         double dq[1];
         sum_prefetched<1, NT>(pf, partials, kMaxCgBlocks, gridDim.x, dq);
@@ -869,7 +882,9 @@ template <class V, int CH, int R, int NT>
 __global__ __launch_bounds__(NT) void llt_lo_kernel(SellArgs<V> M, const V* __restrict__ w,
                                                                const V* __restrict__ d_old, V* __restrict__ d_new,
                                                                CgScalars* sc, const double* partials, int mode) {
-    if (mode == 0 && sc->done) return;
+    // the stop flag is read with the first loads in flight and tested
+    // before the partials' sum (a barrier) and before anything is stored
+    const uint32_t done = mode == 0 ? flag_load(&sc->done) : 0u;
     // the partials' loads go out first; their sum (beta) is taken after the
     // wavefront's first slice group, whose loads cover their latency
     const PartialsPrefetch pf = prefetch_partials<2, NT>(partials + kMaxCgBlocks, kMaxCgBlocks, gridDim.x);
@@ -893,6 +908,7 @@ __global__ __launch_bounds__(NT) void llt_lo_kernel(SellArgs<V> M, const V* __re
     const SliceRows<R> first = slice_rows<V, R>(M, sp.s, sp, lane);
     V acc0[R];
     product(first, acc0);
+    if (done) return;
     double tot[2];
     sum_prefetched<2, NT>(pf, partials + kMaxCgBlocks, kMaxCgBlocks, gridDim.x, tot);
     double bd = 0.0;
