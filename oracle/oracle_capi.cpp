@@ -586,4 +586,14 @@ void orc_step_solve_f64(uint32_t n, const double* xyr, const double* f, double r
     info[3] = rel2;
 }
 
+
+// The oracle's squared norm and 2D cross product of the alpha filter
+// (fmac::sumsq / fmac::diff in their plain mode), for comparison with the
+// reference's Vec2f::sqnorm / cross (vec.hpp:216-238).
+float orc_sumsq2_f32(float x, float y) {
+    const float v[2] = {x, y};
+    return orc::fmac::sumsq(0, false, v, 2);
+}
+float orc_cross2_f32(float ux, float uy, float vx, float vy) { return orc::fmac::diff(0, false, ux, vy, uy, vx); }
+
 }  // extern "C"

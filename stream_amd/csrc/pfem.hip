@@ -56,25 +56,57 @@ struct P1Args {
     uint32_t* flags;
 };
 
-__global__ __launch_bounds__(256) void p1_assemble_kernel(P1Args a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+// Rows of up to kP1Regs entries (the diagonal included) are built in
+// registers: the columns kept sorted by a static-index insertion, each
+// triangle's two contributions added by a predicated pass over the slots
+// (no dynamic indexing, so nothing spills), and the row stored once.
+// Longer rows take p1_row_global, which builds them in place in the
+// output arrays.  Both give the same operations in the same order.
+constexpr int kP1Regs = 16;
+
+// Triangle t of node i: its two other vertices, the cotangent weights of
+// the edges (i, ga) and (i, gb), and its area.
+struct P1TriW {
+    uint32_t ga, gb;
+    double ka, kb, area;
+};
+__device__ __forceinline__ P1TriW p1_triangle(const P1Args& a, uint32_t i, uint32_t t) {
     const uint32_t n = a.n;
-    const uint32_t e0 = a.erow[i];
-    const uint32_t nf = a.erow[i + 1] - e0;
-    const uint32_t base = e0 + i;
-    a.arow[i] = base;
-    if (i == n - 1) a.arow[n] = a.erow[n] + n;
-    if (e0 + nf > a.cap_edges || base + nf + 1 > a.cap) {
-        // only reachable after an edge-capacity overflow of the mesh: the
-        // row's edges (or its matrix entries) would lie past the end of
-        // their arrays
-        raise_flag(a.flags, kErrCapacity);
-        return;
+    auto gneig = [&](uint32_t j) -> uint32_t { return j < kNInf2D ? n + j : a.cand[slab_index(i, j - kNInf2D)]; };
+    const uint16_t e = a.tri[slab_index(i, t)];
+    P1TriW w;
+    w.ga = gneig(e & 255u);
+    w.gb = gneig(e >> 8);
+    uint32_t v[3] = {i, w.ga, w.gb};
+    if (v[1] < v[0]) { const uint32_t x = v[0]; v[0] = v[1]; v[1] = x; }
+    if (v[2] < v[1]) { const uint32_t x = v[1]; v[1] = v[2]; v[2] = x; }
+    if (v[1] < v[0]) { const uint32_t x = v[0]; v[0] = v[1]; v[1] = x; }
+    double p[3][2];
+    for (int k = 0; k < 3; ++k) {
+        const Sph4<double> s = a.P[v[k]];
+        p[k][0] = s.c[0];
+        p[k][1] = s.c[1];
     }
+    const double c0 = cot_at(p[0], p[1], p[2]);
+    const double c1 = cot_at(p[1], p[0], p[2]);
+    const double c2 = cot_at(p[2], p[0], p[1]);
+    const double ux = p[1][0] - p[0][0], uy = p[1][1] - p[0][1];
+    const double wx = p[2][0] - p[0][0], wy = p[2][1] - p[0][1];
+    w.area = 0.5 * fabs(ux * wy - uy * wx);
+    // k(edge) = -cot(opposite)/2: (v1,v2) <- c0, (v0,v2) <- c1, (v0,v1) <- c2;
+    // the edge (i, ga) is opposite gb and (i, gb) opposite ga
+    const double kk[3] = {-0.5 * c0, -0.5 * c1, -0.5 * c2};
+    const int oa = (w.gb == v[0]) ? 0 : ((w.gb == v[1]) ? 1 : 2);
+    const int ob = (w.ga == v[0]) ? 0 : ((w.ga == v[1]) ? 1 : 2);
+    w.ka = kk[oa];
+    w.kb = kk[ob];
+    return w;
+}
+
+// Row i built in place in the output arrays (rows longer than kP1Regs).
+__device__ void p1_row_global(const P1Args& a, uint32_t i, uint32_t e0, uint32_t nf, uint32_t base, double& mass) {
     uint32_t* col = a.acol + base;
     double* val = a.aval + base;
-    // sorted column list (insertion sort; rows hold ~7 entries)
     col[0] = i;
     uint32_t cnt = 1;
     for (uint32_t k = 0; k < nf; ++k) {
@@ -99,39 +131,12 @@ __global__ __launch_bounds__(256) void p1_assemble_kernel(P1Args a) {
         }
         return lo;
     };
-    auto gneig = [&](uint32_t j) -> uint32_t { return j < kNInf2D ? n + j : a.cand[slab_index(i, j - kNInf2D)]; };
-    double mass = 0.0;
     const uint32_t nt = a.nelem[i];
     for (uint32_t t = 0; t < nt; ++t) {
-        const uint16_t e = a.tri[slab_index(i, t)];
-        const uint32_t ga = gneig(e & 255u), gb = gneig(e >> 8);
-        uint32_t v[3] = {i, ga, gb};
-        if (v[1] < v[0]) { const uint32_t x = v[0]; v[0] = v[1]; v[1] = x; }
-        if (v[2] < v[1]) { const uint32_t x = v[1]; v[1] = v[2]; v[2] = x; }
-        if (v[1] < v[0]) { const uint32_t x = v[0]; v[0] = v[1]; v[1] = x; }
-        double p[3][2];
-        for (int k = 0; k < 3; ++k) {
-            const Sph4<double> s = a.P[v[k]];
-            p[k][0] = s.c[0];
-            p[k][1] = s.c[1];
-        }
-        const double c0 = cot_at(p[0], p[1], p[2]);
-        const double c1 = cot_at(p[1], p[0], p[2]);
-        const double c2 = cot_at(p[2], p[0], p[1]);
-        const double ux = p[1][0] - p[0][0], uy = p[1][1] - p[0][1];
-        const double wx = p[2][0] - p[0][0], wy = p[2][1] - p[0][1];
-        const double area = 0.5 * fabs(ux * wy - uy * wx);
-        // k(edge) = -cot(opposite)/2: (v1,v2) <- c0, (v0,v2) <- c1, (v0,v1) <- c2
-        const double kk[3] = {-0.5 * c0, -0.5 * c1, -0.5 * c2};
-        // the two neighbours of i in this triangle and the vertex opposite
-        // each edge (i, nb)
-        for (int s2 = 0; s2 < 2; ++s2) {
-            const uint32_t nb = s2 == 0 ? ga : gb;
-            const uint32_t opp = s2 == 0 ? gb : ga;
-            const int ko = (opp == v[0]) ? 0 : ((opp == v[1]) ? 1 : 2);
-            val[find(nb)] += kk[ko];
-        }
-        mass += area / 3.0;
+        const P1TriW w = p1_triangle(a, i, t);
+        val[find(w.ga)] += w.ka;
+        val[find(w.gb)] += w.kb;
+        mass += w.area / 3.0;
     }
     const bool bi = a.bnd[i] != 0;
     double diag = 0.0;
@@ -143,18 +148,79 @@ __global__ __launch_bounds__(256) void p1_assemble_kernel(P1Args a) {
         }
         diag -= val[k];
     }
-    for (uint32_t k = 0; k < cnt; ++k) {
-        double v;
-        if (k == dpos)
-            v = bi ? 1.0 : diag;
-        else
-            v = (bi || a.bnd[col[k]]) ? 0.0 : val[k];
-        val[k] = v;
-    }
+    for (uint32_t k = 0; k < cnt; ++k)
+        val[k] = (k == dpos) ? (bi ? 1.0 : diag) : ((bi || a.bnd[col[k]]) ? 0.0 : val[k]);
     const double dg = bi ? 1.0 : diag;
-    const double id = 1.0 / dg;
-    a.idiag[i] = id;
-    a.idiag32[i] = (float)id;
+    a.idiag[i] = 1.0 / dg;
+    a.idiag32[i] = (float)(1.0 / dg);
+}
+
+__global__ __launch_bounds__(256) void p1_assemble_kernel(P1Args a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t n = a.n;
+    const uint32_t e0 = a.erow[i];
+    const uint32_t nf = a.erow[i + 1] - e0;
+    const uint32_t base = e0 + i;
+    a.arow[i] = base;
+    if (i == n - 1) a.arow[n] = a.erow[n] + n;
+    if (e0 + nf > a.cap_edges || base + nf + 1 > a.cap) {
+        // only reachable after an edge-capacity overflow of the mesh: the
+        // row's edges (or its matrix entries) would lie past the end of
+        // their arrays
+        raise_flag(a.flags, kErrCapacity);
+        return;
+    }
+    double mass = 0.0;
+    const bool bi = a.bnd[i] != 0;
+    if (nf + 1 > (uint32_t)kP1Regs) {
+        p1_row_global(a, i, e0, nf, base, mass);
+    } else {
+        // sorted columns: insert each neighbour into the ascending list
+        // (unused slots hold 0xffffffff, above every id)
+        uint32_t c[kP1Regs];
+        double v[kP1Regs];
+#pragma unroll
+        for (int k = 0; k < kP1Regs; ++k) {
+            c[k] = 0xffffffffu;
+            v[k] = 0.0;
+        }
+        c[0] = i;
+        for (uint32_t q = 0; q < nf; ++q) {
+            const uint32_t x = a.eneig[e0 + q];
+#pragma unroll
+            for (int k = kP1Regs - 1; k >= 1; --k) c[k] = (c[k] <= x) ? c[k] : (c[k - 1] <= x ? x : c[k - 1]);
+            c[0] = (c[0] <= x) ? c[0] : x;
+        }
+        const uint32_t nt = a.nelem[i];
+        for (uint32_t t = 0; t < nt; ++t) {
+            const P1TriW w = p1_triangle(a, i, t);
+#pragma unroll
+            for (int k = 0; k < kP1Regs; ++k) {
+                if (c[k] == w.ga) v[k] += w.ka;
+                if (c[k] == w.gb) v[k] += w.kb;
+            }
+            mass += w.area / 3.0;
+        }
+        double diag = 0.0;
+#pragma unroll
+        for (int k = 0; k < kP1Regs; ++k)
+            if (c[k] != i && c[k] != 0xffffffffu) diag -= v[k];
+        const uint32_t cnt = nf + 1;
+        uint32_t* col = a.acol + base;
+        double* val = a.aval + base;
+#pragma unroll
+        for (int k = 0; k < kP1Regs; ++k) {
+            if ((uint32_t)k < cnt) {
+                const uint32_t ck = c[k];
+                col[k] = ck;
+                val[k] = (ck == i) ? (bi ? 1.0 : diag) : ((bi || a.bnd[ck]) ? 0.0 : v[k]);
+            }
+        }
+        const double dg = bi ? 1.0 : diag;
+        a.idiag[i] = 1.0 / dg;
+        a.idiag32[i] = (float)(1.0 / dg);
+    }
     a.b[i] = bi ? 0.0 : a.f[i] * mass;
 }
 

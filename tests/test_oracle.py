@@ -487,3 +487,38 @@ def test_lbvha_collisions_against_brute_force():
     rs3 = 0.04
     brute3 = int(np.sum(d2 <= (2 * rs3) ** 2) - n)
     assert ol.lbvha_collisions(c3, rs3, 4, bits=21) == brute3
+
+
+def test_vec2_sqnorm_and_cross_against_reference():
+    """The alpha filter's squared edge lengths and signed area
+    (oracle.hpp: fmac::sumsq and fmac::diff in plain mode) in the operation
+    order of the reference's Vec2f::sqnorm and cross (vec.hpp:216-238,
+    compiled in oracle/_ref): bit for bit on float inputs across a wide
+    range of magnitudes, where any other association or a fused product
+    would round differently."""
+    R = ol.ref()
+    if R is None:
+        pytest.skip("reference headers not built (oracle/_ref)")
+    L = ol.lib()
+    f = ctypes.c_float
+    R.ref_vec2_sqnorm.restype = f
+    R.ref_vec2_sqnorm.argtypes = [f, f]
+    R.ref_vec2_cross.restype = f
+    R.ref_vec2_cross.argtypes = [f, f, f, f]
+    L.orc_sumsq2_f32.restype = f
+    L.orc_sumsq2_f32.argtypes = [f, f]
+    L.orc_cross2_f32.restype = f
+    L.orc_cross2_f32.argtypes = [f, f, f, f]
+    rng = np.random.default_rng(17)
+    v = (rng.standard_normal((3000, 4)) * np.exp2(rng.integers(-20, 20, (3000, 4)))).astype(np.float32)
+    differs_from_fma = 0
+    for ax, ay, bx, by in v:
+        s_ref = np.float32(R.ref_vec2_sqnorm(ax, ay))
+        assert np.float32(L.orc_sumsq2_f32(ax, ay)).tobytes() == s_ref.tobytes()
+        c_ref = np.float32(R.ref_vec2_cross(ax, ay, bx, by))
+        assert np.float32(L.orc_cross2_f32(ax, ay, bx, by)).tobytes() == c_ref.tobytes()
+        # the same order in numpy float32 (each product rounded, then the sum)
+        assert (np.float32(ax * ax) + np.float32(ay * ay)).tobytes() == s_ref.tobytes()
+        fused = np.float32(float(ax) * float(by) - np.float64(np.float32(ay * bx)))
+        differs_from_fma += fused.tobytes() != c_ref.tobytes()
+    assert differs_from_fma > 0  # the check can tell a fused evaluation apart
