@@ -126,13 +126,16 @@ __global__ __launch_bounds__(256) void prep_rows_kernel(uint32_t n, const uint32
     if (threadIdx.x < kPrepVals) part[(size_t)blockIdx.x * kPrepVals + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// One wave: lane l folds blocks l, l + 64, ... in order, then an xor
-// butterfly combines the lanes (min/max exact, sums in a fixed order).
-__global__ __launch_bounds__(64) void prep_fold_kernel(uint32_t nblk, const double* __restrict__ part,
-                                                       double* __restrict__ scal) {
+// One block of kFoldThreads: thread t folds blocks t, t + kFoldThreads, ...
+// in order, an xor butterfly combines each wave's lanes, and thread 0
+// combines the waves in wave order (min/max exact, sums in a fixed order).
+constexpr uint32_t kFoldThreads = 1024;
+__global__ __launch_bounds__(kFoldThreads) void prep_fold_kernel(uint32_t nblk, const double* __restrict__ part,
+                                                                 double* __restrict__ scal) {
+    __shared__ double s_w[kFoldThreads / 64][kPrepVals];
     const uint32_t l = threadIdx.x;
     double b[kPrepVals] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY, 0.0, 0.0};
-    for (uint32_t k = l; k < nblk; k += 64) {
+    for (uint32_t k = l; k < nblk; k += kFoldThreads) {
         const double* p = part + (size_t)k * kPrepVals;
         for (int c = 0; c < 6; c += 2) {
             b[c] = fmin(b[c], p[c]);
@@ -150,8 +153,20 @@ __global__ __launch_bounds__(64) void prep_fold_kernel(uint32_t nblk, const doub
         b[6] += __shfl_xor(b[6], o, 64);
         b[7] += __shfl_xor(b[7], o, 64);
     }
-    if (l == 0)
+    if ((l & 63) == 0)
+        for (int k = 0; k < kPrepVals; ++k) s_w[l / 64][k] = b[k];
+    __syncthreads();
+    if (l == 0) {
+        for (uint32_t w = 1; w < kFoldThreads / 64; ++w) {
+            for (int c = 0; c < 6; c += 2) {
+                b[c] = fmin(b[c], s_w[w][c]);
+                b[c + 1] = fmax(b[c + 1], s_w[w][c + 1]);
+            }
+            b[6] += s_w[w][6];
+            b[7] += s_w[w][7];
+        }
         for (int k = 0; k < kPrepVals; ++k) scal[k] = b[k];
+    }
 }
 
 // Node features, edge features (bf16-rounded) and the transpose map.
@@ -179,13 +194,14 @@ __global__ __launch_bounds__(256) void prep_feat_kernel(uint32_t n, const uint32
             const uint32_t j = col[e];
             if (j == i) {
                 efeat[e] = make_float2(0.f, 0.f);
-                tr[e] = e;
+                if (tr) tr[e] = e;
                 continue;
             }
             const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
             const double ah = val[e] * (sd[lo] * sd[hi]);
             const double len = edge_length(xy, xs, dim, lo, hi) / lbar;
             efeat[e] = make_float2(bfr((float)ah), bfr((float)len));
+            if (!tr) continue;  // inference: no transpose map
             // (j, i) in row j: binary search (columns sorted)
             uint32_t a = row[j], b = row[j + 1];
             while (a < b) {
@@ -441,7 +457,7 @@ __global__ __launch_bounds__(256) void msg_fwd_kernel(uint32_t n, const uint32_t
                 ef[u] = ok ? efeat[eb + u] : make_float2(0.f, 0.f);
             }
 #pragma unroll
-            for (uint32_t u = 0; u < kMsgBatch; ++u) qj[u] = pq[(size_t)jj[u] * kPq + kHid + f];
+            for (uint32_t u = 0; u < kMsgBatch; ++u) qj[u] = jj[u] != i ? pq[(size_t)jj[u] * kPq + kHid + f] : 0.f;
 #pragma unroll
             for (uint32_t u = 0; u < kMsgBatch; ++u) {
                 const uint32_t j = jj[u];
@@ -553,8 +569,10 @@ __global__ __launch_bounds__(256) void head_fwd_kernel(uint32_t n, const uint32_
                 ef[u] = ok ? efeat[eb + u] : make_float2(0.f, 0.f);
                 nz[u] = ok && val[eb + u] != 0.0;
             }
+            // only upper neighbours (j > k) carry an output: the lower part,
+            // the diagonal and the padding load nothing
 #pragma unroll
-            for (uint32_t u = 0; u < kMsgBatch; ++u) sj[u] = rsd[(size_t)jj[u] * kRsd + kHid + f];
+            for (uint32_t u = 0; u < kMsgBatch; ++u) sj[u] = jj[u] > k ? rsd[(size_t)jj[u] * kRsd + kHid + f] : 0.f;
 #pragma unroll
             for (uint32_t u = 0; u < kMsgBatch; ++u) {
                 if (jj[u] <= k) continue;  // lower part, diagonal and padding
@@ -860,7 +878,7 @@ void run_dw(uint32_t n, const float* x0, uint32_t ld0, const float* x1, uint32_t
 // Host side
 // ===========================================================================
 
-void mpnn_prepare(MpnnGraph& g, hipStream_t s) {
+void mpnn_prepare(MpnnGraph& g, hipStream_t s, bool transpose) {
     const uint32_t n = g.n;
     g.sd.resize(n ? n : 1);
     g.dpos.resize(n ? n : 1);
@@ -875,10 +893,11 @@ void mpnn_prepare(MpnnGraph& g, hipStream_t s) {
     g.part.resize((size_t)kPrepVals * nb);
     hipLaunchKernelGGL(prep_rows_kernel, dim3(nb), dim3(256), 0, s, n, g.row, g.col, g.val, g.xy, g.xy_stride, g.dim,
                        g.sd.p, g.dpos.p, g.deg.p, g.part.p);
-    hipLaunchKernelGGL(prep_fold_kernel, dim3(1), dim3(64), 0, s, nb, g.part.p, g.scal.p);
+    hipLaunchKernelGGL(prep_fold_kernel, dim3(1), dim3(kFoldThreads), 0, s, nb, g.part.p, g.scal.p);
     hipLaunchKernelGGL(prep_feat_kernel, dim3(blocks_for(n, 256, 4096)), dim3(256), 0, s, n, g.row, g.col, g.val,
-                       g.xy, g.xy_stride, g.dim, g.bnd, g.sd.p, g.dpos.p, g.scal.p, g.nfeat.p, g.efeat.p, g.tr.p,
-                       ctx().d_flags);
+                       g.xy, g.xy_stride, g.dim, g.bnd, g.sd.p, g.dpos.p, g.scal.p, g.nfeat.p, g.efeat.p,
+                       transpose ? g.tr.p : nullptr, ctx().d_flags);
+    g.has_transpose = transpose;
 }
 
 void MpnnNet::set_params(const float* host, hipStream_t s) {
@@ -1143,6 +1162,8 @@ int32_t PrecLearned_forward(PrecLearned* h, double* loss, float* ghat) {
 int32_t PrecLearned_gradient(PrecLearned* h, double* loss, float* grad) {
     return guarded([&] {
         if (!h) throw std::invalid_argument("PrecLearned_gradient: null handle");
+        if (!h->graph.has_transpose)
+            throw std::invalid_argument("the bound system was prepared for inference only: bind it for training (PrecLearned_bind_step / PrecLearned_set_system)");
         hipStream_t s = ctx().stream;
         h->net.forward(h->graph, s);
         h->net.loss(h->graph, s);
@@ -1186,6 +1207,8 @@ int32_t PrecLearned_time_gemm(PrecLearned* h, uint32_t reps, double* ms, double*
 int32_t PrecLearned_train_step(PrecLearned* h, float lr, double* loss) {
     return guarded([&] {
         if (!h || !(lr > 0.f)) throw std::invalid_argument("PrecLearned_train_step: bad arguments");
+        if (!h->graph.has_transpose)
+            throw std::invalid_argument("the bound system was prepared for inference only: bind it for training (PrecLearned_bind_step / PrecLearned_set_system)");
         hipStream_t s = ctx().stream;
         h->net.forward(h->graph, s);
         h->net.loss(h->graph, s);

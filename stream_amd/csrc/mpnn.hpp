@@ -90,6 +90,7 @@ struct MpnnGraph {
     DBuf<float> nfeat;        // [n][kNodeFeats]
     DBuf<float2> efeat;       // [nnz] (Ahat_ij, len_ij/lbar), rounded to bf16
     DBuf<uint32_t> tr;        // [nnz] position of (j, i) for entry (i, j)
+    bool has_transpose = false;  // tr built (the backward pass needs it)
     DBuf<double> part;        // reduction partials
     DBuf<double> scal;        // [8] xmin, xmax, ymin, ymax, zmin, zmax, len sum, count
     // owned copies for host-provided systems
@@ -98,8 +99,9 @@ struct MpnnGraph {
     DBuf<uint8_t> own_bnd;
 };
 
-// Features, scaling and transpose map of the bound system.
-void mpnn_prepare(MpnnGraph& g, hipStream_t s);
+// Features, scaling and (transpose = true) the transpose map of the bound
+// system; inference does not need the map.
+void mpnn_prepare(MpnnGraph& g, hipStream_t s, bool transpose = true);
 
 struct MpnnNet {
     DBuf<float> w, m, v, grad;   // [P] master weights, Adam moments, gradient

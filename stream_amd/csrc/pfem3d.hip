@@ -439,7 +439,7 @@ struct Step3D {
     }
 
     // Point the network's graph at this step's system (after assembly).
-    void bind_graph(MpnnGraph& g, hipStream_t s) {
+    void bind_graph(MpnnGraph& g, hipStream_t s, bool transpose = true) {
         g.n = n;
         g.nnz = A.nnz;
         g.row = A.row.p;
@@ -449,7 +449,7 @@ struct Step3D {
         g.xy_stride = sizeof(Sph4<double>) / sizeof(double);
         g.dim = 3;
         g.bnd = mesh.node_is_bnd.p;
-        mpnn_prepare(g, s);
+        mpnn_prepare(g, s, transpose);
     }
 
     // SELL copy and factor for the solver; the learned factor is rejected
@@ -465,7 +465,7 @@ struct Step3D {
             if (solver == 3) {
                 if (!learned) throw std::invalid_argument("solver 3 needs a network (PfemStep3D_set_learned)");
                 factor_structure<float>(A, L32, s);
-                bind_graph(learned->graph, s);
+                bind_graph(learned->graph, s, false);  // inference: no transpose map
                 learned->net.forward(learned->graph, s);
                 learned->net.pack(learned->graph, L32, s);
                 const uint32_t bad = read_scalar(ctx().d_flags + kFlagLearned);
