@@ -289,6 +289,38 @@ def test_sell_column_offsets_far_and_boundary():
         assert its < 200
 
 
+def test_sell_long_row():
+    """One row far longer than the rest: its SELL slice is 300 entries wide
+    while every other slice is 3 or 4 (row 0 coupled to rows 1..300, plus a
+    tridiagonal band and couplings 700 rows away, diagonally dominant SPD);
+    the FP64 and mixed solves reach 1e-10 in the true residual computed
+    here with scipy."""
+    from stream_amd.numerics import DeviceCSR, HostCSR, LinSys, PrecJacobi, SolverCG
+
+    _say("sell long row")
+    n = 20_000
+    rng = np.random.default_rng(9)
+    w = -rng.uniform(0.01, 0.02, 300)
+    star = sp.csr_matrix((np.r_[w, w], (np.r_[np.zeros(300), np.arange(1, 301)], np.r_[np.arange(1, 301),
+                                                                                        np.zeros(300)])), (n, n))
+    band = -rng.uniform(0.2, 0.6, n - 1)
+    far = -rng.uniform(0.1, 0.3, n - 700)
+    A = (sp.diags(np.full(n, 6.0)) + sp.diags(band, 1) + sp.diags(band, -1) + sp.diags(far, 700) +
+         sp.diags(far, -700) + star).tocsr()
+    A.sort_indices()
+    assert np.diff(A.indptr).max() > 256
+    b = rng.uniform(-1, 1, n)
+    d64 = DeviceCSR(HostCSR(A.indptr.astype(np.uint32), A.indices.astype(np.uint32), A.data.astype(np.float64)))
+    s64 = LinSys(d64, b)
+    p64 = PrecJacobi(d64)
+    for meth in ("solve_rel", "solve_mixed"):
+        xx = np.zeros(n)
+        its, rel = getattr(SolverCG(), meth)(s64, p64, xx, rtol=1e-10)
+        true_rel = np.linalg.norm(b - A @ xx) / np.linalg.norm(b)
+        print(f"{meth}: {its} iterations, true rel {true_rel:.2e}")
+        assert true_rel <= 1e-10 and rel <= 1e-10
+
+
 def test_dam_break_1m_every_solver_residual_and_error():
     """The 1M dam-break step with every solver: verified true FP64 relative
     residual <= 1e-8, bitwise symmetric matrix, and the error of each
