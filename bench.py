@@ -115,6 +115,16 @@ def sell_update_bytes(n, nnz):
     return n * 42 + (nnz - n) // 2 * 6 + nslices * 6
 
 
+def sell_lo_bytes(n, nnz):
+    """Algorithmic bytes of one L L^T lower-apply launch (z = L w, d_new =
+    z + beta d_old) on the lower SELL of the factor: per row the length
+    (2 B), L_ii (4 B), the own w (4 B), d_old (4 B), d_new written (4 B);
+    per strict-lower entry (nnz - n) / 2 its 16-bit column offset and FP32
+    value (6 B); per 64-row slice its offset and width (6 B)."""
+    nslices = (n + 63) // 64
+    return n * 18 + (nnz - n) // 2 * 6 + nslices * 6
+
+
 def section_bytes(n, nnz, tris, edges):
     """Algorithmic HBM bytes of the mesh-and-assembly stages on n FP64
     points (32-byte padded records), per stage, each array counted once:
@@ -420,6 +430,7 @@ def main():
         kms = step.time_dir_kernel(fp32=args.solver != "fp64", reps=20, llt=llt)
         if llt:
             step.time_update_kernel(reps=20)
+            step.time_update_kernel(reps=20, lower=True)
         cms = ctypes.c_double(0)
         stream_amd.check(stream_amd.lib.Stream_copy_calibration(ctypes.c_uint64(COPY_BYTES), 5, ctypes.byref(cms)))
         print(f"[profile] n={pts.shape[0]} nnz={r['nnz']} dir kernel {kms * 1000:.2f} us, "
@@ -460,9 +471,10 @@ def main():
     ms_pcie = (time.perf_counter() - t0) * 1000.0 / k_pcie
     print(f"[bench] {args.solver} with host-to-device points: {ms_pcie:.2f} ms/step", file=sys.stderr, flush=True)
 
-    # dominant kernel: on the L L^T path the update kernel (update + L^T
-    # apply), reported beside the direction kernel; on the Jacobi paths the
-    # direction kernel (d = z + beta d_old, SpMV, d.q)
+    # dominant kernel: on the L L^T path the longest of the direction
+    # kernel, the update kernel (x, r' and the L^T apply) and the lower apply
+    # (L apply, next direction), the other two reported beside it; on the
+    # Jacobi paths the direction kernel (d = z + beta d_old, SpMV, d.q)
     fp32 = args.solver != "fp64"
     llt = args.solver in ("learned", "fsai")
     n = pts.shape[0]
@@ -483,9 +495,17 @@ def main():
                     sell_dir_bytes(n, nnz, 4 if fp32 else 8, llt=llt), measured_traffic(dname, n, nnz, fp32))
     if llt:
         ums = step.time_update_kernel(reps=200)
-        roofline = roof("llt_update_kernel<float, double>", ums, sell_update_bytes(n, nnz),
+        lms = step.time_update_kernel(reps=200, lower=True)
+        upd_roof = roof("llt_update_kernel<float, double>", ums, sell_update_bytes(n, nnz),
                         measured_traffic("llt_update_kernel", n, nnz, fp32))
-        roofline["direction_kernel"] = dir_roof
+        lo_roof = roof("llt_lo_kernel<float>", lms, sell_lo_bytes(n, nnz),
+                       measured_traffic("llt_lo_kernel", n, nnz, fp32))
+        three = sorted([("direction_kernel", dir_roof), ("update_kernel", upd_roof), ("lower_kernel", lo_roof)],
+                       key=lambda kv: -kv[1]["kernel_ms"])
+        roofline = dict(three[0][1])
+        roofline["role"] = three[0][0]
+        for key, rf in three[1:]:
+            roofline[key] = rf
     else:
         roofline = dir_roof
 

@@ -141,8 +141,8 @@ int32_t PfemStep2D_get_mesh(PfemStep2D* step, uint32_t* elem, uint8_t* node_is_b
 /* Average ms of a CG kernel on the last step's system (HIP events on the
  * library stream, `reps` launches): direction kernel of kind 0 = FP64
  * Jacobi path, 1 = FP32 row-sum Jacobi path, 2 = FP32 row-sum L L^T path;
- * kind 3 = the L L^T update kernel on the learned factor (FSAI without a
- * network). */
+ * kind 3 = the L L^T update kernel (with the solution update), 4 = the
+ * L L^T lower apply, both on the learned factor (FSAI without a network). */
 int32_t PfemStep2D_time_dir_kernel(PfemStep2D* step, int32_t kind, uint32_t reps, double* ms);
 /* no reference counterpart: GPU ms of the mesh-and-assembly stages on the
  * current points, averaged over reps runs; ms [11] = box + Morton codes,

@@ -1318,46 +1318,56 @@ template double time_dir_kernel<float>(const Sell<float>&, bool, bool, uint32_t,
 template double time_dir_kernel<double>(const Sell<double>&, bool, bool, uint32_t, CgWork&, hipStream_t);
 
 // Average duration (ms) of the L L^T update kernel (x += alpha d,
-// r' = r - alpha q, w = L^T r', partials of w.w and r'.r') over `reps`
-// launches on factor L with the configured block size.  The vectors are
-// scratch (ones) and the direction kernel's partials ones, so alpha = 1/G.
+// r' = r - alpha q, w = L^T r', partials of w.w and r'.r') or of the lower
+// apply (z = L w, d_new = z + beta d_old) over `reps` launches on factor L
+// with the configured block size.  The vectors are scratch (ones), the
+// partials ones and the stop flag is never raised (maxit unbounded).
 template <class V, int CH, int NT>
-void update_launches(const LowerFactor<V>& L, uint32_t reps, CgVecs<V>& vv, double* xs, CgWork& w, hipStream_t s) {
-    SellArgs<V> MU = sell_args(L.up);
+void update_launches(const LowerFactor<V>& L, uint32_t reps, CgVecs<V>& vv, double* xs, CgWork& w, hipStream_t s,
+                     bool lower) {
+    SellArgs<V> MU = sell_args(L.up), ML = sell_args(L.lo);
     MU.diag = L.diag.p;
-    for (uint32_t k = 0; k < reps; ++k)
-        hipLaunchKernelGGL((llt_update_kernel<V, double, CH, 1, NT>), dim3(w.blocks), dim3(NT), 0, s, MU, xs,
-                           vv.D[k & 1], vv.RQ[k & 1], vv.RQ[(k + 1) & 1], vv.w, w.sc.p, w.partials.p, 0);
+    ML.diag = L.diag.p;
+    for (uint32_t k = 0; k < reps; ++k) {
+        if (lower)
+            hipLaunchKernelGGL((llt_lo_kernel<V, CH, 1, NT>), dim3(w.blocks), dim3(NT), 0, s, ML, vv.w, vv.D[k & 1],
+                               vv.D[(k + 1) & 1], w.sc.p, w.partials.p, 0);
+        else
+            hipLaunchKernelGGL((llt_update_kernel<V, double, CH, 1, NT>), dim3(w.blocks), dim3(NT), 0, s, MU, xs,
+                               vv.D[k & 1], vv.RQ[k & 1], vv.RQ[(k + 1) & 1], vv.w, w.sc.p, w.partials.p, 0);
+    }
 }
-template <class V> double time_llt_update_kernel(const LowerFactor<V>& L, uint32_t reps, CgWork& w, hipStream_t s) {
+template <class V>
+double time_llt_update_kernel(const LowerFactor<V>& L, uint32_t reps, CgWork& w, hipStream_t s, bool lower) {
     const uint32_t n = L.n;
     w.resize(n, tuning().llt_threads);
     CgVecs<V> vv = carve<V>(w, n);
     CgScalars h{};
     h.maxit = 0xffffffffu;
     h.rz = 1.0;
+    h.rz_old = 1.0;
     SA_HIP(hipMemcpyAsync(w.sc.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
     const size_t nv = ((size_t)n + 63) & ~(size_t)63;
     fill<V>(reinterpret_cast<V*>(vv.P[0]), 10 * nv + 3 * nv, (V)1, s);  // P, RQ, D, r, q, w
     DBuf<double> xs;  // scratch solution
     xs.resize(n);
     fill<double>(xs.p, n, 0.0, s);
-    fill<double>(w.partials.p, kMaxCgBlocks, 1.0, s);
+    fill<double>(w.partials.p, 3 * kMaxCgBlocks, 1.0, s);
     const bool c4 = tuning().sell_chunk == 4;
     const uint32_t nt = w.threads;
     auto launch = [&](uint32_t r) {
         if (c4 && nt == 256)
-            update_launches<V, 4, 256>(L, r, vv, xs.p, w, s);
+            update_launches<V, 4, 256>(L, r, vv, xs.p, w, s, lower);
         else if (c4 && nt == 512)
-            update_launches<V, 4, 512>(L, r, vv, xs.p, w, s);
+            update_launches<V, 4, 512>(L, r, vv, xs.p, w, s, lower);
         else if (c4)
-            update_launches<V, 4, 1024>(L, r, vv, xs.p, w, s);
+            update_launches<V, 4, 1024>(L, r, vv, xs.p, w, s, lower);
         else if (nt == 256)
-            update_launches<V, 8, 256>(L, r, vv, xs.p, w, s);
+            update_launches<V, 8, 256>(L, r, vv, xs.p, w, s, lower);
         else if (nt == 512)
-            update_launches<V, 8, 512>(L, r, vv, xs.p, w, s);
+            update_launches<V, 8, 512>(L, r, vv, xs.p, w, s, lower);
         else
-            update_launches<V, 8, 1024>(L, r, vv, xs.p, w, s);
+            update_launches<V, 8, 1024>(L, r, vv, xs.p, w, s, lower);
     };
     launch(3);
     hipEvent_t e0, e1;
@@ -1373,6 +1383,6 @@ template <class V> double time_llt_update_kernel(const LowerFactor<V>& L, uint32
     SA_HIP(hipEventDestroy(e1));
     return reps ? (double)ms / reps : 0.0;
 }
-template double time_llt_update_kernel<float>(const LowerFactor<float>&, uint32_t, CgWork&, hipStream_t);
+template double time_llt_update_kernel<float>(const LowerFactor<float>&, uint32_t, CgWork&, hipStream_t, bool);
 
 }  // namespace stream_amd

@@ -162,7 +162,9 @@ void true_residual(const DevCsr<double>& A, const double* b, const double* x, do
 // the L L^T path's kernel, else the Jacobi path's).
 template <class V>
 double time_dir_kernel(const Sell<V>& S, bool diff, bool llt, uint32_t reps, CgWork& w, hipStream_t s);
-// Average duration (ms) of the L L^T update kernel over `reps` launches on L.
-template <class V> double time_llt_update_kernel(const LowerFactor<V>& L, uint32_t reps, CgWork& w, hipStream_t s);
+// Average duration (ms) of the L L^T update kernel (lower: the lower apply)
+// over `reps` launches on L.
+template <class V>
+double time_llt_update_kernel(const LowerFactor<V>& L, uint32_t reps, CgWork& w, hipStream_t s, bool lower);
 
 }  // namespace stream_amd

@@ -1326,16 +1326,17 @@ int32_t PfemStep2D_run(PfemStep2D* h, int32_t solver, double rtol, uint32_t maxi
 // 0 = FP64 Jacobi direction kernel (d = z + beta d_old, q = A d, d.q), 1 =
 // FP32 row-sum Jacobi direction kernel, 2 = FP32 row-sum L L^T direction
 // kernel (q = A d on a ready-made d, d.q), 3 = L L^T update kernel (x +=
-// alpha d, r' = r - alpha q, w = L^T r') on the learned factor when a
-// network is set, else the FSAI factor.
+// alpha d, r' = r - alpha q, w = L^T r'), 4 = L L^T lower apply (z = L w,
+// d_new = z + beta d_old); 3 and 4 on the learned factor when a network is
+// set, else the FSAI factor.
 int32_t PfemStep2D_time_dir_kernel(PfemStep2D* h, int32_t kind, uint32_t reps, double* ms) {
     return g([&] {
-        if (kind < 0 || kind > 3) throw std::invalid_argument("time_dir_kernel: kind must be 0, 1, 2 or 3");
+        if (kind < 0 || kind > 4) throw std::invalid_argument("time_dir_kernel: kind must be 0 to 4");
         h->s.require_system();
         CgWork w;
-        if (kind == 3) {
+        if (kind >= 3) {
             h->s.prepare(h->s.learned ? 3 : 2, ctx().stream);
-            *ms = time_llt_update_kernel<float>(h->s.L32, reps, w, ctx().stream);
+            *ms = time_llt_update_kernel<float>(h->s.L32, reps, w, ctx().stream, kind == 4);
             return;
         }
         h->s.prepare(kind ? 1 : 0, ctx().stream);

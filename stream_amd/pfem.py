@@ -128,11 +128,12 @@ class PfemStep2D:
         check(PfemStep2D_time_dir_kernel(self.h, 2 if llt else int(fp32), reps, ctypes.byref(ms)))
         return ms.value
 
-    def time_update_kernel(self, reps: int = 200) -> float:
+    def time_update_kernel(self, reps: int = 200, lower: bool = False) -> float:
         """Average ms of the L L^T update kernel (x += alpha d, r' = r - alpha
-        q, w = L^T r') on the learned factor (FSAI without a network)."""
+        q, w = L^T r') or, with lower, of the lower apply (z = L w, d_new =
+        z + beta d_old), on the learned factor (FSAI without a network)."""
         ms = ctypes.c_double(0.0)
-        check(PfemStep2D_time_dir_kernel(self.h, 3, reps, ctypes.byref(ms)))
+        check(PfemStep2D_time_dir_kernel(self.h, 4 if lower else 3, reps, ctypes.byref(ms)))
         return ms.value
 
     def set_velocity(self, uv):

@@ -105,6 +105,7 @@ def test_step_entry_points_need_an_assembled_system():
             call()
     st.run("fsai", rtol=1e-8)
     assert st.time_dir_kernel(True, 2, llt=True) > 0 and st.time_update_kernel(2) > 0
+    assert st.time_update_kernel(2, lower=True) > 0
     assert st.solution().shape == (pts.shape[0],)
     st.set_points(pts, al, f)
     with pytest.raises(stream_amd.StreamError):
