@@ -171,78 +171,170 @@ template <class S, bool EXACT> __device__ bool local_delaunay(const AshArgs<S>& 
         const uint32_t sj = neig(ni);
         const S pd[2] = {P[sj].c[0], P[sj].c[1]};
         uint32_t rsize = 0;
-        uint32_t ti = 0;
-        while (ti < tlast) {
-            const uint16_t e = T(ti);
-            const uint32_t ea = e & 255u, eb = e >> 8;
-            uint32_t i1 = si, i2 = neig(ea), i3 = neig(eb), i4 = sj;
-            const Sph4<S> sb = P[i2];
-            const Sph4<S> sc = P[i3];
-            S pb[2] = {sb.c[0], sb.c[1]};
-            S pc[2] = {sc.c[0], sc.c[1]};
-            bool certain;
-            S orientation = orient2d_filter(pa, pb, pc, &certain);
-            if (!certain) {
-                if constexpr (!EXACT)
-                    return false;
-                else
-                    orientation = (S)orient2d_exact(pa, pb, pc, ws, kExactWsDoubles, &over);
-            }
-            S err;
-            S det = incircle_filter(pa, pb, pc, pd, &err);
-            if (!((det > err) || (-det > err))) {
-                if constexpr (!EXACT) {
-                    return false;
-                } else {
-                    S qa[2] = {pa[0], pa[1]}, qb[2] = {pb[0], pb[1]}, qc[2] = {pc[0], pc[1]}, qd[2] = {pd[0], pd[1]};
-                    bool swap = false;
-                    auto cs = [&](uint32_t& x, uint32_t& y, S* px, S* py) {
-                        if (x > y) {
-                            const uint32_t t = x;
-                            x = y;
-                            y = t;
-                            const S u0 = px[0], u1 = px[1];
-                            px[0] = py[0];
-                            px[1] = py[1];
-                            py[0] = u0;
-                            py[1] = u1;
-                            swap = !swap;
-                        }
-                    };
-                    cs(i1, i3, qa, qc);
-                    cs(i2, i4, qb, qd);
-                    cs(i1, i2, qa, qb);
-                    cs(i3, i4, qc, qd);
-                    cs(i2, i3, qb, qc);
-                    det = (S)incircle_exact(qa, qb, qc, qd, ws, kExactWsDoubles, &over);
-                    int depth = 0;
-                    while (det == (S)0) {
-                        ++depth;
-                        if (depth == 1) {
-                            bool c1;
-                            det = orient2d_filter(qb, qc, qd, &c1);
-                            if (!c1) det = (S)orient2d_exact(qb, qc, qd, ws, kExactWsDoubles, &over);
-                        } else if (depth == 2) {
-                            bool c2;
-                            det = orient2d_filter(qa, qc, qd, &c2);
-                            if (!c2) det = (S)orient2d_exact(qa, qc, qd, ws, kExactWsDoubles, &over);
-                            det = -det;
-                        } else {
-                            det = ((int)swap + (int)(orientation < (S)0)) == 1 ? (S)1 : (S)-1;
-                        }
+        if constexpr (!EXACT) {
+            // Fast path (filters only; a node whose filter cannot decide is
+            // redone by the exact kernel, which takes the sequential loop below).
+            // Conflict flags of the current triangles with the new point, bit k
+            // of word k >> 6 (f0..f3) for position k: every triangle is tested
+            // once, two per step so that their loads overlap; the next pair's
+            // point ids load while the current pair is tested.
+            uint64_t f0 = 0, f1 = 0, f2 = 0, f3 = 0;
+            auto flag = [&](uint32_t k) -> bool {
+                const uint64_t w = k < 64 ? f0 : (k < 128 ? f1 : (k < 192 ? f2 : f3));
+                return (w >> (k & 63u)) & 1u;
+            };
+            auto set_flag = [&](uint32_t k, bool v) {
+                const uint64_t m = 1ull << (k & 63u);
+                const uint32_t q = k >> 6;
+                f0 = q == 0 ? (v ? f0 | m : f0 & ~m) : f0;
+                f1 = q == 1 ? (v ? f1 | m : f1 & ~m) : f1;
+                f2 = q == 2 ? (v ? f2 | m : f2 & ~m) : f2;
+                f3 = q == 3 ? (v ? f3 | m : f3 & ~m) : f3;
+            };
+            // first flagged position >= k (256 when none)
+            auto next_flag = [&](uint32_t k) -> uint32_t {
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) {
+                    if (k < q * 64 + 64) {
+                        uint64_t m = q == 0 ? f0 : (q == 1 ? f1 : (q == 2 ? f2 : f3));
+                        if (k > q * 64) m &= ~0ull << (k - q * 64);
+                        if (m) return q * 64 + (uint32_t)__builtin_ctzll(m);
                     }
-                    if (swap) det = -det;
+                }
+                return 256u;
+            };
+            auto pair_ids = [&](uint32_t p0, uint32_t (&o)[2][2]) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const uint16_t e = T(u == 0 || p0 + 1 >= tlast ? p0 : p0 + 1);
+                    o[u][0] = neig(e & 255u);
+                    o[u][1] = neig(e >> 8);
+                }
+            };
+            uint32_t ids[2][2];
+            if (tlast) pair_ids(0, ids);
+            for (uint32_t p0 = 0; p0 < tlast; p0 += 2) {
+                const bool two = p0 + 1 < tlast;
+                S pb[2][2], pc[2][2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const Sph4<S> sb = P[ids[u][0]];
+                    const Sph4<S> sc = P[ids[u][1]];
+                    pb[u][0] = sb.c[0];
+                    pb[u][1] = sb.c[1];
+                    pc[u][0] = sc.c[0];
+                    pc[u][1] = sc.c[1];
+                }
+                uint32_t nids[2][2];
+                const bool more = p0 + 2 < tlast;
+                if (more) pair_ids(p0 + 2, nids);
+                S orientation[2], det[2], err[2];
+                bool certain[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    orientation[u] = orient2d_filter(pa, pb[u], pc[u], &certain[u]);
+                    det[u] = incircle_filter(pa, pb[u], pc[u], pd, &err[u]);
+                }
+                if (!certain[0] || !((det[0] > err[0]) || (-det[0] > err[0]))) return false;
+                if (two && (!certain[1] || !((det[1] > err[1]) || (-det[1] > err[1])))) return false;
+                if (orientation[0] < (S)0) det[0] = -det[0];
+                if (orientation[1] < (S)0) det[1] = -det[1];
+                if (det[0] > (S)0) set_flag(p0, true);
+                if (two && det[1] > (S)0) set_flag(p0 + 1, true);
+                if (more) {
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        ids[u][0] = nids[u][0];
+                        ids[u][1] = nids[u][1];
+                    }
                 }
             }
-            if (orientation < (S)0) det = -det;
-            if (det > (S)0) {
-                tag_op(t0, t1, t2, t3, ea, true);
-                tag_op(t0, t1, t2, t3, eb, true);
+            // The cavity: the sequential order of cpp:419-495 (a conflicting
+            // triangle tags its two edges and is replaced by the last one, which
+            // is tested next at the same position) replayed from the flags.
+            for (uint32_t ti = next_flag(0); ti < tlast; ti = next_flag(ti)) {
+                const uint16_t e = T(ti);
+                tag_op(t0, t1, t2, t3, e & 255u, true);
+                tag_op(t0, t1, t2, t3, e >> 8, true);
                 ++rsize;
                 --tlast;
                 T(ti) = T(tlast);
-            } else {
-                ++ti;
+                set_flag(ti, flag(tlast));
+            }
+        } else {
+            uint32_t ti = 0;
+            while (ti < tlast) {
+                const uint16_t e = T(ti);
+                const uint32_t ea = e & 255u, eb = e >> 8;
+                uint32_t i1 = si, i2 = neig(ea), i3 = neig(eb), i4 = sj;
+                const Sph4<S> sb = P[i2];
+                const Sph4<S> sc = P[i3];
+                S pb[2] = {sb.c[0], sb.c[1]};
+                S pc[2] = {sc.c[0], sc.c[1]};
+                bool certain;
+                S orientation = orient2d_filter(pa, pb, pc, &certain);
+                if (!certain) {
+                    if constexpr (!EXACT)
+                        return false;
+                    else
+                        orientation = (S)orient2d_exact(pa, pb, pc, ws, kExactWsDoubles, &over);
+                }
+                S err;
+                S det = incircle_filter(pa, pb, pc, pd, &err);
+                if (!((det > err) || (-det > err))) {
+                    if constexpr (!EXACT) {
+                        return false;
+                    } else {
+                        S qa[2] = {pa[0], pa[1]}, qb[2] = {pb[0], pb[1]}, qc[2] = {pc[0], pc[1]}, qd[2] = {pd[0], pd[1]};
+                        bool swap = false;
+                        auto cs = [&](uint32_t& x, uint32_t& y, S* px, S* py) {
+                            if (x > y) {
+                                const uint32_t t = x;
+                                x = y;
+                                y = t;
+                                const S u0 = px[0], u1 = px[1];
+                                px[0] = py[0];
+                                px[1] = py[1];
+                                py[0] = u0;
+                                py[1] = u1;
+                                swap = !swap;
+                            }
+                        };
+                        cs(i1, i3, qa, qc);
+                        cs(i2, i4, qb, qd);
+                        cs(i1, i2, qa, qb);
+                        cs(i3, i4, qc, qd);
+                        cs(i2, i3, qb, qc);
+                        det = (S)incircle_exact(qa, qb, qc, qd, ws, kExactWsDoubles, &over);
+                        int depth = 0;
+                        while (det == (S)0) {
+                            ++depth;
+                            if (depth == 1) {
+                                bool c1;
+                                det = orient2d_filter(qb, qc, qd, &c1);
+                                if (!c1) det = (S)orient2d_exact(qb, qc, qd, ws, kExactWsDoubles, &over);
+                            } else if (depth == 2) {
+                                bool c2;
+                                det = orient2d_filter(qa, qc, qd, &c2);
+                                if (!c2) det = (S)orient2d_exact(qa, qc, qd, ws, kExactWsDoubles, &over);
+                                det = -det;
+                            } else {
+                                det = ((int)swap + (int)(orientation < (S)0)) == 1 ? (S)1 : (S)-1;
+                            }
+                        }
+                        if (swap) det = -det;
+                    }
+                }
+                if (orientation < (S)0) det = -det;
+                if (det > (S)0) {
+                    tag_op(t0, t1, t2, t3, ea, true);
+                    tag_op(t0, t1, t2, t3, eb, true);
+                    ++rsize;
+                    --tlast;
+                    T(ti) = T(tlast);
+                } else {
+                    ++ti;
+                }
             }
         }
         if (rsize) {

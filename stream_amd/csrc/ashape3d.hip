@@ -165,76 +165,160 @@ __device__ bool local_delaunay3(const Ash3Args<S>& a, uint32_t tid, uint8_t* cyc
         S pe0[3];
         load3(P, curr, pe0);
         uint32_t Rsize = 0;
-        uint32_t ti = 0;
-        while (ti < Tlast) {
-            const uint32_t tl = T(ti);
-            const uint32_t la = tl & 255u, lb = (tl >> 8) & 255u, lc = (tl >> 16) & 255u;
-            uint32_t i2 = neig(la), i3 = neig(lb), i4 = neig(lc);
-            S pb[3], pc[3], pd[3];
-            load3(P, i2, pb);
-            load3(P, i3, pc);
-            load3(P, i4, pd);
-            S err;
-            S det = -insphere_filter(pa0, pb, pc, pd, pe0, &err);
-            if (!((det > err) || (-det > err))) {
-                if constexpr (!EXACT) {
-                    return false;
-                } else {
-                    uint32_t i1 = tid, i5 = curr;
-                    S qa[3] = {pa0[0], pa0[1], pa0[2]}, qe[3] = {pe0[0], pe0[1], pe0[2]};
-                    S* p1 = qa;
-                    S* p2 = pb;
-                    S* p3 = pc;
-                    S* p4 = pd;
-                    S* p5 = qe;
-                    bool swap = false;
-                    auto cs = [&](uint32_t& x, uint32_t& y, S*& px, S*& py) {
-                        if (x > y) {
-                            const uint32_t t = x;
-                            x = y;
-                            y = t;
-                            S* u = px;
-                            px = py;
-                            py = u;
-                            swap = !swap;
-                        }
-                    };
-                    cs(i1, i5, p1, p5);
-                    cs(i2, i4, p2, p4);
-                    cs(i1, i3, p1, p3);
-                    cs(i3, i5, p3, p5);
-                    cs(i1, i2, p1, p2);
-                    cs(i3, i4, p3, p4);
-                    cs(i2, i5, p2, p5);
-                    cs(i2, i3, p2, p3);
-                    cs(i4, i5, p4, p5);
-                    det = (S)-insphere_exact(p1, p2, p3, p4, p5, ws, (int)kExactWs3D, &over);
-                    int depth = 0;
-                    while (det == (S)0) {
-                        ++depth;
-                        if (depth == 1) {
-                            bool c1;
-                            det = orient3d_filter(p2, p3, p4, p5, &c1);
-                            if (!c1) det = (S)orient3d_exact(p2, p3, p4, p5, ws, (int)kExactWs3D, &over);
-                        } else if (depth == 2) {
-                            bool c2;
-                            det = orient3d_filter(p1, p3, p4, p5, &c2);
-                            if (!c2) det = (S)orient3d_exact(p1, p3, p4, p5, ws, (int)kExactWs3D, &over);
-                            det = -det;
-                        } else {
-                            det = swap ? (S)1 : (S)-1;
-                        }
+        if constexpr (!EXACT) {
+            // Fast path (filters only; a node whose filter cannot decide is
+            // redone by the exact kernel, which takes the sequential loop below).
+            // Conflict flags of the current tetrahedra with the new point, bit
+            // k of word k >> 6 (f0..f3) for position k: every tetrahedron is
+            // tested once, two per step so that their loads overlap.
+            uint64_t f0 = 0, f1 = 0, f2 = 0, f3 = 0;
+            auto flag = [&](uint32_t k) -> bool {
+                const uint64_t w = k < 64 ? f0 : (k < 128 ? f1 : (k < 192 ? f2 : f3));
+                return (w >> (k & 63u)) & 1u;
+            };
+            auto set_flag = [&](uint32_t k, bool v) {
+                const uint64_t m = 1ull << (k & 63u);
+                const uint32_t q = k >> 6;
+                f0 = q == 0 ? (v ? f0 | m : f0 & ~m) : f0;
+                f1 = q == 1 ? (v ? f1 | m : f1 & ~m) : f1;
+                f2 = q == 2 ? (v ? f2 | m : f2 & ~m) : f2;
+                f3 = q == 3 ? (v ? f3 | m : f3 & ~m) : f3;
+            };
+            // first flagged position >= k (256 when none)
+            auto next_flag = [&](uint32_t k) -> uint32_t {
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) {
+                    if (k < q * 64 + 64) {
+                        uint64_t m = q == 0 ? f0 : (q == 1 ? f1 : (q == 2 ? f2 : f3));
+                        if (k > q * 64) m &= ~0ull << (k - q * 64);
+                        if (m) return q * 64 + (uint32_t)__builtin_ctzll(m);
                     }
-                    if (swap) det = -det;
+                }
+                return 256u;
+            };
+            // point ids of the pair at t0 (the second repeats the first past
+            // the end); the next pair's ids load while the current pair is tested
+            auto pair_ids = [&](uint32_t t0, uint32_t (&o)[2][3]) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const uint32_t tl = T(u == 0 || t0 + 1 >= Tlast ? t0 : t0 + 1);
+                    o[u][0] = neig(tl & 255u);
+                    o[u][1] = neig((tl >> 8) & 255u);
+                    o[u][2] = neig((tl >> 16) & 255u);
+                }
+            };
+            uint32_t ids[2][3];
+            if (Tlast) pair_ids(0, ids);
+            for (uint32_t t0 = 0; t0 < Tlast; t0 += 2) {
+                const bool two = t0 + 1 < Tlast;
+                S pb[2][3], pc[2][3], pd[2][3];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    load3(P, ids[u][0], pb[u]);
+                    load3(P, ids[u][1], pc[u]);
+                    load3(P, ids[u][2], pd[u]);
+                }
+                uint32_t nids[2][3];
+                const bool more = t0 + 2 < Tlast;
+                if (more) pair_ids(t0 + 2, nids);
+                S det[2], err[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) det[u] = -insphere_filter(pa0, pb[u], pc[u], pd[u], pe0, &err[u]);
+                if (!((det[0] > err[0]) || (-det[0] > err[0]))) return false;
+                if (two && !((det[1] > err[1]) || (-det[1] > err[1]))) return false;
+                if (det[0] > (S)0) set_flag(t0, true);
+                if (two && det[1] > (S)0) set_flag(t0 + 1, true);
+                if (more) {
+#pragma unroll
+                    for (int u = 0; u < 2; ++u)
+#pragma unroll
+                        for (int v = 0; v < 3; ++v) ids[u][v] = nids[u][v];
                 }
             }
-            if (det > (S)0) {
+            // The cavity: the sequential order of cpp:421-495 (a conflicting
+            // tetrahedron is swapped with the last one, which is tested next at
+            // the same position) replayed from the flags.
+            for (uint32_t ti = next_flag(0); ti < Tlast; ti = next_flag(ti)) {
                 ++Rsize;
                 --Tlast;
+                const uint32_t tl = T(ti);
                 T(ti) = T(Tlast);
                 T(Tlast) = tl;
-            } else {
-                ++ti;
+                set_flag(ti, flag(Tlast));
+                set_flag(Tlast, true);
+            }
+        } else {
+            uint32_t ti = 0;
+            while (ti < Tlast) {
+                const uint32_t tl = T(ti);
+                const uint32_t la = tl & 255u, lb = (tl >> 8) & 255u, lc = (tl >> 16) & 255u;
+                uint32_t i2 = neig(la), i3 = neig(lb), i4 = neig(lc);
+                S pb[3], pc[3], pd[3];
+                load3(P, i2, pb);
+                load3(P, i3, pc);
+                load3(P, i4, pd);
+                S err;
+                S det = -insphere_filter(pa0, pb, pc, pd, pe0, &err);
+                if (!((det > err) || (-det > err))) {
+                    if constexpr (!EXACT) {
+                        return false;
+                    } else {
+                        uint32_t i1 = tid, i5 = curr;
+                        S qa[3] = {pa0[0], pa0[1], pa0[2]}, qe[3] = {pe0[0], pe0[1], pe0[2]};
+                        S* p1 = qa;
+                        S* p2 = pb;
+                        S* p3 = pc;
+                        S* p4 = pd;
+                        S* p5 = qe;
+                        bool swap = false;
+                        auto cs = [&](uint32_t& x, uint32_t& y, S*& px, S*& py) {
+                            if (x > y) {
+                                const uint32_t t = x;
+                                x = y;
+                                y = t;
+                                S* u = px;
+                                px = py;
+                                py = u;
+                                swap = !swap;
+                            }
+                        };
+                        cs(i1, i5, p1, p5);
+                        cs(i2, i4, p2, p4);
+                        cs(i1, i3, p1, p3);
+                        cs(i3, i5, p3, p5);
+                        cs(i1, i2, p1, p2);
+                        cs(i3, i4, p3, p4);
+                        cs(i2, i5, p2, p5);
+                        cs(i2, i3, p2, p3);
+                        cs(i4, i5, p4, p5);
+                        det = (S)-insphere_exact(p1, p2, p3, p4, p5, ws, (int)kExactWs3D, &over);
+                        int depth = 0;
+                        while (det == (S)0) {
+                            ++depth;
+                            if (depth == 1) {
+                                bool c1;
+                                det = orient3d_filter(p2, p3, p4, p5, &c1);
+                                if (!c1) det = (S)orient3d_exact(p2, p3, p4, p5, ws, (int)kExactWs3D, &over);
+                            } else if (depth == 2) {
+                                bool c2;
+                                det = orient3d_filter(p1, p3, p4, p5, &c2);
+                                if (!c2) det = (S)orient3d_exact(p1, p3, p4, p5, ws, (int)kExactWs3D, &over);
+                                det = -det;
+                            } else {
+                                det = swap ? (S)1 : (S)-1;
+                            }
+                        }
+                        if (swap) det = -det;
+                    }
+                }
+                if (det > (S)0) {
+                    ++Rsize;
+                    --Tlast;
+                    T(ti) = T(Tlast);
+                    T(Tlast) = tl;
+                } else {
+                    ++ti;
+                }
             }
         }
         if (Rsize) {
