@@ -208,9 +208,71 @@ struct P1TetArgs {
     double* b;
 };
 
-__global__ __launch_bounds__(256) void p1_tet_kernel(P1TetArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+// Terms of tetrahedron t at row i: the gradients of the P1 basis (cross
+// products of the edges from the lowest vertex), the stiffness
+// K_i,other = g_i . g_other / (6 |det|) of the three edges through i, in the
+// order of the vertex pairs (0,1) (0,2) (0,3) (1,2) (1,3) (2,3), and the
+// lumped mass share (|det| / 6) / 4.
+__device__ __forceinline__ void tet_terms(const P1TetArgs& a, uint32_t i, uint32_t t, uint32_t (&oth)[3],
+                                          double (&kv)[3], double& mass_term) {
+    uint32_t v[4] = {a.elem[4ull * t], a.elem[4ull * t + 1], a.elem[4ull * t + 2], a.elem[4ull * t + 3]};
+    sort4(v);
+    double p[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const Sph4<double> s = a.P[v[q]];
+        p[q][0] = s.c[0];
+        p[q][1] = s.c[1];
+        p[q][2] = s.c[2];
+    }
+    double e[3][3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) e[q][c] = p[q + 1][c] - p[0][c];
+    double g[4][3];
+    auto cross = [](const double* x, const double* y, double* o) {
+        o[0] = x[1] * y[2] - x[2] * y[1];
+        o[1] = x[2] * y[0] - x[0] * y[2];
+        o[2] = x[0] * y[1] - x[1] * y[0];
+    };
+    cross(e[1], e[2], g[1]);
+    cross(e[2], e[0], g[2]);
+    cross(e[0], e[1], g[3]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g[0][c] = -((g[1][c] + g[2][c]) + g[3][c]);
+    const double det = e[0][0] * g[1][0] + e[0][1] * g[1][1] + e[0][2] * g[1][2];
+    const double ad = fabs(det);
+    const double den = 6.0 * ad;
+    int li = 0;
+#pragma unroll
+    for (int q = 1; q < 4; ++q)
+        if (v[q] == i) li = q;
+    constexpr int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {1, 2, 3, 2, 3, 3};
+    int o = 0;
+#pragma unroll
+    for (int kk = 0; kk < 6; ++kk) {
+        if (pa[kk] != li && pb[kk] != li) continue;
+        const double* x = g[pa[kk]];
+        const double* y = g[pb[kk]];
+        const double kvv = (x[0] * y[0] + x[1] * y[1] + x[2] * y[2]) / den;
+        const uint32_t other = v[pa[kk] == li ? pb[kk] : pa[kk]];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            if (q == o) {
+                oth[q] = other;
+                kv[q] = kvv;
+            }
+        ++o;
+    }
+    mass_term = (ad / 6.0) / 4.0;
+}
+
+// Row i by one thread (rows with more than 64 tetrahedra or columns): the
+// terms of the incident tetrahedra in incidence order, each added to its
+// column (binary search in the sorted row), then the Dirichlet rows and
+// columns, the diagonal as minus the off-diagonal sum, and the load.
+__device__ void p1_tet_row_serial(const P1TetArgs& a, uint32_t i) {
     const uint32_t r0 = a.arow[i], cnt = a.arow[i + 1] - r0;
     const uint32_t* src = a.tmp + 3ull * a.toff[i] + i;
     uint32_t* col = a.acol + r0;
@@ -230,53 +292,14 @@ __global__ __launch_bounds__(256) void p1_tet_kernel(P1TetArgs a) {
         }
         return lo;
     };
-    constexpr int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {1, 2, 3, 2, 3, 3};
     double mass = 0.0;
     for (uint32_t k = a.toff[i]; k < a.toff[i + 1]; ++k) {
-        const uint32_t t = a.inc[k];
-        uint32_t v[4] = {a.elem[4ull * t], a.elem[4ull * t + 1], a.elem[4ull * t + 2], a.elem[4ull * t + 3]};
-        sort4(v);
-        double p[4][3];
+        uint32_t oth[3];
+        double kv[3], mt;
+        tet_terms(a, i, a.inc[k], oth, kv, mt);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const Sph4<double> s = a.P[v[q]];
-            p[q][0] = s.c[0];
-            p[q][1] = s.c[1];
-            p[q][2] = s.c[2];
-        }
-        double e[3][3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) e[q][c] = p[q + 1][c] - p[0][c];
-        double g[4][3];
-        auto cross = [](const double* x, const double* y, double* o) {
-            o[0] = x[1] * y[2] - x[2] * y[1];
-            o[1] = x[2] * y[0] - x[0] * y[2];
-            o[2] = x[0] * y[1] - x[1] * y[0];
-        };
-        cross(e[1], e[2], g[1]);
-        cross(e[2], e[0], g[2]);
-        cross(e[0], e[1], g[3]);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) g[0][c] = -((g[1][c] + g[2][c]) + g[3][c]);
-        const double det = e[0][0] * g[1][0] + e[0][1] * g[1][1] + e[0][2] * g[1][2];
-        const double ad = fabs(det);
-        const double den = 6.0 * ad;
-        int li = 0;
-#pragma unroll
-        for (int q = 1; q < 4; ++q)
-            if (v[q] == i) li = q;
-#pragma unroll
-        for (int kk = 0; kk < 6; ++kk) {
-            if (pa[kk] != li && pb[kk] != li) continue;
-            const double* x = g[pa[kk]];
-            const double* y = g[pb[kk]];
-            const double kv = (x[0] * y[0] + x[1] * y[1] + x[2] * y[2]) / den;
-            const uint32_t other = v[pa[kk] == li ? pb[kk] : pa[kk]];
-            val[find(other)] += kv;
-        }
-        mass += (ad / 6.0) / 4.0;
+        for (int q = 0; q < 3; ++q) val[find(oth[q])] += kv[q];
+        mass += mt;
     }
     const bool bi = a.bnd[i] != 0;
     double diag = 0.0;
@@ -301,6 +324,66 @@ __global__ __launch_bounds__(256) void p1_tet_kernel(P1TetArgs a) {
     a.idiag[i] = id;
     a.idiag32[i] = (float)id;
     a.b[i] = bi ? 0.0 : a.f[i] * mass;
+}
+
+// One wavefront per row: lane t computes the terms of the row's t-th
+// incident tetrahedron, lane c holds column c of the sorted row and adds the
+// terms that fall on it in tetrahedron order (each tetrahedron reaches a
+// column at most once), so every entry is the same sum in the same order as
+// the serial row; the mass and the diagonal are summed in the serial order
+// too.  Larger rows take the serial code on lane 0.
+__global__ __launch_bounds__(256) void p1_tet_kernel(P1TetArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t i0 = blockIdx.x * 4 + (threadIdx.x >> 6); i0 < a.n; i0 += gridDim.x * 4) {
+        const uint32_t i = __builtin_amdgcn_readfirstlane(i0);
+        const uint32_t b0 = a.toff[i], m = a.toff[i + 1] - b0;
+        const uint32_t r0 = a.arow[i], cnt = a.arow[i + 1] - r0;
+        if (m > 64 || cnt > 64) {
+            if (lane == 0) p1_tet_row_serial(a, i);
+            continue;
+        }
+        uint32_t oth[3] = {~0u, ~0u, ~0u};
+        double kv[3] = {0.0, 0.0, 0.0}, mt = 0.0;
+        if (lane < m) tet_terms(a, i, a.inc[b0 + lane], oth, kv, mt);
+        const uint32_t cc = lane < cnt ? a.tmp[3ull * b0 + i + lane] : ~0u;
+        double acc = 0.0, mass = 0.0;
+        for (uint32_t t = 0; t < m; ++t) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const uint32_t o = (uint32_t)__shfl((int)oth[q], (int)t, 64);
+                const double k = __shfl(kv[q], (int)t, 64);
+                if (o == cc) acc += k;
+            }
+            mass += __shfl(mt, (int)t, 64);
+        }
+        const bool bi = a.bnd[i] != 0;
+        double diag = 0.0;
+        uint32_t dpos = 0;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const uint32_t ck = (uint32_t)__shfl((int)cc, (int)k, 64);
+            const double vk = __shfl(acc, (int)k, 64);
+            if (ck == i)
+                dpos = k;
+            else
+                diag -= vk;
+        }
+        if (lane < cnt) {
+            double vv;
+            if (lane == dpos)
+                vv = bi ? 1.0 : diag;
+            else
+                vv = (bi || a.bnd[cc]) ? 0.0 : acc;
+            a.acol[r0 + lane] = cc;
+            a.aval[r0 + lane] = vv;
+        }
+        if (lane == 0) {
+            const double dg = bi ? 1.0 : diag;
+            const double id = dg != 0.0 ? 1.0 / dg : 1.0;
+            a.idiag[i] = id;
+            a.idiag32[i] = (float)id;
+            a.b[i] = bi ? 0.0 : a.f[i] * mass;
+        }
+    }
 }
 
 template <class T>
@@ -435,7 +518,7 @@ struct Step3D {
         a.idiag = idiag.p;
         a.idiag32 = idiag32.p;
         a.b = b.p;
-        hipLaunchKernelGGL(p1_tet_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(p1_tet_kernel, dim3(grid_for(n, 4)), dim3(256), 0, s, a);
     }
 
     // Point the network's graph at this step's system (after assembly).
