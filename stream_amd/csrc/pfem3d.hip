@@ -108,7 +108,77 @@ __device__ void tet_cols_serial(uint32_t i, const uint32_t* __restrict__ elem, c
 // it.  The output (sorted tetrahedra, sorted distinct columns, count)
 // equals the serial form's; rows with more than 64 incident tetrahedra
 // take the serial form on one lane.
-constexpr uint32_t kColSlots = 4;
+// Key dedupe of one row with NS 64-key slots per lane (3 m + 1 <= 64 NS).
+template <uint32_t NS>
+__device__ __forceinline__ void tet_cols_row(uint32_t i, uint32_t b0, uint32_t m, uint32_t t, uint32_t rt,
+                                             uint32_t lane, const uint32_t* __restrict__ elem,
+                                             uint32_t* __restrict__ inc, uint32_t* __restrict__ tmp,
+                                             uint32_t* __restrict__ ncol) {
+    // keys: position p = slot * 64 + lane; p < 3 m: the (p % 3)-th vertex
+    // other than i of the tetrahedron on lane p / 3; p = 3 m: i itself
+    const uint32_t nk = 3 * m + 1;
+    uint32_t key[NS];
+    bool first[NS];
+#pragma unroll
+    for (uint32_t sl = 0; sl < NS; ++sl) {
+        const uint32_t p = sl * 64 + lane;
+        const uint32_t j = p / 3, q = p - 3 * j;
+        const uint32_t tj = (uint32_t)__shfl((int)t, (int)(j & 63u), 64);
+        uint32_t kv = ~0u;
+        if (p < 3 * m) {
+            uint32_t c = 0;
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                const uint32_t v = elem[4ull * tj + u];
+                if (v != i) {
+                    if (c == q) kv = v;
+                    ++c;
+                }
+            }
+        } else if (p == 3 * m) {
+            kv = i;
+        }
+        key[sl] = kv;
+        first[sl] = p < nk;
+    }
+    if (lane < m) inc[b0 + rt] = t;  // every lane has read its tetrahedron
+    // pass 1: a key is kept when no earlier position holds the same value
+#pragma unroll
+    for (uint32_t S = 0; S < NS; ++S) {
+        if (S * 64 >= nk) break;
+        const uint32_t lim = nk - S * 64 < 64 ? nk - S * 64 : 64;
+        for (uint32_t L = 0; L < lim; ++L) {
+            const uint32_t kP = (uint32_t)__shfl((int)key[S], (int)L, 64);
+            const uint32_t P = S * 64 + L;
+#pragma unroll
+            for (uint32_t sl = 0; sl < NS; ++sl)
+                if (kP == key[sl] && P < sl * 64 + lane) first[sl] = false;
+        }
+    }
+    // pass 2: a kept key's position = number of kept keys below it
+    uint32_t rank[NS];
+#pragma unroll
+    for (uint32_t sl = 0; sl < NS; ++sl) rank[sl] = 0;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (uint32_t S = 0; S < NS; ++S) {
+        if (S * 64 >= nk) break;
+        cnt += (uint32_t)__popcll(__ballot(first[S]));
+        const uint32_t lim = nk - S * 64 < 64 ? nk - S * 64 : 64;
+        for (uint32_t L = 0; L < lim; ++L) {
+            const uint32_t kP = (uint32_t)__shfl((int)key[S], (int)L, 64);
+            const bool fP = __shfl((int)first[S], (int)L, 64) != 0;
+#pragma unroll
+            for (uint32_t sl = 0; sl < NS; ++sl) rank[sl] += (fP && kP < key[sl]) ? 1u : 0u;
+        }
+    }
+    uint32_t* c = tmp + 3ull * b0 + i;
+#pragma unroll
+    for (uint32_t sl = 0; sl < NS; ++sl)
+        if (first[sl]) c[rank[sl]] = key[sl];
+    if (lane == 0) ncol[i] = cnt;
+}
+
 __global__ __launch_bounds__(256) void tet_cols_kernel(uint32_t n, const uint32_t* __restrict__ elem,
                                                        const uint32_t* __restrict__ toff, uint32_t* __restrict__ inc,
                                                        uint32_t* __restrict__ tmp, uint32_t* __restrict__ ncol) {
@@ -125,69 +195,16 @@ __global__ __launch_bounds__(256) void tet_cols_kernel(uint32_t n, const uint32_
         // rank of this lane's tetrahedron among the row's (ids are distinct)
         uint32_t rt = 0;
         for (uint32_t l = 0; l < m; ++l) rt += (uint32_t)__shfl((int)t, (int)l, 64) < t ? 1u : 0u;
-        // keys: position p = slot * 64 + lane; p < 3 m: the (p % 3)-th vertex
-        // other than i of the tetrahedron on lane p / 3; p = 3 m: i itself
-        const uint32_t nk = 3 * m + 1;
-        uint32_t key[kColSlots];
-        bool first[kColSlots];
-#pragma unroll
-        for (uint32_t sl = 0; sl < kColSlots; ++sl) {
-            const uint32_t p = sl * 64 + lane;
-            const uint32_t j = p / 3, q = p - 3 * j;
-            const uint32_t tj = (uint32_t)__shfl((int)t, (int)(j & 63u), 64);
-            uint32_t kv = ~0u;
-            if (p < 3 * m) {
-                uint32_t c = 0;
-#pragma unroll
-                for (uint32_t u = 0; u < 4; ++u) {
-                    const uint32_t v = elem[4ull * tj + u];
-                    if (v != i) {
-                        if (c == q) kv = v;
-                        ++c;
-                    }
-                }
-            } else if (p == 3 * m) {
-                kv = i;
-            }
-            key[sl] = kv;
-            first[sl] = p < nk;
-        }
-        if (lane < m) inc[b0 + rt] = t;  // every lane has read its tetrahedron
-        // pass 1: a key is kept when no earlier position holds the same value
-#pragma unroll
-        for (uint32_t S = 0; S < kColSlots; ++S) {
-            if (S * 64 >= nk) break;
-            const uint32_t lim = nk - S * 64 < 64 ? nk - S * 64 : 64;
-            for (uint32_t L = 0; L < lim; ++L) {
-                const uint32_t kP = (uint32_t)__shfl((int)key[S], (int)L, 64);
-                const uint32_t P = S * 64 + L;
-#pragma unroll
-                for (uint32_t sl = 0; sl < kColSlots; ++sl)
-                    if (kP == key[sl] && P < sl * 64 + lane) first[sl] = false;
-            }
-        }
-        // pass 2: a kept key's position = number of kept keys below it
-        uint32_t rank[kColSlots];
-#pragma unroll
-        for (uint32_t sl = 0; sl < kColSlots; ++sl) rank[sl] = 0;
-        uint32_t cnt = 0;
-#pragma unroll
-        for (uint32_t S = 0; S < kColSlots; ++S) {
-            if (S * 64 >= nk) break;
-            cnt += (uint32_t)__popcll(__ballot(first[S]));
-            const uint32_t lim = nk - S * 64 < 64 ? nk - S * 64 : 64;
-            for (uint32_t L = 0; L < lim; ++L) {
-                const uint32_t kP = (uint32_t)__shfl((int)key[S], (int)L, 64);
-                const bool fP = __shfl((int)first[S], (int)L, 64) != 0;
-#pragma unroll
-                for (uint32_t sl = 0; sl < kColSlots; ++sl) rank[sl] += (fP && kP < key[sl]) ? 1u : 0u;
-            }
-        }
-        uint32_t* c = tmp + 3ull * b0 + i;
-#pragma unroll
-        for (uint32_t sl = 0; sl < kColSlots; ++sl)
-            if (first[sl]) c[rank[sl]] = key[sl];
-        if (lane == 0) ncol[i] = cnt;
+        // only the slots that hold keys are walked
+        const uint32_t ns = (3 * m + 1 + 63) / 64;
+        if (ns <= 1)
+            tet_cols_row<1>(i, b0, m, t, rt, lane, elem, inc, tmp, ncol);
+        else if (ns == 2)
+            tet_cols_row<2>(i, b0, m, t, rt, lane, elem, inc, tmp, ncol);
+        else if (ns == 3)
+            tet_cols_row<3>(i, b0, m, t, rt, lane, elem, inc, tmp, ncol);
+        else
+            tet_cols_row<4>(i, b0, m, t, rt, lane, elem, inc, tmp, ncol);
     }
 }
 
