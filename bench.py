@@ -374,8 +374,8 @@ def main():
                          "(the byte-counter calibration); prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the solver runs on a single "
+              "device: no row partition, halo exchange or collective reductions are implemented)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
