@@ -370,8 +370,9 @@ def main():
                     help="time the path's components at 1M points (LBVH step, TRBVH, Mesh2D phases, P1, CG "
                          "kernels) and print them as one JSON line instead of the headline")
     ap.add_argument("--profile-kernels", action="store_true",
-                    help="counter runs: one step, then 20 direction-kernel launches and 5 copies of 256 MiB "
-                         "(the byte-counter calibration); prints no JSON")
+                    help="counter runs: one step, then 20 launches of the direction kernel (and, on the L L^T "
+                         "path, of the update kernel and the lower apply) and 5 copies of 256 MiB (the "
+                         "byte-counter calibration); prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the solver runs on a single "
