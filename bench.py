@@ -13,8 +13,8 @@ message-passing network (weights in stream_amd/weights), FP32 SpMV and
 preconditioner, FP64 dots and residual replacement.  The FP64 Jacobi-PCG
 step rate is measured beside it in the same run.
 
-Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build targets
-one GPU.
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: the step runs on one
+GPU.
 """
 from __future__ import annotations
 
