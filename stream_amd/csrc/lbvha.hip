@@ -104,8 +104,8 @@ __global__ __launch_bounds__(256) void agglo_build_kernel(uint32_t n, const uint
             // carries its own running cost (Ct * area of its object at a
             // leaf) and adds Ci * area(parent) + stored cost of the sibling
             // (0 at a leaf).  The reference lets either child arrive second;
-            // this build always takes the right child as the carrier, which
-            // is what a sequential run of the reference's loop produces.
+            // here the right child is always the carrier, which is what a
+            // sequential run of the reference's loop produces.
             const uint32_t lc = side == 0 ? idx : sibling;
             const uint32_t rc = side == 0 ? sibling : idx;
             const float carry = rc < n ? sah_term<S, D>(kSahCt, coherent_load(&data[rc])) : ld_f32(&sah[rc]);
