@@ -66,45 +66,44 @@ template <class S> __global__ void infinity_kernel(const BoxD<S, 2>* gbox, Sph4<
 }
 
 // Candidate search (cpp:310-357): stack DFS, right child popped first,
-// leaf children tested when their parent is visited.
+// leaf children tested when their parent is visited.  The child that the
+// next pop would return is kept in a register and an internal child's node
+// word is loaded with its box; the stack (63 entries + the register = the
+// 64-deep stack) holds node words.  The visit order, the candidate order and
+// the capacity error are those of the plain push/pop loop.
 template <class S> __global__ __launch_bounds__(256) void fill_kernel(AshArgs<S> a) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= a.n) return;
     const Sph4<S> si = a.P[tid];
     const S two_delta = Consts2D<S>::two_delta;
     const S rs = two_delta * si.r;
-    uint32_t stack[64];
+    uint32_t stack[63];
     int ss = 0;
-    stack[ss++] = 0;
+    bool live = a.n >= 2;
+    uint32_t w = live ? a.node[0] : 0u;
     uint32_t nloc = kNInf2D;
-    if (a.n < 2) ss = 0;
     // a valid tree has n-1 internal nodes: the visit cap only guards a
     // malformed topology
-    for (uint32_t visits = 0; ss != 0; ++visits) {
+    for (uint32_t visits = 0; live; ++visits) {
         if (visits >= a.n) {
             raise_flag(a.flags, kErrCapacity);
             break;
         }
-        const uint32_t cur = stack[--ss];
-        const uint32_t w = a.node[cur];
         const uint32_t sep = w & 0x3fffffffu;
         const uint32_t cl = w >> 30;
+        bool hit[2] = {false, false};
+        uint32_t wc[2] = {0u, 0u};
 #pragma unroll
         for (uint32_t ch = 0; ch < 2; ++ch) {
             const uint32_t c = sep + ch;
             if (!(cl & (ch + 1))) {
                 const BoxD<S, 2> bb = a.ibox[c];
+                wc[ch] = a.node[c];
                 const S dmaxx = fmax(si.c[0] - bb.hi[0], (S)0);
                 const S dmaxy = fmax(si.c[1] - bb.hi[1], (S)0);
                 const S dminx = fmax(bb.lo[0] - si.c[0], (S)0);
                 const S dminy = fmax(bb.lo[1] - si.c[1], (S)0);
-                const bool hit = dmaxx * dmaxx + dminx * dminx + dmaxy * dmaxy + dminy * dminy < rs * rs;
-                if (hit) {
-                    if (ss < 64)
-                        stack[ss++] = c;
-                    else
-                        raise_flag(a.flags, kErrCapacity);
-                }
+                hit[ch] = dmaxx * dmaxx + dminx * dminx + dmaxy * dmaxy + dminy * dminy < rs * rs;
             } else if (tid != c) {
                 const Sph4<S> sj = a.P[c];
                 const S ma = fmin(si.r, sj.r);
@@ -116,6 +115,22 @@ template <class S> __global__ __launch_bounds__(256) void fill_kernel(AshArgs<S>
                     ++nloc;
                 }
             }
+        }
+        if (hit[0] && hit[1]) {
+            if (ss < 63) {
+                stack[ss++] = wc[0];
+                w = wc[1];
+            } else {
+                // a full stack drops the second child and visits the first
+                raise_flag(a.flags, kErrCapacity);
+                w = wc[0];
+            }
+        } else if (hit[0] || hit[1]) {
+            w = hit[1] ? wc[1] : wc[0];
+        } else if (ss != 0) {
+            w = stack[--ss];
+        } else {
+            live = false;
         }
         if (nloc > kMaxFill2D) break;
     }

@@ -65,47 +65,50 @@ template <class S> __global__ void infinity3_kernel(const BoxD<S, 3>* gbox, Sph4
 // tested when their parent is visited; the first 124 candidates are kept
 // (the reference clamps its list to 128 entries including the 4 infinity
 // nodes and warns), queue[1] counts clamped nodes.
+//
+// The DFS pushes the hit children of a node in order and pops the last one
+// pushed.  Here that last child is kept in a register instead of going
+// through the stack, and an internal child's node word is loaded with its
+// box, so descending costs one round of loads; the stack (private memory,
+// 63 entries + the register = the 64-deep stack) holds node words, so a
+// pop needs no node load.  The visit order, and so the candidate order, is
+// that of the plain push/pop loop, and so is the capacity error.
 template <class S> __global__ __launch_bounds__(256) void fill3_kernel(Ash3Args<S> a) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= a.n) return;
     const Sph4<S> si = a.P[tid];
     const S two_delta = Consts3D<S>::two_delta;
     const S rs = two_delta * si.r;
-    uint32_t stack[64];
+    uint32_t stack[63];
     int ss = 0;
-    stack[ss++] = 0;
+    bool live = a.n >= 2;
+    uint32_t w = live ? a.node[0] : 0u;
     uint32_t nloc = 0;
     bool over = false;
-    if (a.n < 2) ss = 0;
-    for (uint32_t visits = 0; ss != 0 && !over; ++visits) {
+    for (uint32_t visits = 0; live && !over; ++visits) {
         if (visits >= a.n) {
             raise_flag(a.flags, kErrCapacity);
             break;
         }
-        const uint32_t cur = stack[--ss];
-        const uint32_t w = a.node[cur];
         const uint32_t sep = w & 0x3fffffffu;
         const uint32_t cl = w >> 30;
+        bool hit[2] = {false, false};
+        uint32_t wc[2] = {0u, 0u};
 #pragma unroll
         for (uint32_t ch = 0; ch < 2; ++ch) {
             const uint32_t c = sep + ch;
             if (!(cl & (ch + 1))) {
                 const BoxD<S, 3> bb = a.ibox[c];
+                wc[ch] = a.node[c];
                 const S dmaxx = fmax(si.c[0] - bb.hi[0], (S)0);
                 const S dmaxy = fmax(si.c[1] - bb.hi[1], (S)0);
                 const S dmaxz = fmax(si.c[2] - bb.hi[2], (S)0);
                 const S dminx = fmax(bb.lo[0] - si.c[0], (S)0);
                 const S dminy = fmax(bb.lo[1] - si.c[1], (S)0);
                 const S dminz = fmax(bb.lo[2] - si.c[2], (S)0);
-                const bool hit =
+                hit[ch] =
                     dmaxx * dmaxx + dminx * dminx + dmaxy * dmaxy + dminy * dminy + dmaxz * dmaxz + dminz * dminz <
                     rs * rs;
-                if (hit) {
-                    if (ss < 64)
-                        stack[ss++] = c;
-                    else
-                        raise_flag(a.flags, kErrCapacity);
-                }
             } else if (tid != c && !over) {
                 const Sph4<S> sj = a.P[c];
                 const S ma = fmin(si.r, sj.r);
@@ -118,6 +121,22 @@ template <class S> __global__ __launch_bounds__(256) void fill3_kernel(Ash3Args<
                         over = true;
                 }
             }
+        }
+        if (hit[0] && hit[1]) {
+            if (ss < 63) {
+                stack[ss++] = wc[0];
+                w = wc[1];
+            } else {
+                // a full stack drops the second child and visits the first
+                raise_flag(a.flags, kErrCapacity);
+                w = wc[0];
+            }
+        } else if (hit[0] || hit[1]) {
+            w = hit[1] ? wc[1] : wc[0];
+        } else if (ss != 0) {
+            w = stack[--ss];
+        } else {
+            live = false;
         }
     }
     if (over) atomicAdd(&a.queue[1], 1u);
