@@ -809,12 +809,12 @@ __global__ __launch_bounds__(256) void transpose_weights_kernel(const uint16_t* 
     wt[t] = wb[src + r * cols + c];
 }
 
-// L = D^-1/2 Ghat^T into the SELL factor: up slot t of row k holds
-// L_jk = sd_j Ghat_kj for the t-th upper entry j; diag L_kk = sd_k Ghat_kk.
+// L = D^-1/2 Ghat^T into the SELL factor: the up slot of CSR entry (k, j),
+// j > k, holds L_jk = sd_j Ghat_kj; diag L_kk = sd_k Ghat_kk.
 __global__ __launch_bounds__(256) void pack_kernel(uint32_t n, const uint32_t* __restrict__ row,
                                                    const uint32_t* __restrict__ col, const double* __restrict__ sd,
                                                    const uint32_t* __restrict__ dpos, const float* __restrict__ gval,
-                                                   const uint32_t* __restrict__ up_off, float* __restrict__ up_val,
+                                                   const uint32_t* __restrict__ slot, float* __restrict__ up_val,
                                                    float* __restrict__ diag, uint32_t* __restrict__ flags) {
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
@@ -824,15 +824,12 @@ __global__ __launch_bounds__(256) void pack_kernel(uint32_t n, const uint32_t* _
     const float lkk = (float)(sd[k] * (double)gd);
     bad |= !(lkk > 0.f) || !isfinite(lkk);
     diag[k] = lkk;
-    const uint32_t base = up_off[k / kSlice] + k % kSlice;
-    uint32_t t = 0;
     for (uint32_t e = row[k]; e < row[k + 1]; ++e) {
         const uint32_t j = col[e];
         if (j <= k) continue;
         const float v = (float)(sd[j] * (double)gval[e]);
         bad |= !isfinite(v);
-        up_val[base + t * kSlice] = v;
-        ++t;
+        up_val[slot[e]] = v;
     }
     if (bad) atomicMax(&flags[kFlagLearned], 1u);
 }
@@ -1042,7 +1039,7 @@ void MpnnNet::adam(float lr, float b1, float b2, float eps, hipStream_t s) {
 void MpnnNet::pack(const MpnnGraph& g, LowerFactor<float>& L, hipStream_t s) {
     if (!g.n) return;
     hipLaunchKernelGGL(pack_kernel, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, g.row, g.col, g.sd.p, g.dpos.p,
-                       gval.p, L.up.off.p, L.up.val.p, L.diag.p, ctx().d_flags);
+                       gval.p, L.slot.p, L.up.val.p, L.diag.p, ctx().d_flags);
     mirror_lower(L, s);
 }
 

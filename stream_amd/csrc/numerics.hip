@@ -371,11 +371,12 @@ __device__ __forceinline__ bool in_part(uint32_t c, uint32_t i, int part) {
     return part == kSellOffDiag ? c != i : (part == kSellLower ? c < i : c > i);
 }
 
-// One wave per slice: row lengths and the slice width (max length).
+// One wave per slice: row lengths, the slice width (max length) and the
+// slice's entry count (its number of slots).
 __global__ __launch_bounds__(256) void sell_len_kernel(uint32_t n, uint32_t nslices, const uint32_t* __restrict__ row,
                                                        const uint32_t* __restrict__ col, int part,
                                                        uint16_t* __restrict__ len, uint16_t* __restrict__ width,
-                                                       uint32_t* __restrict__ w64) {
+                                                       uint32_t* __restrict__ count) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t s = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (s >= nslices) return;
@@ -384,56 +385,78 @@ __global__ __launch_bounds__(256) void sell_len_kernel(uint32_t n, uint32_t nsli
     if (i < n)
         for (uint32_t e = row[i]; e < row[i + 1]; ++e) l += in_part(col[e], i, part) ? 1u : 0u;
     if (i < n) len[i] = (uint16_t)l;
-    uint32_t m = l;
+    uint32_t m = l, tot = l;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+    for (int o = 32; o > 0; o >>= 1) {
+        m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+        tot += (uint32_t)__shfl_xor((int)tot, o, 64);
+    }
     if (lane == 0) {
         width[s] = (uint16_t)m;
-        w64[s] = m * kSlice;
+        count[s] = tot;
     }
 }
 
+// Slot of this lane's entry at a step whose participating lanes are the set
+// bits of m (the step starts at slot seg).
+__device__ __forceinline__ uint32_t step_slot(uint32_t seg, uint64_t m) {
+    return seg + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// One wave per slice (blocks of 256 rows): the k-th entry of the part of
+// every row longer than k goes to step k, in lane order.  The row sum and
+// the diagonal are accumulated over the CSR row in order.
 template <class V>
-__global__ __launch_bounds__(256) void sell_fill_kernel(uint32_t n, const uint32_t* __restrict__ row,
+__global__ __launch_bounds__(256) void sell_fill_kernel(uint32_t n, uint32_t nslices, const uint32_t* __restrict__ row,
                                                         const uint32_t* __restrict__ col,
                                                         const double* __restrict__ val64,
                                                         const float* __restrict__ val32, int form, int part,
                                                         const uint32_t* __restrict__ off,
                                                         const uint16_t* __restrict__ width,
-                                                        uint32_t* __restrict__ scol, int16_t* __restrict__ scol16,
-                                                        V* __restrict__ sval, V* __restrict__ diag) {
+                                                        const uint16_t* __restrict__ len, uint32_t* __restrict__ scol,
+                                                        int16_t* __restrict__ scol16, V* __restrict__ sval,
+                                                        V* __restrict__ diag, uint32_t* __restrict__ entry_slot) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t s = i / kSlice, lane = i % kSlice;
-    const uint32_t base = off[s] + lane;
-    uint32_t k = 0;
+    const uint32_t s = i / kSlice;
+    if (s >= nslices) return;  // whole wavefronts past the last slice
+    const bool live = i < n;
     double rowsum = 0.0, dv = 0.0;
-    for (uint32_t e = row[i]; e < row[i + 1]; ++e) {
-        const uint32_t c = col[e];
-        const double v = val64 ? val64[e] : (double)val32[e];
-        rowsum += v;
-        if (c == i) dv = v;
-        if (in_part(c, i, part)) {
-            const int32_t o = (int32_t)c - (int32_t)i;
-            scol[base + k * kSlice] = c;
-            scol16[base + k * kSlice] = (o >= -32767 && o <= 32767) ? (int16_t)o : kColEscape;
-            sval[base + k * kSlice] = (V)v;
-            ++k;
+    uint32_t e = 0;
+    if (live) {
+        e = row[i];
+        for (uint32_t q = row[i]; q < row[i + 1]; ++q) {
+            const double v = val64 ? val64[q] : (double)val32[q];
+            rowsum += v;
+            if (col[q] == i) dv = v;
         }
     }
-    for (; k < width[s]; ++k) {
-        scol[base + k * kSlice] = i;
-        scol16[base + k * kSlice] = 0;
-        sval[base + k * kSlice] = (V)0;
+    const uint32_t L = live ? (uint32_t)len[i] : 0u;
+    const uint32_t W = width[s];
+    uint32_t seg = off[s];
+    for (uint32_t k = 0; k < W; ++k) {
+        const bool ok = k < L;
+        const uint64_t m = __ballot(ok);
+        const uint32_t slot = step_slot(seg, m);
+        seg += (uint32_t)__popcll(m);
+        if (ok) {
+            while (!in_part(col[e], i, part)) ++e;
+            const uint32_t c = col[e];
+            const int32_t o = (int32_t)c - (int32_t)i;
+            scol[slot] = c;
+            scol16[slot] = (o >= -32767 && o <= 32767) ? (int16_t)o : kColEscape;
+            sval[slot] = (V)(val64 ? val64[e] : (double)val32[e]);
+            if (entry_slot) entry_slot[e] = slot;
+            ++e;
+        }
     }
-    diag[i] = (V)(form == kSellRowSum ? rowsum : dv);
+    if (live) diag[i] = (V)(form == kSellRowSum ? rowsum : dv);
 }
 
 }  // namespace
 
 template <class V>
 void build_sell(uint32_t n, const uint32_t* row, const uint32_t* col, const double* val64, const float* val32,
-                SellForm form, SellPart part, Sell<V>& S, hipStream_t s) {
+                SellForm form, SellPart part, Sell<V>& S, hipStream_t s, uint32_t* entry_slot) {
     S.n = n;
     S.nslices = (n + kSlice - 1) / kSlice;
     const uint32_t ns = S.nslices ? S.nslices : 1;
@@ -452,16 +475,19 @@ void build_sell(uint32_t n, const uint32_t* row, const uint32_t* col, const doub
     exclusive_scan_u32(S.tmp.p, S.off.p, S.nslices, S.scan_ws.p, s);
     S.entries = read_scalar(S.off.p + S.nslices);
     check_error_flag();  // a scan look-back that ran out of spins
-    S.col.resize(S.entries ? S.entries : 1);
-    S.col16.resize(S.entries ? S.entries : 1);
-    S.val.resize(S.entries ? S.entries : 1);
-    hipLaunchKernelGGL((sell_fill_kernel<V>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, row, col, val64, val32,
-                       (int)form, (int)part, S.off.p, S.width.p, S.col.p, S.col16.p, S.val.p, S.diag.p);
+    // kSlice spare slots: the slot a lane computes for a step it sits out
+    // can be one past the last entry
+    S.col.resize(S.entries + kSlice);
+    S.col16.resize(S.entries + kSlice);
+    S.val.resize(S.entries + kSlice);
+    hipLaunchKernelGGL((sell_fill_kernel<V>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, S.nslices, row, col, val64,
+                       val32, (int)form, (int)part, S.off.p, S.width.p, S.len.p, S.col.p, S.col16.p, S.val.p, S.diag.p,
+                       entry_slot);
 }
 template void build_sell<float>(uint32_t, const uint32_t*, const uint32_t*, const double*, const float*, SellForm,
-                                SellPart, Sell<float>&, hipStream_t);
+                                SellPart, Sell<float>&, hipStream_t, uint32_t*);
 template void build_sell<double>(uint32_t, const uint32_t*, const uint32_t*, const double*, const float*, SellForm,
-                                 SellPart, Sell<double>&, hipStream_t);
+                                 SellPart, Sell<double>&, hipStream_t, uint32_t*);
 
 namespace {
 
@@ -482,24 +508,31 @@ template <class V> SellArgs<V> sell_args(const Sell<V>& S) {
     return {S.n, S.nslices, S.off.p, S.width.p, S.len.p, S.col.p, S.col16.p, S.val.p, S.diag.p, tuning().xcd_map};
 }
 
-// Entries k .. k + CH - 1 of a row (slot base + e * kSlice, row length L,
-// row ii): values and columns, the padding as column ii with value 0.  The
-// 16-bit offsets and the values are all loaded before any escaped column
-// (one entry in a few hundred on Morton-ordered meshes) is fetched from the
-// 32-bit copy.
+// Entries k .. k + CH - 1 of a row (row length L, row ii) from the steps
+// starting at slot seg (advanced past them): values and columns, past the
+// row's end column ii with value 0.  Every lane of the wavefront takes part
+// (the ballots).  The 16-bit offsets and the values are all loaded before
+// any escaped column (one entry in a few hundred on Morton-ordered meshes)
+// is fetched from the 32-bit copy.
 template <class V, int CH>
-__device__ __forceinline__ void sell_load(const SellArgs<V>& M, uint32_t base, uint32_t k, uint32_t L, uint32_t ii,
+__device__ __forceinline__ void sell_load(const SellArgs<V>& M, uint32_t& seg, uint32_t k, uint32_t L, uint32_t ii,
                                           uint32_t (&c)[CH], V (&a)[CH]) {
-    int16_t o[CH];
+    // offsets widened to 32 bits as they load: two 16-bit values packed in
+    // one register would need each load to land before the next one issues
+    int32_t o[CH];
+    uint32_t e[CH];
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
         const bool ok = k + u < L;
-        o[u] = ok ? M.col16[base + (k + u) * kSlice] : (int16_t)0;
-        a[u] = ok ? M.val[base + (k + u) * kSlice] : (V)0;
+        const uint64_t m = __ballot(ok);
+        e[u] = step_slot(seg, m);
+        seg += (uint32_t)__popcll(m);
+        o[u] = ok ? (int32_t)M.col16[e[u]] : 0;
+        a[u] = ok ? M.val[e[u]] : (V)0;
     }
 #pragma unroll
     for (int u = 0; u < CH; ++u)
-        c[u] = o[u] == kColEscape ? M.col[base + (k + u) * kSlice] : (uint32_t)((int32_t)ii + (int32_t)o[u]);
+        c[u] = o[u] == (int32_t)kColEscape ? M.col[e[u]] : (uint32_t)((int32_t)ii + o[u]);
 }
 
 // Slices of one wavefront: [s, e) with stride step.
@@ -528,7 +561,8 @@ __device__ __forceinline__ SliceSpan slice_span(uint32_t nslices, uint32_t map) 
 }
 
 // R slices of one wavefront processed together (s, s + step, ...): this
-// lane's row in each, its length, SELL slot base, and the widest slice.
+// lane's row in each, its length, the slice's first slot, and the widest
+// slice.
 // R slices at once put R rows' loads in flight before any of their
 // gathers; the solvers instantiate R = 1, since R = 2 measured slower on
 // the 1M systems (more registers, fewer resident waves, no shorter chain).
@@ -550,24 +584,27 @@ __device__ __forceinline__ SliceRows<R> slice_rows(const SellArgs<V>& M, uint32_
         q.valid[r] = has && q.i[r] < M.n;
         q.ii[r] = q.valid[r] ? q.i[r] : 0u;
         q.L[r] = q.valid[r] ? (uint32_t)M.len[q.i[r]] : 0u;
-        q.base[r] = has ? M.off[sr] + lane : 0u;
+        q.base[r] = has ? M.off[sr] : 0u;
         q.W = max(q.W, has ? (uint32_t)M.width[sr] : 0u);
     }
     return q;
 }
 
 // acc[r] = term(r, acc[r], a_k, fetch(c_k)) over the entries k of this
-// lane's R rows in ascending order (the padding past a row's length as
+// lane's R rows in ascending order (steps past a row's length read as
 // column ii, value 0).  Per step the CH entries of all R rows are loaded,
 // then all their gathers are issued, then the terms are added.
 template <class V, int R, int CH, class G, class Fetch, class Term>
 __device__ __forceinline__ void sell_rows_product(const SellArgs<V>& M, const SliceRows<R>& q, V (&acc)[R],
                                                   Fetch fetch, Term term) {
+    uint32_t seg[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) seg[r] = q.base[r];
     for (uint32_t k = 0; k < q.W; k += CH) {
         uint32_t c[R][CH];
         V a[R][CH];
 #pragma unroll
-        for (int r = 0; r < R; ++r) sell_load<V, CH>(M, q.base[r], k, q.L[r], q.ii[r], c[r], a[r]);
+        for (int r = 0; r < R; ++r) sell_load<V, CH>(M, seg[r], k, q.L[r], q.ii[r], c[r], a[r]);
         G g[R][CH];
 #pragma unroll
         for (int r = 0; r < R; ++r)

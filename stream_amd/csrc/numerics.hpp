@@ -4,13 +4,17 @@
 // linear_system.cpp:27-43, PrecJacobi preconditioner_jacobi.cpp:29-60,
 // ConjugateGradient conjugate_gradient.cpp:44-208).
 //
-// Matrix layout for the iterations: SELL-64.  The off-diagonal entries of
-// 64 consecutive rows (one wavefront, one row per lane) form a slice stored
-// column-major: entry k of row r (slice s = r/64, lane l = r%64) lives at
-// off[s] + 64k + l, so the k-th load of a wave is one contiguous 256 B
-// (FP32) or 512 B (FP64) access.  Slices are as wide as their longest row;
-// per-row lengths predicate the loads, so padding costs memory but no
-// traffic.  The diagonal is a separate array.  CSR stays the interchange
+// Matrix layout for the iterations: packed SELL-64.  The off-diagonal
+// entries of 64 consecutive rows (one wavefront, one row per lane) form a
+// slice stored step by step: step k holds the k-th entries of the rows
+// longer than k, in lane order and without gaps, so the k-th load of a
+// wave reads one contiguous run and no slot is padding (a slice of rows of
+// 3 and 6 entries costs 3 + 6 slots, not 2 x 6).  Entry k of row r (slice
+// s = r/64, lane l = r%64) lives at off[s] + sum over the slice's rows of
+// min(len, k) + (number of lanes below l whose row is longer than k); a
+// wave computes that from a ballot of (len > k) and a lane-mask count, so
+// the row loops load no index table.  The slice width (longest row) bounds
+// the steps.  The diagonal is a separate array.  CSR stays the interchange
 // format (reference DeviceCSR, assembly output, FP64 true residuals).
 //
 // CG keeps every scalar on the device.  One iteration is two kernels:
@@ -46,8 +50,8 @@ constexpr int16_t kColEscape = -32768;
 template <class V> struct Sell {
     uint32_t n = 0;
     uint32_t nslices = 0;
-    uint32_t entries = 0;    // allocated slots (with padding)
-    DBuf<uint32_t> off;      // [nslices + 1]
+    uint32_t entries = 0;    // slots (= off-diagonal entries; the arrays hold kSlice more)
+    DBuf<uint32_t> off;      // [nslices + 1] first slot of each slice
     DBuf<uint16_t> width;    // [nslices]
     DBuf<uint16_t> len;      // [n]
     DBuf<uint32_t> col;      // [entries]
@@ -67,10 +71,12 @@ enum SellPart : int32_t { kSellOffDiag = 0, kSellLower = 1, kSellUpper = 2 };
 
 // Build a SELL copy of a CSR matrix (values from val64 or val32, one of them
 // non-null).  part selects all off-diagonals, the strict lower or the strict
-// upper triangle.  Synchronises once (allocation size).
+// upper triangle.  entry_slot (optional, [nnz]): entry_slot[e] = the slot of
+// CSR entry e, written for the entries of the part only.  Synchronises once
+// (allocation size).
 template <class V>
 void build_sell(uint32_t n, const uint32_t* row, const uint32_t* col, const double* val64, const float* val32,
-                SellForm form, SellPart part, Sell<V>& S, hipStream_t s);
+                SellForm form, SellPart part, Sell<V>& S, hipStream_t s, uint32_t* entry_slot = nullptr);
 
 // Device-resident CG scalars (one 128-byte record).
 struct CgScalars {
@@ -112,6 +118,11 @@ template <class V> struct LowerFactor {
     uint32_t n = 0;
     Sell<V> lo, up;
     DBuf<V> diag;
+    // the CSR pattern the halves were built from (borrowed) and, per CSR
+    // entry, its slot in lo (column < row) or up (column > row)
+    const uint32_t* row = nullptr;
+    const uint32_t* col = nullptr;
+    DBuf<uint32_t> slot;
 };
 
 struct SolveStats {

@@ -15,20 +15,24 @@ constexpr uint32_t kFsaiRegs = 8;  // dense solve held in registers up to 8 x 8
 template <class V>
 __global__ __launch_bounds__(256) void fsai_kernel(uint32_t n, const uint32_t* __restrict__ row,
                                                    const uint32_t* __restrict__ col, const double* __restrict__ val,
-                                                   const uint32_t* __restrict__ up_off,
                                                    const uint16_t* __restrict__ up_len,
-                                                   const uint32_t* __restrict__ up_col, V* __restrict__ up_val,
+                                                   const uint32_t* __restrict__ slot, V* __restrict__ up_val,
                                                    V* __restrict__ diag) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t s = i / kSlice, lane = i % kSlice;
-    const uint32_t base = up_off[s] + lane;
     const uint32_t lu = up_len[i];
     const uint32_t m = lu + 1 < kFsaiRegs ? lu + 1 : kFsaiRegs;
+    // P: i, then the first upper entries of row i in CSR order (the order of
+    // the factor's upper slots)
     uint32_t P[kFsaiRegs];
     P[0] = i;
 #pragma unroll
-    for (uint32_t k = 1; k < kFsaiRegs; ++k) P[k] = k < m ? up_col[base + (k - 1) * kSlice] : 0xffffffffu;
+    for (uint32_t k = 1; k < kFsaiRegs; ++k) P[k] = 0xffffffffu;
+    {
+        uint32_t k = 1;
+        for (uint32_t e = row[i]; e < row[i + 1] && k < m; ++e)
+            if (col[e] > i) P[k++] = col[e];
+    }
     // M[b][a] (a <= b) packed lower triangle, row-major
     double M[kFsaiRegs * (kFsaiRegs + 1) / 2];
 #pragma unroll
@@ -91,30 +95,30 @@ __global__ __launch_bounds__(256) void fsai_kernel(uint32_t n, const uint32_t* _
     }
     const double sc = 1.0 / sqrt(g[0]);
     diag[i] = (V)(g[0] * sc);
-    for (uint32_t k = 1; k <= lu; ++k) up_val[base + (k - 1) * kSlice] = (V)(k < m ? g[k] * sc : 0.0);
+    uint32_t k = 1;
+    for (uint32_t e = row[i]; e < row[i + 1]; ++e)
+        if (col[e] > i) {
+            up_val[slot[e]] = (V)(k < m ? g[k] * sc : 0.0);
+            ++k;
+        }
 }
 
-// lo(i, j) = up(j, i): each row j writes its strict-upper values into the
-// matching strict-lower slot of row i.
+// lo(i, j) = up(j, i): row i reads, for each strict-lower entry (i, j),
+// the value of entry (j, i) of row j (slots from the CSR entry map; a lower
+// entry whose transpose is not in the pattern keeps its value).
 template <class V>
-__global__ __launch_bounds__(256) void mirror_kernel(uint32_t n, const uint32_t* __restrict__ up_off,
-                                                     const uint16_t* __restrict__ up_len,
-                                                     const uint32_t* __restrict__ up_col,
-                                                     const V* __restrict__ up_val, const uint32_t* __restrict__ lo_off,
-                                                     const uint16_t* __restrict__ lo_len,
-                                                     const uint32_t* __restrict__ lo_col, V* __restrict__ lo_val) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const uint32_t ub = up_off[j / kSlice] + j % kSlice;
-    const uint32_t lu = up_len[j];
-    for (uint32_t k = 0; k < lu; ++k) {
-        const uint32_t i = up_col[ub + k * kSlice];
-        const V v = up_val[ub + k * kSlice];
-        const uint32_t lb = lo_off[i / kSlice] + i % kSlice;
-        const uint32_t ll = lo_len[i];
-        for (uint32_t t = 0; t < ll; ++t)
-            if (lo_col[lb + t * kSlice] == j) {
-                lo_val[lb + t * kSlice] = v;
+__global__ __launch_bounds__(256) void mirror_kernel(uint32_t n, const uint32_t* __restrict__ row,
+                                                     const uint32_t* __restrict__ col,
+                                                     const uint32_t* __restrict__ slot,
+                                                     const V* __restrict__ up_val, V* __restrict__ lo_val) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (uint32_t e = row[i]; e < row[i + 1]; ++e) {
+        const uint32_t j = col[e];
+        if (j >= i) continue;
+        for (uint32_t f = row[j]; f < row[j + 1]; ++f)
+            if (col[f] == i) {
+                lo_val[slot[e]] = up_val[slot[f]];
                 break;
             }
     }
@@ -124,22 +128,25 @@ __global__ __launch_bounds__(256) void mirror_kernel(uint32_t n, const uint32_t*
 
 template <class V> void factor_structure(const DevCsr<double>& A, LowerFactor<V>& L, hipStream_t s) {
     L.n = A.n;
-    build_sell<V>(A.n, A.row.p, A.col.p, A.val.p, nullptr, kSellStandard, kSellUpper, L.up, s);
-    build_sell<V>(A.n, A.row.p, A.col.p, A.val.p, nullptr, kSellStandard, kSellLower, L.lo, s);
+    L.row = A.row.p;
+    L.col = A.col.p;
+    L.slot.resize(A.nnz ? A.nnz : 1);
+    build_sell<V>(A.n, A.row.p, A.col.p, A.val.p, nullptr, kSellStandard, kSellUpper, L.up, s, L.slot.p);
+    build_sell<V>(A.n, A.row.p, A.col.p, A.val.p, nullptr, kSellStandard, kSellLower, L.lo, s, L.slot.p);
     L.diag.resize(A.n ? A.n : 1);
 }
 
 template <class V> void mirror_lower(LowerFactor<V>& L, hipStream_t s) {
     if (!L.n) return;
-    hipLaunchKernelGGL((mirror_kernel<V>), dim3(grid_for(L.n, 256)), dim3(256), 0, s, L.n, L.up.off.p, L.up.len.p,
-                       L.up.col.p, L.up.val.p, L.lo.off.p, L.lo.len.p, L.lo.col.p, L.lo.val.p);
+    hipLaunchKernelGGL((mirror_kernel<V>), dim3(grid_for(L.n, 256)), dim3(256), 0, s, L.n, L.row, L.col, L.slot.p,
+                       L.up.val.p, L.lo.val.p);
 }
 
 template <class V> void fsai_factor(const DevCsr<double>& A, LowerFactor<V>& L, hipStream_t s) {
     factor_structure(A, L, s);
     if (!A.n) return;
     hipLaunchKernelGGL((fsai_kernel<V>), dim3(grid_for(A.n, 256)), dim3(256), 0, s, A.n, A.row.p, A.col.p, A.val.p,
-                       L.up.off.p, L.up.len.p, L.up.col.p, L.up.val.p, L.diag.p);
+                       L.up.len.p, L.slot.p, L.up.val.p, L.diag.p);
     mirror_lower(L, s);
 }
 

@@ -11,8 +11,9 @@
 //     slots.
 // Storage (LowerFactor): up = strict upper entries of row i (L_ji, j > i),
 // lo = strict lower entries of row i (L_ij, j < i), diag = L_ii; both as
-// SELL-64 copies of A's pattern, so the two half products of the apply are
-// coalesced like the SpMV.
+// packed SELL-64 copies of A's pattern, so the two half products of the
+// apply are coalesced like the SpMV.  Producers write a value through
+// LowerFactor::slot (CSR entry -> slot of its half).
 #pragma once
 #include "numerics.hpp"
 
@@ -20,7 +21,8 @@ namespace stream_amd {
 
 constexpr uint32_t kFsaiMax = 16;
 
-// Build the lo/up SELL structures of L on A's pattern (values zeroed).
+// Build the lo/up SELL structures of L on A's pattern (values copied from A
+// until a producer overwrites them) and the CSR entry -> slot map.
 template <class V> void factor_structure(const DevCsr<double>& A, LowerFactor<V>& L, hipStream_t s);
 
 // Fill L with the classical FSAI factor of A (FP64 solves, stored in V).
