@@ -484,11 +484,15 @@ __device__ bool local_delaunay3(const Ash3Args<S>& a, uint32_t tid, uint8_t* cyc
     return true;
 }
 
-template <class S> __global__ __launch_bounds__(64) void delaunay3_kernel(Ash3Args<S> a) {
-    __shared__ uint8_t cyc[kCycle3D * 64];
-    const uint32_t tid = blockIdx.x * 64 + threadIdx.x;
+// Four wavefronts per block (one LDS cycle buffer each), one node per lane.
+// Lanes keep consecutive (Morton-ordered) nodes: neighbouring nodes share
+// most candidates, so their coordinate loads hit the same cache lines.
+constexpr uint32_t kDelaunay3Block = 256;
+template <class S> __global__ __launch_bounds__(kDelaunay3Block) void delaunay3_kernel(Ash3Args<S> a) {
+    __shared__ uint8_t cyc[kDelaunay3Block / 64][kCycle3D * 64];
+    const uint32_t tid = blockIdx.x * kDelaunay3Block + threadIdx.x;
     if (tid >= a.n) return;
-    if (!local_delaunay3<S, false>(a, tid, cyc, nullptr)) {
+    if (!local_delaunay3<S, false>(a, tid, cyc[threadIdx.x >> 6], nullptr)) {
         const uint32_t k = atomicAdd(&a.queue[0], 1u);
         a.queue[2 + k] = tid;
     }
@@ -606,7 +610,8 @@ template <class S> void Ash3D<S>::compute(hipStream_t s) {
     fill<uint32_t>(queue.p, 2, 0u, s);
     hipLaunchKernelGGL((fill3_kernel<S>), dim3(grid_for(n, 256)), dim3(256), 0, s, a);
     section_mark(s, kMarkFill);
-    hipLaunchKernelGGL((delaunay3_kernel<S>), dim3((n + 63) / 64), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((delaunay3_kernel<S>), dim3((n + kDelaunay3Block - 1) / kDelaunay3Block),
+                       dim3(kDelaunay3Block), 0, s, a);
     section_mark(s, kMarkDelaunayMain);
     hipLaunchKernelGGL((delaunay3_exact_kernel<S>), dim3(kExactWaves3D), dim3(64), 0, s, a);
     section_mark(s, kMarkDelaunay);
