@@ -404,8 +404,14 @@ __device__ __forceinline__ uint32_t step_slot(uint32_t seg, uint64_t m) {
 }
 
 // One wave per slice (blocks of 256 rows): the k-th entry of the part of
-// every row longer than k goes to step k, in lane order.  The row sum and
-// the diagonal are accumulated over the CSR row in order.
+// every row longer than k goes to step k, in lane order.  The wave first
+// records, for each step, the ballot of the rows that reach it and the
+// step's first slot (in LDS, up to kFillSteps steps); then every lane walks
+// its CSR row in order (independent loads) and places its k-th part entry
+// at first slot + the number of lower lanes in the ballot.  Steps beyond
+// kFillSteps (rows that long are rare) count the slice's lengths directly.
+// The row sum and the diagonal are accumulated over the row in order.
+constexpr uint32_t kFillSteps = 256;
 template <class V>
 __global__ __launch_bounds__(256) void sell_fill_kernel(uint32_t n, uint32_t nslices, const uint32_t* __restrict__ row,
                                                         const uint32_t* __restrict__ col,
@@ -416,40 +422,57 @@ __global__ __launch_bounds__(256) void sell_fill_kernel(uint32_t n, uint32_t nsl
                                                         const uint16_t* __restrict__ len, uint32_t* __restrict__ scol,
                                                         int16_t* __restrict__ scol16, V* __restrict__ sval,
                                                         V* __restrict__ diag, uint32_t* __restrict__ entry_slot) {
+    __shared__ uint64_t smask[4][kFillSteps];
+    __shared__ uint32_t sfirst[4][kFillSteps];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t s = i / kSlice;
-    if (s >= nslices) return;  // whole wavefronts past the last slice
+    const uint32_t s = i / kSlice, lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const bool has = s < nslices;  // wavefronts past the last slice only meet the barrier
     const bool live = i < n;
-    double rowsum = 0.0, dv = 0.0;
-    uint32_t e = 0;
-    if (live) {
-        e = row[i];
-        for (uint32_t q = row[i]; q < row[i + 1]; ++q) {
-            const double v = val64 ? val64[q] : (double)val32[q];
-            rowsum += v;
-            if (col[q] == i) dv = v;
-        }
-    }
     const uint32_t L = live ? (uint32_t)len[i] : 0u;
-    const uint32_t W = width[s];
-    uint32_t seg = off[s];
-    for (uint32_t k = 0; k < W; ++k) {
-        const bool ok = k < L;
-        const uint64_t m = __ballot(ok);
-        const uint32_t slot = step_slot(seg, m);
-        seg += (uint32_t)__popcll(m);
-        if (ok) {
-            while (!in_part(col[e], i, part)) ++e;
-            const uint32_t c = col[e];
-            const int32_t o = (int32_t)c - (int32_t)i;
-            scol[slot] = c;
-            scol16[slot] = (o >= -32767 && o <= 32767) ? (int16_t)o : kColEscape;
-            sval[slot] = (V)(val64 ? val64[e] : (double)val32[e]);
-            if (entry_slot) entry_slot[e] = slot;
-            ++e;
+    const uint32_t W = has ? (uint32_t)width[s] : 0u;
+    const uint32_t base = has ? off[s] : 0u;
+    {
+        uint32_t seg = base;
+        const uint32_t ws = W < kFillSteps ? W : kFillSteps;
+        for (uint32_t k = 0; k < ws; ++k) {
+            const uint64_t m = __ballot(k < L);
+            if (lane == 0) {
+                smask[wv][k] = m;
+                sfirst[wv][k] = seg;
+            }
+            seg += (uint32_t)__popcll(m);
         }
     }
-    if (live) diag[i] = (V)(form == kSellRowSum ? rowsum : dv);
+    __syncthreads();
+    if (!live) return;
+    const uint64_t below = (1ull << lane) - 1ull;
+    double rowsum = 0.0, dv = 0.0;
+    uint32_t k = 0;
+    for (uint32_t e = row[i]; e < row[i + 1]; ++e) {
+        const uint32_t c = col[e];
+        const double v = val64 ? val64[e] : (double)val32[e];
+        rowsum += v;
+        if (c == i) dv = v;
+        if (!in_part(c, i, part)) continue;
+        uint32_t slot;
+        if (k < kFillSteps) {
+            slot = sfirst[wv][k] + (uint32_t)__popcll(smask[wv][k] & below);
+        } else {
+            slot = base;
+            for (uint32_t l = 0; l < kSlice; ++l) {
+                const uint32_t r = s * kSlice + l;
+                const uint32_t Ll = r < n ? (uint32_t)len[r] : 0u;
+                slot += (Ll < k ? Ll : k) + ((l < lane && Ll > k) ? 1u : 0u);
+            }
+        }
+        const int32_t o = (int32_t)c - (int32_t)i;
+        scol[slot] = c;
+        scol16[slot] = (o >= -32767 && o <= 32767) ? (int16_t)o : kColEscape;
+        sval[slot] = (V)v;
+        if (entry_slot) entry_slot[e] = slot;
+        ++k;
+    }
+    diag[i] = (V)(form == kSellRowSum ? rowsum : dv);
 }
 
 }  // namespace
