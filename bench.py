@@ -193,8 +193,9 @@ def imr26_leg(runs=5):
 def tank3d_leg(steps=2, weights3d=None):
     """The 3D step on the sloshing-tank cloud (1,996,920 FP64 points,
     alpha = 1.3h): LBVH + alpha-shape tetrahedra + P1 + CG to rtol 1e-8,
-    mixed and FP64 Jacobi-PCG (and the learned mixed PCG when 3D weights
-    are given); steps/s and stage times of the last step."""
+    mixed and FP64 Jacobi-PCG, the mixed PCG with the classical FSAI factor
+    (and the learned mixed PCG when 3D weights are given); steps/s and stage
+    times of the last step."""
     import numpy as np
 
     import stream_amd
@@ -205,7 +206,7 @@ def tank3d_leg(steps=2, weights3d=None):
     f = np.sin(3.0 * pts[:, 0]) * np.cos(2.0 * pts[:, 1]) + pts[:, 2]
     st = PfemStep3D(pts, al, f)
     out = {"points": int(len(pts))}
-    solvers = ("mixed", "fp64")
+    solvers = ("mixed", "fp64", "fsai")
     if weights3d:
         from stream_amd.learned import PrecLearned
 
@@ -357,7 +358,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 companion measurement")
     ap.add_argument("--tank3d", action="store_true",
                     help="also time the 3D step: 1,996,920-point sloshing tank, alpha-shape tetrahedra, "
-                         "P1 assembly, mixed FP32/FP64 Jacobi-PCG and FP64 Jacobi-PCG to rtol 1e-8")
+                         "P1 assembly, mixed FP32/FP64 Jacobi-PCG, FP64 Jacobi-PCG and FSAI mixed PCG to rtol 1e-8")
     ap.add_argument("--sequence3d", action="store_true",
                     help="the 3D advect-remesh-solve sequence instead of the headline: 8,012,034-point tank, "
                          "--seq-steps steps of LBVH + alpha-shape tetrahedra + P1 + learned mixed PCG (rtol 1e-8), "
@@ -365,7 +366,7 @@ def main():
     ap.add_argument("--seq-steps", type=int, default=50)
     ap.add_argument("--weights3d", default=WEIGHTS3D, help="weight prefix of the 3D learned solver")
     ap.add_argument("--only-tank3d", action="store_true",
-                    help="time only the 2M 3D tank step (mixed, FP64 Jacobi and learned PCG) and print its JSON")
+                    help="time only the 2M 3D tank step (mixed, FP64 Jacobi, FSAI and learned PCG) and print its JSON")
     ap.add_argument("--components", action="store_true",
                     help="time the path's components at 1M points (LBVH step, TRBVH, Mesh2D phases, P1, CG "
                          "kernels) and print them as one JSON line instead of the headline")
