@@ -294,10 +294,9 @@ def test_sell_long_row():
     """One row far longer than the rest: its SELL slice is 300 entries wide
     (the build places the steps past its 256-step table by counting the
     slice's row lengths) while every other slice is 3 or 4 (row 0 coupled to
-    rows 1..300, plus a
-    tridiagonal band and couplings 700 rows away, diagonally dominant SPD);
-    the FP64 and mixed solves reach 1e-10 in the true residual computed
-    here with scipy."""
+    rows 1..300, plus a tridiagonal band and couplings 700 rows away,
+    diagonally dominant SPD); the FP64 and mixed solves reach 1e-10 in the
+    true residual computed here with scipy."""
     from stream_amd.numerics import DeviceCSR, HostCSR, LinSys, PrecJacobi, SolverCG
 
     _say("sell long row")
