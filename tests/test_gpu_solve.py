@@ -257,6 +257,31 @@ def test_reference_api_random_spd_vs_scipy():
         assert rel <= 1e-10
 
 
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 129, 200])
+def test_reference_api_ragged_sizes(n):
+    """Sizes that leave the last SELL slice partly empty, with rows that hold
+    only their diagonal (rows of length 0 in the SELL) between rows of up to
+    ~10 off-diagonals: the FP64 and mixed solves reach 1e-10 in a true
+    residual computed here."""
+    from stream_amd.numerics import DeviceCSR, HostCSR, LinSys, PrecJacobi, SolverCG
+
+    _say(f"reference api ragged n={n}")
+    rng = np.random.default_rng(100 + n)
+    M = sp.random(n, n, density=min(1.0, 3.0 / n), random_state=rng, format="csr")
+    keep = rng.uniform(size=n) < 0.7  # the other rows and columns stay diagonal
+    M = sp.diags(keep.astype(float)) @ M @ sp.diags(keep.astype(float))
+    A = (M @ M.T + sp.eye(n) * 4.0).tocsr()
+    A.sort_indices()
+    b = rng.uniform(-1, 1, n)
+    d64 = DeviceCSR(HostCSR(A.indptr.astype(np.uint32), A.indices.astype(np.uint32), A.data.astype(np.float64)))
+    s64 = LinSys(d64, b)
+    p64 = PrecJacobi(d64)
+    for meth in ("solve_rel", "solve_mixed"):
+        xx = np.zeros(n)
+        its, rel = getattr(SolverCG(), meth)(s64, p64, xx, rtol=1e-10)
+        assert np.linalg.norm(b - A @ xx) / np.linalg.norm(b) <= 1e-10, (meth, its, rel)
+
+
 def test_sell_column_offsets_far_and_boundary():
     """SELL columns are stored as 16-bit offsets from the row, with the
     32-bit column where the offset leaves [-32767, 32767].  Couplings at
