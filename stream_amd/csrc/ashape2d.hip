@@ -69,15 +69,18 @@ template <class S> __global__ void infinity_kernel(const BoxD<S, 2>* gbox, Sph4<
 // leaf children tested when their parent is visited.  The child that the
 // next pop would return is kept in a register and an internal child's node
 // word is loaded with its box; the stack (63 entries + the register = the
-// 64-deep stack) holds node words.  The visit order, the candidate order and
-// the capacity error are those of the plain push/pop loop.
+// 64-deep stack) holds node words, its first 16 levels in LDS.  The visit
+// order, the candidate order and the capacity error are those of the plain
+// push/pop loop.
+constexpr int kFillLds = 16;  // stack levels held in LDS ([level][thread]); deeper ones in private memory
 template <class S> __global__ __launch_bounds__(256) void fill_kernel(AshArgs<S> a) {
+    __shared__ uint32_t lstack[kFillLds][256];
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= a.n) return;
     const Sph4<S> si = a.P[tid];
     const S two_delta = Consts2D<S>::two_delta;
     const S rs = two_delta * si.r;
-    uint32_t stack[63];
+    uint32_t stack[63 - kFillLds];
     int ss = 0;
     bool live = a.n >= 2;
     uint32_t w = live ? a.node[0] : 0u;
@@ -118,7 +121,11 @@ template <class S> __global__ __launch_bounds__(256) void fill_kernel(AshArgs<S>
         }
         if (hit[0] && hit[1]) {
             if (ss < 63) {
-                stack[ss++] = wc[0];
+                if (ss < kFillLds)
+                    lstack[ss][threadIdx.x] = wc[0];
+                else
+                    stack[ss - kFillLds] = wc[0];
+                ++ss;
                 w = wc[1];
             } else {
                 // a full stack drops the second child and visits the first
@@ -128,7 +135,8 @@ template <class S> __global__ __launch_bounds__(256) void fill_kernel(AshArgs<S>
         } else if (hit[0] || hit[1]) {
             w = hit[1] ? wc[1] : wc[0];
         } else if (ss != 0) {
-            w = stack[--ss];
+            --ss;
+            w = ss < kFillLds ? lstack[ss][threadIdx.x] : stack[ss - kFillLds];
         } else {
             live = false;
         }

@@ -69,17 +69,20 @@ template <class S> __global__ void infinity3_kernel(const BoxD<S, 3>* gbox, Sph4
 // The DFS pushes the hit children of a node in order and pops the last one
 // pushed.  Here that last child is kept in a register instead of going
 // through the stack, and an internal child's node word is loaded with its
-// box, so descending costs one round of loads; the stack (private memory,
-// 63 entries + the register = the 64-deep stack) holds node words, so a
-// pop needs no node load.  The visit order, and so the candidate order, is
+// box, so descending costs one round of loads; the stack (63 entries + the
+// register = the 64-deep stack; the first 16 levels in LDS, laid out level
+// by level so that the lanes' words are consecutive, the rest in private
+// memory) holds node words, so a pop needs no node load.  The visit order, and so the candidate order, is
 // that of the plain push/pop loop, and so is the capacity error.
+constexpr int kFillLds = 16;  // stack levels held in LDS ([level][thread]); deeper ones in private memory
 template <class S> __global__ __launch_bounds__(256) void fill3_kernel(Ash3Args<S> a) {
+    __shared__ uint32_t lstack[kFillLds][256];
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= a.n) return;
     const Sph4<S> si = a.P[tid];
     const S two_delta = Consts3D<S>::two_delta;
     const S rs = two_delta * si.r;
-    uint32_t stack[63];
+    uint32_t stack[63 - kFillLds];
     int ss = 0;
     bool live = a.n >= 2;
     uint32_t w = live ? a.node[0] : 0u;
@@ -124,7 +127,11 @@ template <class S> __global__ __launch_bounds__(256) void fill3_kernel(Ash3Args<
         }
         if (hit[0] && hit[1]) {
             if (ss < 63) {
-                stack[ss++] = wc[0];
+                if (ss < kFillLds)
+                    lstack[ss][threadIdx.x] = wc[0];
+                else
+                    stack[ss - kFillLds] = wc[0];
+                ++ss;
                 w = wc[1];
             } else {
                 // a full stack drops the second child and visits the first
@@ -134,7 +141,8 @@ template <class S> __global__ __launch_bounds__(256) void fill3_kernel(Ash3Args<
         } else if (hit[0] || hit[1]) {
             w = hit[1] ? wc[1] : wc[0];
         } else if (ss != 0) {
-            w = stack[--ss];
+            --ss;
+            w = ss < kFillLds ? lstack[ss][threadIdx.x] : stack[ss - kFillLds];
         } else {
             live = false;
         }
