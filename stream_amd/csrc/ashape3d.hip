@@ -175,8 +175,8 @@ __device__ bool local_delaunay3(const Ash3Args<S>& a, uint32_t tid, uint8_t* cyc
     T(1) = pack_tet(1, 2, 3);
     T(2) = pack_tet(0, 1, 3);
     T(3) = pack_tet(0, 3, 2);
-    for (uint32_t j = 0; j + kNInf3D < ncand; ++j) a.active[slab3(tid, j, kCand3D)] = 0;
     if (ncand - kNInf3D < 3) {
+        for (uint32_t j = 0; j + kNInf3D < ncand; ++j) a.active[slab3(tid, j, kCand3D)] = 0;
         a.node_is_bnd[tid] = 1;
         a.nelem_out[tid] = 0;
         a.nelem[tid] = 0;
@@ -349,7 +349,8 @@ __device__ bool local_delaunay3(const Ash3Args<S>& a, uint32_t tid, uint8_t* cyc
             }
         }
         if (Rsize) {
-            for (uint32_t v = 0; v < kCycle3D; ++v) C(v) = (uint8_t)kEmpty;
+            // the star's tetrahedra use local ids below pi only
+            for (uint32_t v = 0; v < pi; ++v) C(v) = (uint8_t)kEmpty;
             --Rsize;
             const uint32_t tlst = T(Tlast + Rsize);
             const uint32_t ta = tlst & 255u, tb = (tlst >> 8) & 255u, tc3 = (tlst >> 16) & 255u;
@@ -411,7 +412,7 @@ __device__ bool local_delaunay3(const Ash3Args<S>& a, uint32_t tid, uint8_t* cyc
     // alpha filter (cpp:590-682)
     bool bnd = false;
     uint32_t nout = 0, nloc = 0;
-    for (uint32_t v = 0; v < kCycle3D; ++v) C(v) = 0;
+    for (uint32_t v = 0; v < ncand; ++v) C(v) = 0;
     uint32_t ti = 0;
     while (ti < Tlast) {
         const uint32_t tl = T(ti);
@@ -462,9 +463,6 @@ __device__ bool local_delaunay3(const Ash3Args<S>& a, uint32_t tid, uint8_t* cyc
         const S wn = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
         const S rc = wn / ((S)4 * vol * vol);
         if (rc <= qa.r * qa.r && rc <= qb.r * qb.r && rc <= qc.r * qc.r && rc <= qd.r * qd.r) {
-            a.active[slab3(tid, la - kNInf3D, kCand3D)] += 1;
-            a.active[slab3(tid, lb - kNInf3D, kCand3D)] += 1;
-            a.active[slab3(tid, lc - kNInf3D, kCand3D)] += 1;
             C(la) = (uint8_t)(C(la) + 1);
             C(lb) = (uint8_t)(C(lb) + 1);
             C(lc) = (uint8_t)(C(lc) + 1);
@@ -483,8 +481,14 @@ __device__ bool local_delaunay3(const Ash3Args<S>& a, uint32_t tid, uint8_t* cyc
             T(ti) = T(Tlast);
         }
     }
+    // per candidate, the number of kept tetrahedra using it (counted in
+    // LDS, stored once; the infinity nodes are in no kept tetrahedron)
     uint32_t nf = 0;
-    for (uint32_t v = 0; v < kCycle3D; ++v) nf += C(v) != 0;
+    for (uint32_t v = kNInf3D; v < ncand; ++v) {
+        const uint8_t c = C(v);
+        a.active[slab3(tid, v - kNInf3D, kCand3D)] = c;
+        nf += c != 0;
+    }
     a.node_is_bnd[tid] = bnd;
     a.nelem_out[tid] = (uint8_t)nout;
     a.nelem[tid] = (uint8_t)nloc;
