@@ -251,10 +251,35 @@ __device__ bool local_delaunay3(const Ash3Args<S>& a, uint32_t tid, uint8_t* cyc
                 S det[2], err[2];
 #pragma unroll
                 for (int u = 0; u < 2; ++u) det[u] = -insphere_filter(pa0, pb[u], pc[u], pd[u], pe0, &err[u]);
-                if (!((det[0] > err[0]) || (-det[0] > err[0]))) return false;
-                if (two && !((det[1] > err[1]) || (-det[1] > err[1]))) return false;
-                if (det[0] > (S)0) set_flag(t0, true);
-                if (two && det[1] > (S)0) set_flag(t0 + 1, true);
+                bool pos[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    pos[u] = det[u] > (S)0;
+                    if (u == 1 && !two) break;
+                    if ((det[u] > err[u]) || (-det[u] > err[u])) continue;
+                    if constexpr (sizeof(S) == sizeof(double)) {
+                        return false;
+                    } else {
+                        // FP32 inputs: the FP64 filter on the same (exactly
+                        // widened) coordinates certifies most of the signs the
+                        // FP32 filter cannot; the rest go to the exact kernel
+                        double qa[3], qb[3], qc[3], qd[3], qe[3];
+#pragma unroll
+                        for (int v = 0; v < 3; ++v) {
+                            qa[v] = pa0[v];
+                            qb[v] = pb[u][v];
+                            qc[v] = pc[u][v];
+                            qd[v] = pd[u][v];
+                            qe[v] = pe0[v];
+                        }
+                        double e2;
+                        const double d2 = -insphere_filter(qa, qb, qc, qd, qe, &e2);
+                        if (!((d2 > e2) || (-d2 > e2))) return false;
+                        pos[u] = d2 > 0.0;
+                    }
+                }
+                if (pos[0]) set_flag(t0, true);
+                if (two && pos[1]) set_flag(t0 + 1, true);
                 if (more) {
 #pragma unroll
                     for (int u = 0; u < 2; ++u)
