@@ -297,14 +297,34 @@ template <class S, bool EXACT> __device__ bool local_delaunay(const AshArgs<S>& 
                 bool certain;
                 S orientation = orient2d_filter(pa, pb, pc, &certain);
                 if (!certain) {
-                    if constexpr (!EXACT)
+                    if constexpr (!EXACT) {
                         return false;
-                    else
-                        orientation = (S)orient2d_exact(pa, pb, pc, ws, kExactWsDoubles, &over);
+                    } else {
+                        // FP32 inputs: the FP64 filter on the exactly widened
+                        // coordinates first; the exact stage if it cannot tell
+                        bool c64 = false;
+                        if constexpr (sizeof(S) != sizeof(double)) {
+                            const double qa[2] = {pa[0], pa[1]}, qb[2] = {pb[0], pb[1]}, qc[2] = {pc[0], pc[1]};
+                            const double o64 = orient2d_filter(qa, qb, qc, &c64);
+                            if (c64) orientation = o64 < 0.0 ? (S)-1 : (S)1;
+                        }
+                        if (!c64) orientation = (S)orient2d_exact(pa, pb, pc, ws, kExactWsDoubles, &over);
+                    }
                 }
                 S err;
                 S det = incircle_filter(pa, pb, pc, pd, &err);
-                if (!((det > err) || (-det > err))) {
+                bool c64 = false;
+                if constexpr (EXACT && sizeof(S) != sizeof(double)) {
+                    if (!((det > err) || (-det > err))) {
+                        const double wa[2] = {pa[0], pa[1]}, wb[2] = {pb[0], pb[1]}, wc[2] = {pc[0], pc[1]},
+                                     wd[2] = {pd[0], pd[1]};
+                        double e64;
+                        const double d64 = incircle_filter(wa, wb, wc, wd, &e64);
+                        c64 = (d64 > e64) || (-d64 > e64);
+                        if (c64) det = d64 > 0.0 ? (S)1 : (S)-1;
+                    }
+                }
+                if (!c64 && !((det > err) || (-det > err))) {
                     if constexpr (!EXACT) {
                         return false;
                     } else {
