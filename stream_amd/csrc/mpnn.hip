@@ -240,7 +240,7 @@ struct GemmArgs {
     uint32_t lda0;
     const float* add1;
     uint32_t lda1;
-    float* u;  // kEpiResRelu: pre-activation [n][N]
+    float* u;  // kEpiResRelu: pre-activation [n][N] (null: not stored)
 };
 
 // Y[n][N] = bf16(X[n][K]) * bf16(W) with FP32 accumulation; X columns
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) void gemm_rows_kernel(GemmArgs a) {
                 if (rr >= a.n) continue;
                 float v = acc[i] + bc;
                 if (EPI == kEpiResRelu) {
-                    a.u[(size_t)rr * N + c] = v;
+                    if (a.u) a.u[(size_t)rr * N + c] = v;
                     v = a.add0[(size_t)rr * a.lda0 + c] + fmaxf(v, 0.f);
                 } else if (EPI == kEpiAdd) {
                     if (a.add0) v += a.add0[(size_t)rr * a.lda0 + c];
@@ -950,7 +950,7 @@ void MpnnNet::reserve(uint32_t n, uint32_t nnz) {
     lsc.resize(4);
 }
 
-void MpnnNet::forward(const MpnnGraph& g, hipStream_t s) {
+void MpnnNet::forward(const MpnnGraph& g, hipStream_t s, bool keep_pre) {
     if (!weights_set) throw std::invalid_argument("network weights not set");
     const uint32_t n = g.n;
     reserve(n, g.nnz);
@@ -967,7 +967,7 @@ void MpnnNet::forward(const MpnnGraph& g, hipStream_t s) {
         a.bias = w.p + kParams.u_b[l];
         a.add0 = H[l].p;
         a.lda0 = kHid;
-        a.u = U[l].p;
+        a.u = keep_pre ? U[l].p : nullptr;
         run_gemm<128, 64, 64, kEpiResRelu>(a, s);
     }
     run_gemm<64, 64, 192, kEpiStore>(gargs(n, H[kLayers].p, kHid, wt.p + kWtDec, RSD.p, kRsd), s);

@@ -123,7 +123,9 @@ struct MpnnNet {
     void set_params(const float* host, hipStream_t s);
     void get_params(float* host);
     void reserve(uint32_t n, uint32_t nnz);
-    void forward(const MpnnGraph& g, hipStream_t s);
+    // keep_pre: store the layers' pre-activations for a backward pass
+    // (inference passes false and writes none)
+    void forward(const MpnnGraph& g, hipStream_t s, bool keep_pre = true);
     void loss(const MpnnGraph& g, hipStream_t s);      // after forward: lsc, y
     void backward(const MpnnGraph& g, hipStream_t s);  // after loss: grad
     void adam(float lr, float b1, float b2, float eps, hipStream_t s);

@@ -566,7 +566,7 @@ struct Step3D {
                 if (!learned) throw std::invalid_argument("solver 3 needs a network (PfemStep3D_set_learned)");
                 factor_structure<float>(A, L32, s);
                 bind_graph(learned->graph, s, false);  // inference: no transpose map
-                learned->net.forward(learned->graph, s);
+                learned->net.forward(learned->graph, s, false);
                 learned->net.pack(learned->graph, L32, s);
                 const uint32_t bad = read_scalar(ctx().d_flags + kFlagLearned);
                 if (bad) {
