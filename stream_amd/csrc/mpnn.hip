@@ -170,7 +170,7 @@ __global__ __launch_bounds__(kFoldThreads) void prep_fold_kernel(uint32_t nblk, 
 }
 
 // Node features, edge features (bf16-rounded) and the transpose map.
-__global__ __launch_bounds__(256) void prep_feat_kernel(uint32_t n, const uint32_t* __restrict__ row,
+__global__ __launch_bounds__(256) void prep_feat_kernel(uint32_t n, uint32_t lanes, const uint32_t* __restrict__ row,
                                                         const uint32_t* __restrict__ col,
                                                         const double* __restrict__ val, const double* __restrict__ xy,
                                                         uint32_t xs, uint32_t dim, const uint8_t* __restrict__ bnd,
@@ -183,14 +183,20 @@ __global__ __launch_bounds__(256) void prep_feat_kernel(uint32_t n, const uint32
     if (dim == 3) ext = fmax(ext, scal[5] - scal[4]);
     if (!(ext > 0.0)) ext = 1.0;
     const double lbar = scal[7] > 0.0 ? scal[6] / scal[7] : 1.0;
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const double aii = dpos[i] != ~0u ? val[dpos[i]] : 0.0;
-        float* nf = nfeat + (size_t)i * kNodeFeats;
-        for (uint32_t a = 0; a < 3; ++a)
-            nf[a] = a < dim ? (float)((xy[(size_t)i * xs + a] - scal[2 * a]) / ext) : 0.f;
-        nf[3] = bnd[i] ? 1.f : 0.f;
-        nf[4] = (float)(aii / 8.0);
-        for (uint32_t e = row[i]; e < row[i + 1]; ++e) {
+    // `lanes` lanes (8 or 16) per row: the row's entries in consecutive
+    // lanes, so the edge-feature stores of a wavefront are whole segments
+    const uint32_t sub = threadIdx.x % lanes;
+    const uint32_t rows_per_block = 256 / lanes;
+    for (uint32_t i = blockIdx.x * rows_per_block + threadIdx.x / lanes; i < n; i += gridDim.x * rows_per_block) {
+        if (sub == 0) {
+            const double aii = dpos[i] != ~0u ? val[dpos[i]] : 0.0;
+            float* nf = nfeat + (size_t)i * kNodeFeats;
+            for (uint32_t a = 0; a < 3; ++a)
+                nf[a] = a < dim ? (float)((xy[(size_t)i * xs + a] - scal[2 * a]) / ext) : 0.f;
+            nf[3] = bnd[i] ? 1.f : 0.f;
+            nf[4] = (float)(aii / 8.0);
+        }
+        for (uint32_t e = row[i] + sub; e < row[i + 1]; e += lanes) {
             const uint32_t j = col[e];
             if (j == i) {
                 efeat[e] = make_float2(0.f, 0.f);
@@ -891,7 +897,9 @@ void mpnn_prepare(MpnnGraph& g, hipStream_t s, bool transpose) {
     hipLaunchKernelGGL(prep_rows_kernel, dim3(nb), dim3(256), 0, s, n, g.row, g.col, g.val, g.xy, g.xy_stride, g.dim,
                        g.sd.p, g.dpos.p, g.deg.p, g.part.p);
     hipLaunchKernelGGL(prep_fold_kernel, dim3(1), dim3(kFoldThreads), 0, s, nb, g.part.p, g.scal.p);
-    hipLaunchKernelGGL(prep_feat_kernel, dim3(blocks_for(n, 256, 4096)), dim3(256), 0, s, n, g.row, g.col, g.val,
+    const uint32_t lanes = (uint64_t)g.nnz > 10ull * n ? 16u : 8u;  // ~15 entries per row in 3D, ~7 in 2D
+    hipLaunchKernelGGL(prep_feat_kernel, dim3(blocks_for(n, 256 / lanes, 65536)), dim3(256), 0, s, n, lanes, g.row,
+                       g.col, g.val,
                        g.xy, g.xy_stride, g.dim, g.bnd, g.sd.p, g.dpos.p, g.scal.p, g.nfeat.p, g.efeat.p,
                        transpose ? g.tr.p : nullptr, ctx().d_flags);
     g.has_transpose = transpose;
