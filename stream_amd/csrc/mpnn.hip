@@ -562,8 +562,12 @@ __global__ __launch_bounds__(256) void head_fwd_kernel(uint32_t n, const uint32_
         const uint32_t k = uniform(k0);
         const float r = rsd[(size_t)k * kRsd + f];
         const uint32_t e1 = row[k + 1];
+        // columns are sorted, so the upper entries (the only ones with an
+        // output) follow the diagonal
+        const uint32_t dk = dpos[k];
+        const uint32_t e0 = dk != ~0u ? dk + 1 : row[k];
         // loads of up to kMsgBatch edges are issued before any is used
-        for (uint32_t eb = row[k]; eb < e1; eb += kMsgBatch) {
+        for (uint32_t eb = e0; eb < e1; eb += kMsgBatch) {
             uint32_t jj[kMsgBatch];
             float2 ef[kMsgBatch];
             float sj[kMsgBatch];
@@ -591,7 +595,7 @@ __global__ __launch_bounds__(256) void head_fwd_kernel(uint32_t n, const uint32_
         const float o = wsumf(bfr(fmaxf(zd, 0.f)) * wd) + kDiagShift;
         if (f == 0) {
             od[k] = o;
-            if (dpos[k] != ~0u) gval[dpos[k]] = bnd[k] ? 1.f : softplusf(o);
+            if (dk != ~0u) gval[dk] = bnd[k] ? 1.f : softplusf(o);
         }
     }
 }
@@ -1119,6 +1123,14 @@ int32_t PrecLearned_set_system_dim(PrecLearned* h, uint32_t dim, uint32_t n, con
         if (dim != 2 && dim != 3) throw std::invalid_argument("PrecLearned_set_system: dim must be 2 or 3");
         const uint32_t nnz = row[n];
         if (nnz && (!col || !val)) throw std::invalid_argument("PrecLearned_set_system: null argument");
+        // the kernels rely on ascending columns within a row (the upper part
+        // follows the diagonal; transpose positions by binary search)
+        for (uint32_t i = 0; i < n; ++i) {
+            if (row[i] > row[i + 1]) throw std::invalid_argument("PrecLearned_set_system: row pointers decrease");
+            for (uint32_t e = row[i] + 1; e < row[i + 1]; ++e)
+                if (col[e] <= col[e - 1])
+                    throw std::invalid_argument("PrecLearned_set_system: columns of a row must be ascending");
+        }
         MpnnGraph& g = h->graph;
         g.own_row.resize(n + 1);
         g.own_col.resize(nnz ? nnz : 1);

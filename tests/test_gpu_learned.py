@@ -94,6 +94,24 @@ def test_forward_matches_oracle(small):
     assert abs(loss - ref_loss) <= 1e-4 * max(1.0, abs(ref_loss))
 
 
+def test_set_system_rejects_unsorted_columns(small):
+    """The network's kernels rely on ascending columns within each row; a
+    system whose row lists them out of order is refused."""
+    from stream_amd import StreamError
+    from stream_amd.learned import PrecLearned
+
+    row, col, val, b, xy, bnd = small
+    col2 = np.array(col, np.uint32).copy()
+    val2 = np.array(val, np.float64).copy()
+    i = int(np.argmax(np.diff(row) >= 2))  # a row with two entries or more
+    e = int(row[i])
+    col2[e], col2[e + 1] = col2[e + 1], col2[e]
+    val2[e], val2[e + 1] = val2[e + 1], val2[e]
+    net = PrecLearned(_params())
+    with pytest.raises(StreamError, match="ascending"):
+        net.set_system(row, col2, val2, xy, bnd)
+
+
 def test_zero_head_is_jacobi(small):
     from stream_amd.learned import PrecLearned
 
