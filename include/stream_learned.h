@@ -33,7 +33,8 @@ int32_t PrecLearned_set_params(PrecLearned* net, const float* w, uint32_t count)
 int32_t PrecLearned_get_params(PrecLearned* net, float* w, uint32_t count);
 int32_t PrecLearned_set_adam(PrecLearned* net, float beta1, float beta2, float eps);
 /* Host system: CSR (sorted columns, symmetric pattern, FP64 values), points
- * [n][2] in row order, Dirichlet flags [n].  Copied to the device. */
+ * [n][2] in row order, Dirichlet flags [n].  Copied to the device.  A row
+ * whose columns are not strictly ascending is refused (invalid argument). */
 int32_t PrecLearned_set_system(PrecLearned* net, uint32_t n, const uint32_t* row, const uint32_t* col,
                                const double* val, const double* xy, const uint8_t* bnd);
 /* the same with coordinates [n][dim], dim 2 or 3 */
