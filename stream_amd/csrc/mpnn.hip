@@ -820,12 +820,14 @@ __global__ __launch_bounds__(256) void transpose_weights_kernel(const uint16_t* 
 }
 
 // L = D^-1/2 Ghat^T into the SELL factor: the up slot of CSR entry (k, j),
-// j > k, holds L_jk = sd_j Ghat_kj; diag L_kk = sd_k Ghat_kk.
+// j > k, and the lo slot of entry (j, k) hold L_jk = sd_j Ghat_kj; diag
+// L_kk = sd_k Ghat_kk.
 __global__ __launch_bounds__(256) void pack_kernel(uint32_t n, const uint32_t* __restrict__ row,
                                                    const uint32_t* __restrict__ col, const double* __restrict__ sd,
                                                    const uint32_t* __restrict__ dpos, const float* __restrict__ gval,
                                                    const uint32_t* __restrict__ slot, float* __restrict__ up_val,
-                                                   float* __restrict__ diag, uint32_t* __restrict__ flags) {
+                                                   float* __restrict__ lo_val, float* __restrict__ diag,
+                                                   uint32_t* __restrict__ flags) {
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
     bool bad = false;
@@ -840,6 +842,8 @@ __global__ __launch_bounds__(256) void pack_kernel(uint32_t n, const uint32_t* _
         const float v = (float)(sd[j] * (double)gval[e]);
         bad |= !isfinite(v);
         up_val[slot[e]] = v;
+        const uint32_t f = csr_find(row, col, j, k);  // (j, k): the lo slot of L_jk
+        if (f != ~0u) lo_val[slot[f]] = v;
     }
     if (bad) atomicMax(&flags[kFlagLearned], 1u);
 }
@@ -1051,8 +1055,7 @@ void MpnnNet::adam(float lr, float b1, float b2, float eps, hipStream_t s) {
 void MpnnNet::pack(const MpnnGraph& g, LowerFactor<float>& L, hipStream_t s) {
     if (!g.n) return;
     hipLaunchKernelGGL(pack_kernel, dim3(grid_for(g.n, 256)), dim3(256), 0, s, g.n, g.row, g.col, g.sd.p, g.dpos.p,
-                       gval.p, L.slot.p, L.up.val.p, L.diag.p, ctx().d_flags);
-    mirror_lower(L, s);
+                       gval.p, L.slot.p, L.up.val.p, L.lo.val.p, L.diag.p, ctx().d_flags);
 }
 
 void mpnn_check_flags() {

@@ -118,10 +118,8 @@ template <class V> struct LowerFactor {
     uint32_t n = 0;
     Sell<V> lo, up;
     DBuf<V> diag;
-    // the CSR pattern the halves were built from (borrowed) and, per CSR
-    // entry, its slot in lo (column < row) or up (column > row)
-    const uint32_t* row = nullptr;
-    const uint32_t* col = nullptr;
+    // per CSR entry of the pattern the halves were built from, its slot in
+    // lo (column < row) or up (column > row)
     DBuf<uint32_t> slot;
 };
 

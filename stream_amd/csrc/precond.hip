@@ -17,7 +17,7 @@ __global__ __launch_bounds__(256) void fsai_kernel(uint32_t n, const uint32_t* _
                                                    const uint32_t* __restrict__ col, const double* __restrict__ val,
                                                    const uint16_t* __restrict__ up_len,
                                                    const uint32_t* __restrict__ slot, V* __restrict__ up_val,
-                                                   V* __restrict__ diag) {
+                                                   V* __restrict__ lo_val, V* __restrict__ diag) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t lu = up_len[i];
@@ -95,66 +95,38 @@ __global__ __launch_bounds__(256) void fsai_kernel(uint32_t n, const uint32_t* _
     }
     const double sc = 1.0 / sqrt(g[0]);
     diag[i] = (V)(g[0] * sc);
+    // L_ji (j > i) into the up slot of (i, j) and the lo slot of (j, i)
     uint32_t k = 1;
     for (uint32_t e = row[i]; e < row[i + 1]; ++e)
         if (col[e] > i) {
-            up_val[slot[e]] = (V)(k < m ? g[k] * sc : 0.0);
+            const V v = (V)(k < m ? g[k] * sc : 0.0);
+            up_val[slot[e]] = v;
+            const uint32_t f = csr_find(row, col, col[e], i);
+            if (f != ~0u) lo_val[slot[f]] = v;
             ++k;
         }
-}
-
-// lo(i, j) = up(j, i): row i reads, for each strict-lower entry (i, j),
-// the value of entry (j, i) of row j (slots from the CSR entry map; a lower
-// entry whose transpose is not in the pattern keeps its value).
-template <class V>
-__global__ __launch_bounds__(256) void mirror_kernel(uint32_t n, const uint32_t* __restrict__ row,
-                                                     const uint32_t* __restrict__ col,
-                                                     const uint32_t* __restrict__ slot,
-                                                     const V* __restrict__ up_val, V* __restrict__ lo_val) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    for (uint32_t e = row[i]; e < row[i + 1]; ++e) {
-        const uint32_t j = col[e];
-        if (j >= i) continue;
-        for (uint32_t f = row[j]; f < row[j + 1]; ++f)
-            if (col[f] == i) {
-                lo_val[slot[e]] = up_val[slot[f]];
-                break;
-            }
-    }
 }
 
 }  // namespace
 
 template <class V> void factor_structure(const DevCsr<double>& A, LowerFactor<V>& L, hipStream_t s) {
     L.n = A.n;
-    L.row = A.row.p;
-    L.col = A.col.p;
     L.slot.resize(A.nnz ? A.nnz : 1);
     build_sell<V>(A.n, A.row.p, A.col.p, A.val.p, nullptr, kSellStandard, kSellUpper, L.up, s, L.slot.p);
     build_sell<V>(A.n, A.row.p, A.col.p, A.val.p, nullptr, kSellStandard, kSellLower, L.lo, s, L.slot.p);
     L.diag.resize(A.n ? A.n : 1);
 }
 
-template <class V> void mirror_lower(LowerFactor<V>& L, hipStream_t s) {
-    if (!L.n) return;
-    hipLaunchKernelGGL((mirror_kernel<V>), dim3(grid_for(L.n, 256)), dim3(256), 0, s, L.n, L.row, L.col, L.slot.p,
-                       L.up.val.p, L.lo.val.p);
-}
-
 template <class V> void fsai_factor(const DevCsr<double>& A, LowerFactor<V>& L, hipStream_t s) {
     factor_structure(A, L, s);
     if (!A.n) return;
     hipLaunchKernelGGL((fsai_kernel<V>), dim3(grid_for(A.n, 256)), dim3(256), 0, s, A.n, A.row.p, A.col.p, A.val.p,
-                       L.up.len.p, L.slot.p, L.up.val.p, L.diag.p);
-    mirror_lower(L, s);
+                       L.up.len.p, L.slot.p, L.up.val.p, L.lo.val.p, L.diag.p);
 }
 
 template void factor_structure<float>(const DevCsr<double>&, LowerFactor<float>&, hipStream_t);
 template void factor_structure<double>(const DevCsr<double>&, LowerFactor<double>&, hipStream_t);
 template void fsai_factor<float>(const DevCsr<double>&, LowerFactor<float>&, hipStream_t);
 template void fsai_factor<double>(const DevCsr<double>&, LowerFactor<double>&, hipStream_t);
-template void mirror_lower<float>(LowerFactor<float>&, hipStream_t);
-template void mirror_lower<double>(LowerFactor<double>&, hipStream_t);
 
 }  // namespace stream_amd
