@@ -475,6 +475,90 @@ __global__ __launch_bounds__(256) void sell_fill_kernel(uint32_t n, uint32_t nsl
     diag[i] = (V)(form == kSellRowSum ? rowsum : dv);
 }
 
+// Standard form (the diagonal is the row's diagonal entry, no row sum): one
+// block of 256 threads per slice, 4 lanes per row, so a wavefront reads 16
+// rows' entries 4 at a time instead of 64 rows' one at a time.  Wavefront 0
+// records the slice's step ballots and first slots in LDS; a lane's part
+// entry then takes its index k in the row from the ballot of its row's 4
+// lanes, and its slot as in sell_fill_kernel.
+constexpr uint32_t kFillGroup = 4;
+template <class V>
+__global__ __launch_bounds__(256) void sell_fill_std_kernel(uint32_t n, uint32_t nslices,
+                                                            const uint32_t* __restrict__ row,
+                                                            const uint32_t* __restrict__ col,
+                                                            const double* __restrict__ val64,
+                                                            const float* __restrict__ val32, int part,
+                                                            const uint32_t* __restrict__ off,
+                                                            const uint16_t* __restrict__ width,
+                                                            const uint16_t* __restrict__ len,
+                                                            uint32_t* __restrict__ scol, int16_t* __restrict__ scol16,
+                                                            V* __restrict__ sval, V* __restrict__ diag,
+                                                            uint32_t* __restrict__ entry_slot) {
+    __shared__ uint64_t smask[kFillSteps];
+    __shared__ uint32_t sfirst[kFillSteps];
+    const uint32_t s = blockIdx.x;
+    if (s >= nslices) return;  // the whole block
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t base = off[s];
+    const uint32_t W = width[s];
+    if (threadIdx.x < 64) {
+        const uint32_t i = s * kSlice + lane;
+        const uint32_t L = i < n ? (uint32_t)len[i] : 0u;
+        uint32_t seg = base;
+        const uint32_t ws = W < kFillSteps ? W : kFillSteps;
+        for (uint32_t k = 0; k < ws; ++k) {
+            const uint64_t m = __ballot(k < L);
+            if (lane == 0) {
+                smask[k] = m;
+                sfirst[k] = seg;
+            }
+            seg += (uint32_t)__popcll(m);
+        }
+    }
+    __syncthreads();
+    const uint32_t rl = threadIdx.x / kFillGroup;  // row within the slice
+    const uint32_t sub = threadIdx.x % kFillGroup;
+    const uint32_t i = s * kSlice + rl;
+    const bool live = i < n;
+    const uint32_t e0 = live ? row[i] : 0u, e1 = live ? row[i + 1] : 0u;
+    const uint32_t gshift = lane & ~(kFillGroup - 1u);  // the row's lanes in a ballot
+    const uint64_t below_row = (1ull << rl) - 1ull;
+    uint32_t kbase = 0;
+    bool has_diag = false;
+    for (uint32_t q = 0;; q += kFillGroup) {
+        const uint32_t e = e0 + q + sub;
+        const bool act = e < e1;
+        if (__ballot(act) == 0) break;
+        const uint32_t c = act ? col[e] : 0u;
+        const bool inp = act && in_part(c, i, part);
+        const uint32_t grp = (uint32_t)(__ballot(inp) >> gshift) & ((1u << kFillGroup) - 1u);
+        if (inp) {
+            const uint32_t k = kbase + (uint32_t)__popc(grp & ((1u << sub) - 1u));
+            uint32_t slot;
+            if (k < kFillSteps) {
+                slot = sfirst[k] + (uint32_t)__popcll(smask[k] & below_row);
+            } else {
+                slot = base;
+                for (uint32_t l = 0; l < kSlice; ++l) {
+                    const uint32_t r = s * kSlice + l;
+                    const uint32_t Ll = r < n ? (uint32_t)len[r] : 0u;
+                    slot += (Ll < k ? Ll : k) + ((l < rl && Ll > k) ? 1u : 0u);
+                }
+            }
+            const int32_t o = (int32_t)c - (int32_t)i;
+            scol[slot] = c;
+            scol16[slot] = (o >= -32767 && o <= 32767) ? (int16_t)o : kColEscape;
+            sval[slot] = (V)(val64 ? val64[e] : (double)val32[e]);
+            if (entry_slot) entry_slot[e] = slot;
+        }
+        const bool isd = act && c == i;
+        if (isd) diag[i] = (V)(val64 ? val64[e] : (double)val32[e]);
+        has_diag |= ((uint32_t)(__ballot(isd) >> gshift) & ((1u << kFillGroup) - 1u)) != 0u;
+        kbase += (uint32_t)__popc(grp);
+    }
+    if (live && sub == 0 && !has_diag) diag[i] = (V)0;  // a row without a diagonal entry
+}
+
 }  // namespace
 
 template <class V>
@@ -503,9 +587,14 @@ void build_sell(uint32_t n, const uint32_t* row, const uint32_t* col, const doub
     S.col.resize(S.entries + kSlice);
     S.col16.resize(S.entries + kSlice);
     S.val.resize(S.entries + kSlice);
-    hipLaunchKernelGGL((sell_fill_kernel<V>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, S.nslices, row, col, val64,
-                       val32, (int)form, (int)part, S.off.p, S.width.p, S.len.p, S.col.p, S.col16.p, S.val.p, S.diag.p,
-                       entry_slot);
+    if (form == kSellStandard)
+        hipLaunchKernelGGL((sell_fill_std_kernel<V>), dim3(S.nslices), dim3(256), 0, s, n, S.nslices, row, col, val64,
+                           val32, (int)part, S.off.p, S.width.p, S.len.p, S.col.p, S.col16.p, S.val.p, S.diag.p,
+                           entry_slot);
+    else
+        hipLaunchKernelGGL((sell_fill_kernel<V>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, S.nslices, row, col,
+                           val64, val32, (int)form, (int)part, S.off.p, S.width.p, S.len.p, S.col.p, S.col16.p,
+                           S.val.p, S.diag.p, entry_slot);
 }
 template void build_sell<float>(uint32_t, const uint32_t*, const uint32_t*, const double*, const float*, SellForm,
                                 SellPart, Sell<float>&, hipStream_t, uint32_t*);
