@@ -403,97 +403,32 @@ __device__ __forceinline__ uint32_t step_slot(uint32_t seg, uint64_t m) {
     return seg + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// One wave per slice (blocks of 256 rows): the k-th entry of the part of
-// every row longer than k goes to step k, in lane order.  The wave first
-// records, for each step, the ballot of the rows that reach it and the
-// step's first slot (in LDS, up to kFillSteps steps); then every lane walks
-// its CSR row in order (independent loads) and places its k-th part entry
-// at first slot + the number of lower lanes in the ballot.  Steps beyond
+constexpr uint32_t kFillSteps = 256;  // step tables held in LDS per slice
+
+// One block of 256 threads per slice, 4 lanes per row: a wavefront reads
+// 16 rows' entries 4 at a time.  Wavefront 0 first records, for each step k
+// of the slice, the ballot of the rows longer than k and the step's first
+// slot (in LDS, up to kFillSteps steps).  The k-th part entry of a row
+// (k from the ballot of the row's 4 lanes) goes to step k, at the step's
+// first slot + the number of lower rows in the ballot; steps beyond
 // kFillSteps (rows that long are rare) count the slice's lengths directly.
-// The row sum and the diagonal are accumulated over the row in order.
-constexpr uint32_t kFillSteps = 256;
+// The diagonal is the row's diagonal entry (standard form) or its FP64 row
+// sum (row-sum form), added entry by entry in row order through the row's
+// lanes, as a one-lane walk of the row would.
+constexpr uint32_t kFillGroup = 4;
+static_assert(kSlice * kFillGroup == 256, "a block of 256 threads holds one slice");
 template <class V>
-__global__ __launch_bounds__(256) void sell_fill_kernel(uint32_t n, uint32_t nslices, const uint32_t* __restrict__ row,
+__global__ __launch_bounds__(256) void sell_fill_kernel(uint32_t n, uint32_t nslices,
+                                                        const uint32_t* __restrict__ row,
                                                         const uint32_t* __restrict__ col,
                                                         const double* __restrict__ val64,
                                                         const float* __restrict__ val32, int form, int part,
                                                         const uint32_t* __restrict__ off,
                                                         const uint16_t* __restrict__ width,
-                                                        const uint16_t* __restrict__ len, uint32_t* __restrict__ scol,
-                                                        int16_t* __restrict__ scol16, V* __restrict__ sval,
-                                                        V* __restrict__ diag, uint32_t* __restrict__ entry_slot) {
-    __shared__ uint64_t smask[4][kFillSteps];
-    __shared__ uint32_t sfirst[4][kFillSteps];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t s = i / kSlice, lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    const bool has = s < nslices;  // wavefronts past the last slice only meet the barrier
-    const bool live = i < n;
-    const uint32_t L = live ? (uint32_t)len[i] : 0u;
-    const uint32_t W = has ? (uint32_t)width[s] : 0u;
-    const uint32_t base = has ? off[s] : 0u;
-    {
-        uint32_t seg = base;
-        const uint32_t ws = W < kFillSteps ? W : kFillSteps;
-        for (uint32_t k = 0; k < ws; ++k) {
-            const uint64_t m = __ballot(k < L);
-            if (lane == 0) {
-                smask[wv][k] = m;
-                sfirst[wv][k] = seg;
-            }
-            seg += (uint32_t)__popcll(m);
-        }
-    }
-    __syncthreads();
-    if (!live) return;
-    const uint64_t below = (1ull << lane) - 1ull;
-    double rowsum = 0.0, dv = 0.0;
-    uint32_t k = 0;
-    for (uint32_t e = row[i]; e < row[i + 1]; ++e) {
-        const uint32_t c = col[e];
-        const double v = val64 ? val64[e] : (double)val32[e];
-        rowsum += v;
-        if (c == i) dv = v;
-        if (!in_part(c, i, part)) continue;
-        uint32_t slot;
-        if (k < kFillSteps) {
-            slot = sfirst[wv][k] + (uint32_t)__popcll(smask[wv][k] & below);
-        } else {
-            slot = base;
-            for (uint32_t l = 0; l < kSlice; ++l) {
-                const uint32_t r = s * kSlice + l;
-                const uint32_t Ll = r < n ? (uint32_t)len[r] : 0u;
-                slot += (Ll < k ? Ll : k) + ((l < lane && Ll > k) ? 1u : 0u);
-            }
-        }
-        const int32_t o = (int32_t)c - (int32_t)i;
-        scol[slot] = c;
-        scol16[slot] = (o >= -32767 && o <= 32767) ? (int16_t)o : kColEscape;
-        sval[slot] = (V)v;
-        if (entry_slot) entry_slot[e] = slot;
-        ++k;
-    }
-    diag[i] = (V)(form == kSellRowSum ? rowsum : dv);
-}
-
-// Standard form (the diagonal is the row's diagonal entry, no row sum): one
-// block of 256 threads per slice, 4 lanes per row, so a wavefront reads 16
-// rows' entries 4 at a time instead of 64 rows' one at a time.  Wavefront 0
-// records the slice's step ballots and first slots in LDS; a lane's part
-// entry then takes its index k in the row from the ballot of its row's 4
-// lanes, and its slot as in sell_fill_kernel.
-constexpr uint32_t kFillGroup = 4;
-template <class V>
-__global__ __launch_bounds__(256) void sell_fill_std_kernel(uint32_t n, uint32_t nslices,
-                                                            const uint32_t* __restrict__ row,
-                                                            const uint32_t* __restrict__ col,
-                                                            const double* __restrict__ val64,
-                                                            const float* __restrict__ val32, int part,
-                                                            const uint32_t* __restrict__ off,
-                                                            const uint16_t* __restrict__ width,
-                                                            const uint16_t* __restrict__ len,
-                                                            uint32_t* __restrict__ scol, int16_t* __restrict__ scol16,
-                                                            V* __restrict__ sval, V* __restrict__ diag,
-                                                            uint32_t* __restrict__ entry_slot) {
+                                                        const uint16_t* __restrict__ len,
+                                                        uint32_t* __restrict__ scol, int16_t* __restrict__ scol16,
+                                                        V* __restrict__ sval, V* __restrict__ diag,
+                                                        uint32_t* __restrict__ entry_slot) {
     __shared__ uint64_t smask[kFillSteps];
     __shared__ uint32_t sfirst[kFillSteps];
     const uint32_t s = blockIdx.x;
@@ -525,11 +460,20 @@ __global__ __launch_bounds__(256) void sell_fill_std_kernel(uint32_t n, uint32_t
     const uint64_t below_row = (1ull << rl) - 1ull;
     uint32_t kbase = 0;
     bool has_diag = false;
+    double rowsum = 0.0;
     for (uint32_t q = 0;; q += kFillGroup) {
         const uint32_t e = e0 + q + sub;
         const bool act = e < e1;
         if (__ballot(act) == 0) break;
         const uint32_t c = act ? col[e] : 0u;
+        const double v = act ? (val64 ? val64[e] : (double)val32[e]) : 0.0;
+        if (form == kSellRowSum) {
+#pragma unroll
+            for (uint32_t j = 0; j < kFillGroup; ++j) {
+                const double vj = __shfl(v, (int)(gshift + j), 64);
+                if (e0 + q + j < e1) rowsum += vj;
+            }
+        }
         const bool inp = act && in_part(c, i, part);
         const uint32_t grp = (uint32_t)(__ballot(inp) >> gshift) & ((1u << kFillGroup) - 1u);
         if (inp) {
@@ -548,15 +492,20 @@ __global__ __launch_bounds__(256) void sell_fill_std_kernel(uint32_t n, uint32_t
             const int32_t o = (int32_t)c - (int32_t)i;
             scol[slot] = c;
             scol16[slot] = (o >= -32767 && o <= 32767) ? (int16_t)o : kColEscape;
-            sval[slot] = (V)(val64 ? val64[e] : (double)val32[e]);
+            sval[slot] = (V)v;
             if (entry_slot) entry_slot[e] = slot;
         }
         const bool isd = act && c == i;
-        if (isd) diag[i] = (V)(val64 ? val64[e] : (double)val32[e]);
+        if (isd && form != kSellRowSum) diag[i] = (V)v;
         has_diag |= ((uint32_t)(__ballot(isd) >> gshift) & ((1u << kFillGroup) - 1u)) != 0u;
         kbase += (uint32_t)__popc(grp);
     }
-    if (live && sub == 0 && !has_diag) diag[i] = (V)0;  // a row without a diagonal entry
+    if (live && sub == 0) {
+        if (form == kSellRowSum)
+            diag[i] = (V)rowsum;
+        else if (!has_diag)
+            diag[i] = (V)0;  // a row without a diagonal entry
+    }
 }
 
 }  // namespace
@@ -587,14 +536,9 @@ void build_sell(uint32_t n, const uint32_t* row, const uint32_t* col, const doub
     S.col.resize(S.entries + kSlice);
     S.col16.resize(S.entries + kSlice);
     S.val.resize(S.entries + kSlice);
-    if (form == kSellStandard)
-        hipLaunchKernelGGL((sell_fill_std_kernel<V>), dim3(S.nslices), dim3(256), 0, s, n, S.nslices, row, col, val64,
-                           val32, (int)part, S.off.p, S.width.p, S.len.p, S.col.p, S.col16.p, S.val.p, S.diag.p,
-                           entry_slot);
-    else
-        hipLaunchKernelGGL((sell_fill_kernel<V>), dim3(grid_for(n, 256)), dim3(256), 0, s, n, S.nslices, row, col,
-                           val64, val32, (int)form, (int)part, S.off.p, S.width.p, S.len.p, S.col.p, S.col16.p,
-                           S.val.p, S.diag.p, entry_slot);
+    hipLaunchKernelGGL((sell_fill_kernel<V>), dim3(S.nslices), dim3(256), 0, s, n, S.nslices, row, col, val64, val32,
+                       (int)form, (int)part, S.off.p, S.width.p, S.len.p, S.col.p, S.col16.p, S.val.p, S.diag.p,
+                       entry_slot);
 }
 template void build_sell<float>(uint32_t, const uint32_t*, const uint32_t*, const double*, const float*, SellForm,
                                 SellPart, Sell<float>&, hipStream_t, uint32_t*);
