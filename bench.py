@@ -489,7 +489,21 @@ def main():
                 "traffic_source": None if traffic is None else
                 "committed counter summary (profiles/, rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE of "
                 "bench.py --profile-kernels), not measured in this run",
-                "kernel": kname, "kernel_ms": round(kms, 5), "alg_bytes_per_launch": int(alg_bytes)}
+                "kernel": kname, "kernel_ms": round(kms, 5), "alg_bytes_per_launch": int(alg_bytes),
+                "copy_same_bytes": copy_ceiling(alg_bytes, achieved)}
+
+    def copy_ceiling(alg_bytes, achieved):
+        # a plain device copy (16-byte loads and stores) that moves the same
+        # bytes in one launch, timed the same way (back-to-back launches
+        # between HIP events): what the HBM sustains at this transfer size
+        import ctypes
+
+        half = int(alg_bytes) // 2 // 16 * 16
+        cms = ctypes.c_double(0)
+        stream_amd.check(stream_amd.lib.Stream_copy_calibration(ctypes.c_uint64(half), 200, ctypes.byref(cms)))
+        gbs = 2 * half / (cms.value * 1e-3) / 1e9
+        return {"bytes_moved": 2 * half, "kernel_ms": round(cms.value, 5), "achieved": round(gbs, 1),
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel_vs_copy": round(achieved / gbs, 4)}
 
     kms = step.time_dir_kernel(fp32=fp32, reps=200, llt=llt)
     dname = "llt_dir_kernel" if llt else "cg_dir_kernel"
