@@ -47,6 +47,18 @@ __global__ __launch_bounds__(256) void tet_fill_kernel(uint32_t ne, const uint32
     }
 }
 
+// x on lane l, where l is the same on every lane of the wavefront: a scalar
+// read of one lane's register (v_readlane) instead of a cross-lane permute
+// through the LDS unit; the value is the one __shfl(x, l) returns.
+__device__ __forceinline__ uint32_t lane_u32(uint32_t x, uint32_t l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)l);
+}
+__device__ __forceinline__ double lane_f64(double x, uint32_t l) {
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    const uint64_t lo = lane_u32((uint32_t)b, l), hi = lane_u32((uint32_t)(b >> 32), l);
+    return __longlong_as_double((long long)(lo | (hi << 32)));
+}
+
 __device__ __forceinline__ void sort4(uint32_t* v) {
     uint32_t t;
     if (v[0] > v[1]) { t = v[0]; v[0] = v[1]; v[1] = t; }
@@ -148,7 +160,7 @@ __device__ __forceinline__ void tet_cols_row(uint32_t i, uint32_t b0, uint32_t m
         if (S * 64 >= nk) break;
         const uint32_t lim = nk - S * 64 < 64 ? nk - S * 64 : 64;
         for (uint32_t L = 0; L < lim; ++L) {
-            const uint32_t kP = (uint32_t)__shfl((int)key[S], (int)L, 64);
+            const uint32_t kP = lane_u32(key[S], L);
             const uint32_t P = S * 64 + L;
 #pragma unroll
             for (uint32_t sl = 0; sl < NS; ++sl)
@@ -166,8 +178,8 @@ __device__ __forceinline__ void tet_cols_row(uint32_t i, uint32_t b0, uint32_t m
         cnt += (uint32_t)__popcll(__ballot(first[S]));
         const uint32_t lim = nk - S * 64 < 64 ? nk - S * 64 : 64;
         for (uint32_t L = 0; L < lim; ++L) {
-            const uint32_t kP = (uint32_t)__shfl((int)key[S], (int)L, 64);
-            const bool fP = __shfl((int)first[S], (int)L, 64) != 0;
+            const uint32_t kP = lane_u32(key[S], L);
+            const bool fP = lane_u32(first[S] ? 1u : 0u, L) != 0u;
 #pragma unroll
             for (uint32_t sl = 0; sl < NS; ++sl) rank[sl] += (fP && kP < key[sl]) ? 1u : 0u;
         }
@@ -194,7 +206,7 @@ __global__ __launch_bounds__(256) void tet_cols_kernel(uint32_t n, const uint32_
         const uint32_t t = lane < m ? inc[b0 + lane] : ~0u;
         // rank of this lane's tetrahedron among the row's (ids are distinct)
         uint32_t rt = 0;
-        for (uint32_t l = 0; l < m; ++l) rt += (uint32_t)__shfl((int)t, (int)l, 64) < t ? 1u : 0u;
+        for (uint32_t l = 0; l < m; ++l) rt += lane_u32(t, l) < t ? 1u : 0u;
         // only the slots that hold keys are walked
         const uint32_t ns = (3 * m + 1 + 63) / 64;
         if (ns <= 1)
@@ -367,18 +379,18 @@ __global__ __launch_bounds__(256) void p1_tet_kernel(P1TetArgs a) {
         for (uint32_t t = 0; t < m; ++t) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                const uint32_t o = (uint32_t)__shfl((int)oth[q], (int)t, 64);
-                const double k = __shfl(kv[q], (int)t, 64);
+                const uint32_t o = lane_u32(oth[q], t);
+                const double k = lane_f64(kv[q], t);
                 if (o == cc) acc += k;
             }
-            mass += __shfl(mt, (int)t, 64);
+            mass += lane_f64(mt, t);
         }
         const bool bi = a.bnd[i] != 0;
         double diag = 0.0;
         uint32_t dpos = 0;
         for (uint32_t k = 0; k < cnt; ++k) {
-            const uint32_t ck = (uint32_t)__shfl((int)cc, (int)k, 64);
-            const double vk = __shfl(acc, (int)k, 64);
+            const uint32_t ck = lane_u32(cc, k);
+            const double vk = lane_f64(acc, k);
             if (ck == i)
                 dpos = k;
             else
