@@ -27,23 +27,81 @@ namespace stream_amd {
 
 namespace {
 
-__global__ __launch_bounds__(256) void tet_count_kernel(uint32_t ne, const uint32_t* __restrict__ elem,
-                                                        uint32_t* __restrict__ cnt) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ne) return;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) atomicAdd(&cnt[elem[4ull * t + k]], 1u);
+// Node -> tetrahedron incidence, counted and filled block by block: a
+// block's 256 tetrahedra (at most 1024 distinct vertices) are first gathered
+// in an LDS table of kIncSlots slots by open addressing (more slots than
+// keys, so a probe ends at the vertex's slot or at an empty one; vertex ids
+// are below n, never the empty marker), then each distinct vertex takes one
+// global atomic for the block: consecutive tetrahedra share most vertices.
+// The order of a node's list depends on the atomics' order; the column-set
+// kernel sorts every list by tetrahedron id.
+constexpr uint32_t kIncBlock = 256;
+constexpr uint32_t kIncSlots = 2048;
+static_assert(kIncSlots > 4 * kIncBlock, "the table must hold every vertex of a block with a slot to spare");
+
+__device__ __forceinline__ uint32_t inc_slot(uint32_t v, uint32_t* hkey) {
+    uint32_t h = (v * 2654435761u) >> 21;  // 11 bits: kIncSlots = 2048
+    for (;;) {
+        const uint32_t old = atomicCAS(&hkey[h], ~0u, v);
+        if (old == ~0u || old == v) return h;
+        h = (h + 1u) & (kIncSlots - 1u);
+    }
 }
 
-__global__ __launch_bounds__(256) void tet_fill_kernel(uint32_t ne, const uint32_t* __restrict__ elem,
-                                                       const uint32_t* __restrict__ toff, uint32_t* __restrict__ cur,
-                                                       uint32_t* __restrict__ inc) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ne) return;
+__device__ __forceinline__ void inc_table_clear(uint32_t* hkey, uint32_t* hcnt) {
+    for (uint32_t j = threadIdx.x; j < kIncSlots; j += kIncBlock) {
+        hkey[j] = ~0u;
+        hcnt[j] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(kIncBlock) void tet_count_kernel(uint32_t ne, const uint32_t* __restrict__ elem,
+                                                              uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t hkey[kIncSlots], hcnt[kIncSlots];
+    inc_table_clear(hkey, hcnt);
+    __syncthreads();
+    const uint32_t t = blockIdx.x * kIncBlock + threadIdx.x;
+    if (t < ne) {
+        const uint4 e = reinterpret_cast<const uint4*>(elem)[t];
+        const uint32_t v[4] = {e.x, e.y, e.z, e.w};
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t v = elem[4ull * t + k];
-        inc[toff[v] + atomicAdd(&cur[v], 1u)] = t;
+        for (int k = 0; k < 4; ++k) atomicAdd(&hcnt[inc_slot(v[k], hkey)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < kIncSlots; j += kIncBlock)
+        if (hkey[j] != ~0u) atomicAdd(&cnt[hkey[j]], hcnt[j]);
+}
+
+// Each distinct vertex of the block reserves its block count of places in
+// its list with one atomic; a reference takes the reserved run's start +
+// its rank among the block's references to that vertex.
+__global__ __launch_bounds__(kIncBlock) void tet_fill_kernel(uint32_t ne, const uint32_t* __restrict__ elem,
+                                                             const uint32_t* __restrict__ toff,
+                                                             uint32_t* __restrict__ cur, uint32_t* __restrict__ inc) {
+    __shared__ uint32_t hkey[kIncSlots], hcnt[kIncSlots];
+    inc_table_clear(hkey, hcnt);
+    __syncthreads();
+    const uint32_t t = blockIdx.x * kIncBlock + threadIdx.x;
+    const bool live = t < ne;
+    uint32_t h[4] = {0u, 0u, 0u, 0u}, r[4] = {0u, 0u, 0u, 0u};
+    if (live) {
+        const uint4 e = reinterpret_cast<const uint4*>(elem)[t];
+        const uint32_t v[4] = {e.x, e.y, e.z, e.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            h[k] = inc_slot(v[k], hkey);
+            r[k] = atomicAdd(&hcnt[h[k]], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < kIncSlots; j += kIncBlock) {
+        const uint32_t v = hkey[j];
+        if (v != ~0u) hcnt[j] = toff[v] + atomicAdd(&cur[v], hcnt[j]);  // the run's start replaces the count
+    }
+    __syncthreads();
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) inc[hcnt[h[k]] + r[k]] = t;
     }
 }
 
@@ -512,13 +570,15 @@ struct Step3D {
         scan_ws.resize(scan_workspace_words(n + 1));
         fill<uint32_t>(tcnt.p, n + 1, 0u, s);
         fill<uint32_t>(tcur.p, n + 1, 0u, s);
-        if (ne) hipLaunchKernelGGL(tet_count_kernel, dim3(grid_for(ne, 256)), dim3(256), 0, s, ne, mesh.elemglob.p, tcnt.p);
+        if (ne)
+            hipLaunchKernelGGL(tet_count_kernel, dim3((ne + kIncBlock - 1) / kIncBlock), dim3(kIncBlock), 0, s, ne,
+                               mesh.elemglob.p, tcnt.p);
         exclusive_scan_u32(tcnt.p, toff.p, n, scan_ws.p, s);
         inc.resize(4ull * ne + 1);
         tmp.resize(12ull * ne + n + 1);
         if (ne)
-            hipLaunchKernelGGL(tet_fill_kernel, dim3(grid_for(ne, 256)), dim3(256), 0, s, ne, mesh.elemglob.p, toff.p,
-                               tcur.p, inc.p);
+            hipLaunchKernelGGL(tet_fill_kernel, dim3((ne + kIncBlock - 1) / kIncBlock), dim3(kIncBlock), 0, s, ne,
+                               mesh.elemglob.p, toff.p, tcur.p, inc.p);
         hipLaunchKernelGGL(tet_cols_kernel, dim3(grid_for(n, 4)), dim3(256), 0, s, n, mesh.elemglob.p, toff.p,
                            inc.p, tmp.p, ncol.p);
         exclusive_scan_u32(ncol.p, A.row.p, n, scan_ws.p, s);
